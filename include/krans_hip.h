@@ -1,0 +1,171 @@
+/*
+ * krans_hip.h - C ABI of the KraNS-MI null-space engine (libkrans_hip.so).
+ *
+ * The reference (Deep-SANE/krans) exposes no C ABI: its boundary is the C++
+ * plugin API of include/krans/{MatrixInterface,VectorInterface,
+ * VectorFactory}.hpp and the two solver entry points. This header is the
+ * flat layer under the C++ adapters of include/krans/adapters/hip/, and what
+ * a ctypes / FFI binding calls. Every function returns a status code
+ * (KH_OK = 0); kh_last_error() gives the thread's last message. The codes map
+ * to the reference's exception types: KH_EINVAL <-> std::invalid_argument,
+ * KH_ERUNTIME <-> std::runtime_error, KH_ENOMEM <-> std::bad_alloc.
+ *
+ * Threading: one context per device; handles are not thread-safe, calls on
+ * one context are serialised by the caller (the reference is single-threaded:
+ * its solvers take const refs and never synchronise).
+ */
+#ifndef KRANS_HIP_H
+#define KRANS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KH_OK 0
+#define KH_EINVAL 1
+#define KH_ERUNTIME 2
+#define KH_ENOMEM 3
+#define KH_EDEVICE 4
+
+/* precision of a solve: FP64 everywhere, or FP32 Krylov/LOBPCG blocks with
+ * FP64 inner products, projected problems and polish residuals */
+#define KH_PREC_FP64 0
+#define KH_PREC_MIXED 1
+
+typedef struct kh_context_s kh_context;
+typedef struct kh_matrix_s kh_matrix;
+typedef struct kh_rng_s kh_rng;
+
+/* Field for field krans::solvers::CyclicLanczosOptions
+ * (include/krans/Solvers/CyclicLanczosSolver.hpp:84-146); the two non-POD
+ * members are replaced by `y0` (initial_right_space[0]) in
+ * kh_cyclic_lanczos and by `precond_jacobi` (M^{-1} = diag(A A^T)^{-1}). */
+typedef struct kh_cyclic_options {
+  int64_t k, ncv, max_restarts;
+  double tol_abs, tol_rel;
+  int32_t reorth;       /* 0 none, 1 single, 2 double (ReorthMode) */
+  int32_t use_harmonic; /* refined extraction when nonzero */
+  double target_tau;
+  int32_t nullity_known, rank_known;
+  int64_t cg_maxit;
+  double cg_rel_tol, cg_breakdown_tol;
+  int32_t projector_passes, grad_max_iters, frob_samples;
+  double prescreen_factor;
+  int64_t stagnation_restarts;
+  int32_t return_left_space, verbose;
+  int32_t precond_jacobi, reserved;
+} kh_cyclic_options;
+
+/* krans::solvers::CyclicLanczosResult (CyclicLanczosSolver.hpp:149-167) plus
+ * operator-application counts and phase times. */
+typedef struct kh_cyclic_result {
+  int64_t n_right, n_left, restarts, krylov_steps_total;
+  double frob_norm_est;
+  int64_t converged_to_target, cg_solves, cg_maxit_hits, spmv, spmtv, cg_iters;
+  double t_gk, t_tiny, t_cand, t_polish, t_fallback, t_rng;
+} kh_cyclic_result;
+
+/* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and
+ * initial_block are replaced by `initial_block` in kh_lobpcg. */
+typedef struct kh_lobpcg_options {
+  int64_t nev, block_size, max_iters;
+  double tol_abs, tol_rel;
+  int32_t soft_locking, gs_reorth;
+  double restart, gs_tol;
+  int32_t frob_samples, nullity_known, rank_known, verbose;
+  int64_t max_tiny_dim, progress_every;
+} kh_lobpcg_options;
+
+/* krans::solvers::LOBPCGResult (LOBPCGSolver.hpp:117-139). */
+typedef struct kh_lobpcg_result {
+  int64_t n_right, iters, locked;
+  double frob_norm_est, accept_threshold, best_unaccepted_residual;
+  int64_t block_size, converged_to_target, spmv, spmtv;
+  double t_gram, t_sparse, t_host;
+} kh_lobpcg_result;
+
+const char *kh_last_error(void);
+int kh_version(void);
+int kh_device_count(int *n);
+
+/* --- context ------------------------------------------------------------ */
+int kh_context_create(int device, kh_context **out);
+int kh_context_destroy(kh_context *ctx);
+int kh_context_sync(kh_context *ctx);
+
+/* --- matrices ------------------------------------------------------------ */
+/* Canonical CSR on the host (rowptr[0]=0, columns strictly ascending per
+ * row, int32 indices); the device copy gets a CSC twin built from it.
+ * Replaces EigenSparseMatrixAdapter(const SparseMat&)
+ * (include/krans/adapters/eigen/EigenSparseMatrixAdapter.hpp:20-24). */
+int kh_matrix_create_csr(kh_context *ctx, int64_t m, int64_t n,
+                         const int32_t *rowptr, const int32_t *col,
+                         const double *val, kh_matrix **out);
+int kh_matrix_destroy(kh_matrix *A);
+/* rows(), cols() (MatrixInterface.hpp:20-23), nnz and exact ||A||_F */
+int kh_matrix_info(const kh_matrix *A, int64_t *m, int64_t *n, int64_t *nnz,
+                   double *fro);
+/* Download the device structures (bit-exact parity checks). */
+int kh_matrix_download_csr(const kh_matrix *A, int32_t *rowptr, int32_t *col,
+                           double *val);
+int kh_matrix_download_csc(const kh_matrix *A, int32_t *colptr,
+                           int32_t *rowidx, double *val, int32_t *perm);
+/* y = A x or y = A^T x on host vectors (MatrixInterface::multiply /
+ * multiply_transpose, MatrixInterface.hpp:14-19); precision 0/1 = FP64/FP32. */
+int kh_matrix_multiply(kh_matrix *A, int transpose, int precision,
+                       const double *x, double *y);
+
+/* --- reference-compatible random stream (StdVectorFactory.hpp:16-58) ---- */
+int kh_rng_create(uint32_t seed, kh_rng **out);
+int kh_rng_destroy(kh_rng *r);
+int kh_rng_normal(kh_rng *r, int64_t n, double *out);
+
+/* --- solvers --------------------------------------------------------------
+ * find_nullspace_cyclic_lanczos_ex (CyclicLanczosSolver.hpp:170-178).
+ * Bases are written column-major: right (n x right_cap), left (m x
+ * left_cap); counts in the result. `y0` may be NULL. */
+int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
+                      int precision, const double *y0, double *right,
+                      int64_t right_cap, double *left, int64_t left_cap,
+                      kh_cyclic_result *res);
+/* find_nullspace_lobpcg_ex (LOBPCGSolver.hpp:142-149); theta and
+ * residuals_Av per accepted vector; initial_block n x n_initial or NULL. */
+int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt,
+              int precision, const double *initial_block, int64_t n_initial,
+              double *right, int64_t right_cap, double *theta,
+              double *residuals, kh_lobpcg_result *res);
+
+/* --- device buffers and kernel entry points (tests, benchmarks) ---------- */
+int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out);
+int kh_dev_free(kh_context *ctx, void *p);
+int kh_h2d(kh_context *ctx, void *dst, const void *src, size_t bytes);
+int kh_d2h(kh_context *ctx, void *dst, const void *src, size_t bytes);
+/* y = op(A) x on device vectors, s columns (ld = vector length) */
+int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s,
+                const void *x, void *y);
+/* out = a.b (FP64 accumulate); precision of a, b */
+int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b,
+               int64_t len, double *out);
+/* Two classical Gram-Schmidt passes of w against the s columns of V
+ * (len x s, column-major): the reorthogonalisation of
+ * CyclicLanczosSolver.cpp:399-407 in CGS2 form. h_out gets the summed
+ * coefficients of both passes, norm_out ||w|| afterwards. */
+int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s,
+                void *w, int64_t len, double *h_out, double *norm_out);
+/* G = Z^T [Z AZ]: the S and G Gram blocks of LOBPCGSolver.cpp:67-80 on
+ * FP32 MFMA (precision 1) or FP64 (precision 0); Z, AZ len x k column-major,
+ * S and G k x k FP64 out (host). */
+int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ,
+                int k, int64_t len, double *S, double *G);
+/* Average device time (ms) of `reps` launches of y = op(A) x, measured with
+ * HIP events on the library stream. */
+int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps,
+                 double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KRANS_HIP_H */

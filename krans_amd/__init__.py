@@ -1,0 +1,476 @@
+"""KraNS-MI: MI355X-native null-space engine (Python host binding).
+
+Mirrors the reference's plugin API for the null-space hot path:
+
+* ``HipCsrMatrixAdapter``  -- MatrixInterface over a device CSR with CSC twin
+  (replaces EigenSparseMatrixAdapter, include/krans/adapters/eigen/
+  EigenSparseMatrixAdapter.hpp:12-68);
+* ``HipVectorFactory``     -- VectorFactory with the reference's seeded
+  std::mt19937 + normal_distribution stream (StdVectorFactory.hpp:16-58);
+* ``CyclicLanczosOptions`` / ``LOBPCGOptions`` with the reference's fields and
+  defaults (CyclicLanczosSolver.hpp:84-146, LOBPCGSolver.hpp:71-115);
+* ``find_nullspace_cyclic_lanczos[_ex]`` / ``find_nullspace_lobpcg[_ex]``.
+
+Everything calls libkrans_hip.so through its C ABI (include/krans_hip.h).
+There is no CPU fallback: importing the solvers without the built library
+raises.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field, fields
+from typing import List, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libkrans_hip.so")
+
+KH_OK, KH_EINVAL, KH_ERUNTIME, KH_ENOMEM, KH_EDEVICE = 0, 1, 2, 3, 4
+PREC = {"fp64": 0, "mixed": 1}
+
+
+class _CyclicOpts(C.Structure):
+    _fields_ = [
+        ("k", C.c_int64), ("ncv", C.c_int64), ("max_restarts", C.c_int64),
+        ("tol_abs", C.c_double), ("tol_rel", C.c_double),
+        ("reorth", C.c_int32), ("use_harmonic", C.c_int32),
+        ("target_tau", C.c_double),
+        ("nullity_known", C.c_int32), ("rank_known", C.c_int32),
+        ("cg_maxit", C.c_int64),
+        ("cg_rel_tol", C.c_double), ("cg_breakdown_tol", C.c_double),
+        ("projector_passes", C.c_int32), ("grad_max_iters", C.c_int32),
+        ("frob_samples", C.c_int32),
+        ("prescreen_factor", C.c_double),
+        ("stagnation_restarts", C.c_int64),
+        ("return_left_space", C.c_int32), ("verbose", C.c_int32),
+        ("precond_jacobi", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class _CyclicRes(C.Structure):
+    _fields_ = [
+        ("n_right", C.c_int64), ("n_left", C.c_int64), ("restarts", C.c_int64),
+        ("krylov_steps_total", C.c_int64), ("frob_norm_est", C.c_double),
+        ("converged_to_target", C.c_int64), ("cg_solves", C.c_int64),
+        ("cg_maxit_hits", C.c_int64), ("spmv", C.c_int64), ("spmtv", C.c_int64),
+        ("cg_iters", C.c_int64),
+        ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
+        ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
+    ]
+
+
+class _LobpcgOpts(C.Structure):
+    _fields_ = [
+        ("nev", C.c_int64), ("block_size", C.c_int64), ("max_iters", C.c_int64),
+        ("tol_abs", C.c_double), ("tol_rel", C.c_double),
+        ("soft_locking", C.c_int32), ("gs_reorth", C.c_int32),
+        ("restart", C.c_double), ("gs_tol", C.c_double),
+        ("frob_samples", C.c_int32), ("nullity_known", C.c_int32),
+        ("rank_known", C.c_int32), ("verbose", C.c_int32),
+        ("max_tiny_dim", C.c_int64), ("progress_every", C.c_int64),
+    ]
+
+
+class _LobpcgRes(C.Structure):
+    _fields_ = [
+        ("n_right", C.c_int64), ("iters", C.c_int64), ("locked", C.c_int64),
+        ("frob_norm_est", C.c_double), ("accept_threshold", C.c_double),
+        ("best_unaccepted_residual", C.c_double), ("block_size", C.c_int64),
+        ("converged_to_target", C.c_int64), ("spmv", C.c_int64), ("spmtv", C.c_int64),
+        ("t_gram", C.c_double), ("t_sparse", C.c_double), ("t_host", C.c_double),
+    ]
+
+
+# Every exported symbol of include/krans_hip.h with its ctypes signature.
+P, I32, I64, D, SZ = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
+SIGNATURES = {
+    "kh_last_error": (C.c_char_p, []),
+    "kh_version": (I32, []),
+    "kh_device_count": (I32, [C.POINTER(C.c_int)]),
+    "kh_context_create": (I32, [I32, C.POINTER(P)]),
+    "kh_context_destroy": (I32, [P]),
+    "kh_context_sync": (I32, [P]),
+    "kh_matrix_create_csr": (I32, [P, I64, I64, P, P, P, C.POINTER(P)]),
+    "kh_matrix_destroy": (I32, [P]),
+    "kh_matrix_info": (I32, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), C.POINTER(D)]),
+    "kh_matrix_download_csr": (I32, [P, P, P, P]),
+    "kh_matrix_download_csc": (I32, [P, P, P, P, P]),
+    "kh_matrix_multiply": (I32, [P, I32, I32, P, P]),
+    "kh_rng_create": (I32, [C.c_uint32, C.POINTER(P)]),
+    "kh_rng_destroy": (I32, [P]),
+    "kh_rng_normal": (I32, [P, I64, P]),
+    "kh_cyclic_lanczos": (I32, [P, P, C.POINTER(_CyclicOpts), I32, P, P, I64, P, I64,
+                                C.POINTER(_CyclicRes)]),
+    "kh_lobpcg": (I32, [P, P, C.POINTER(_LobpcgOpts), I32, P, I64, P, I64, P, P,
+                        C.POINTER(_LobpcgRes)]),
+    "kh_dev_alloc": (I32, [P, SZ, C.POINTER(P)]),
+    "kh_dev_free": (I32, [P, P]),
+    "kh_h2d": (I32, [P, P, P, SZ]),
+    "kh_d2h": (I32, [P, P, P, SZ]),
+    "kh_spmm_dev": (I32, [P, I32, I32, I32, P, P]),
+    "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
+    "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
+    "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
+    "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libkrans_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libkrans_hip.so not built ({LIB_PATH}); run python -m krans_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class KransError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc == KH_OK:
+        return
+    msg = lib().kh_last_error().decode(errors="replace")
+    if rc == KH_EINVAL:
+        raise ValueError(msg)
+    if rc == KH_ENOMEM:
+        raise MemoryError(msg)
+    raise KransError(msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class HipContext:
+    """One device, one stream (kh_context)."""
+
+    def __init__(self, device=0):
+        h = P()
+        _check(lib().kh_context_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().kh_context_sync(self.h))
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = HipContext(0)
+    return _default_ctx
+
+
+class HipCsrMatrixAdapter:
+    """MatrixInterface over a device CSR matrix with its CSC twin."""
+
+    def __init__(self, m, n, rowptr, col, val, ctx=None):
+        self.ctx = ctx or default_context()
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if rowptr.shape[0] != m + 1:
+            raise ValueError("rowptr must have m+1 entries")
+        h = P()
+        _check(lib().kh_matrix_create_csr(self.ctx.h, m, n, _ptr(rowptr), _ptr(col),
+                                          _ptr(val), C.byref(h)))
+        self.h = h
+        mm, nn, nz, fr = I64(), I64(), I64(), D()
+        _check(lib().kh_matrix_info(h, C.byref(mm), C.byref(nn), C.byref(nz), C.byref(fr)))
+        self.m, self.n, self.nnz, self.fro = mm.value, nn.value, nz.value, fr.value
+
+    @classmethod
+    def from_csr(cls, A, ctx=None):
+        """From any object with m, n, rowptr, col, val (e.g. workloads.CsrMatrix)."""
+        return cls(A.m, A.n, A.rowptr, A.col, A.val, ctx=ctx)
+
+    def rows(self):
+        return self.m
+
+    def cols(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kh_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def multiply(self, x, precision="fp64"):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.n,):
+            raise ValueError("HipCsrMatrixAdapter::multiply size mismatch")
+        y = np.empty(self.m)
+        _check(lib().kh_matrix_multiply(self.h, 0, PREC[precision], _ptr(x), _ptr(y)))
+        return y
+
+    def multiply_transpose(self, x, precision="fp64"):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.m,):
+            raise ValueError("HipCsrMatrixAdapter::multiply_transpose size mismatch")
+        y = np.empty(self.n)
+        _check(lib().kh_matrix_multiply(self.h, 1, PREC[precision], _ptr(x), _ptr(y)))
+        return y
+
+    def download_csr(self):
+        rp = np.empty(self.m + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        va = np.empty(self.nnz, np.float64)
+        _check(lib().kh_matrix_download_csr(self.h, _ptr(rp), _ptr(ci), _ptr(va)))
+        return rp, ci, va
+
+    def download_csc(self):
+        cp = np.empty(self.n + 1, np.int32)
+        ri = np.empty(self.nnz, np.int32)
+        vt = np.empty(self.nnz, np.float64)
+        perm = np.empty(self.nnz, np.int32)
+        _check(lib().kh_matrix_download_csc(self.h, _ptr(cp), _ptr(ri), _ptr(vt), _ptr(perm)))
+        return cp, ri, vt, perm
+
+    def time_spmv(self, transpose=False, precision="fp64", reps=50):
+        out = D()
+        _check(lib().kh_time_spmv(self.h, int(transpose), PREC[precision], reps, C.byref(out)))
+        return out.value
+
+
+class HipVectorFactory:
+    """VectorFactory: the reference's seeded N(0,1) stream; vectors of the
+    solvers live on the device, this object owns the random stream."""
+
+    def __init__(self, default_size, seed=0):
+        if seed == 0:
+            seed = int.from_bytes(os.urandom(4), "little") or 1
+        self.default_size = default_size
+        self.seed = seed
+        h = P()
+        _check(lib().kh_rng_create(seed, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().kh_rng_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def create_random_of_size(self, n):
+        out = np.empty(n)
+        _check(lib().kh_rng_normal(self.h, n, _ptr(out)))
+        return out
+
+    def create_random(self):
+        return self.create_random_of_size(self.default_size)
+
+    def create_zero_of_size(self, n):
+        return np.zeros(n)
+
+    def create_zero(self):
+        return np.zeros(self.default_size)
+
+
+@dataclass
+class CyclicLanczosOptions:
+    """CyclicLanczosSolver.hpp:84-146 (same names and defaults)."""
+    k: int = 1
+    ncv: int = 48
+    max_restarts: int = 80
+    tol_abs: float = 1e-12
+    tol_rel: float = 1e-10
+    reorth: int = 2  # ReorthMode: 0 None, 1 Single, 2 Double
+    use_harmonic: bool = True
+    target_tau: float = 0.0
+    nullity_known: int = -1
+    rank_known: int = -1
+    initial_right_space: List[np.ndarray] = field(default_factory=list)
+    precond_AAt: Optional[str] = None  # None or "jacobi" (diag(A A^T)^-1)
+    cg_maxit: int = 80
+    cg_rel_tol: float = 1e-10
+    cg_breakdown_tol: float = 1e-30
+    projector_passes: int = 1
+    grad_max_iters: int = 3000
+    frob_samples: int = 4
+    prescreen_factor: float = 0.8
+    stagnation_restarts: int = 30
+    return_left_space: bool = False
+    verbose: bool = False
+
+    def _c(self):
+        o = _CyclicOpts()
+        for f in fields(self):
+            if f.name in ("initial_right_space", "precond_AAt"):
+                continue
+            setattr(o, f.name, int(getattr(self, f.name)) if isinstance(getattr(self, f.name), bool)
+                    else getattr(self, f.name))
+        if self.precond_AAt not in (None, "jacobi"):
+            raise ValueError("precond_AAt must be None or 'jacobi' on the HIP backend")
+        o.precond_jacobi = 1 if self.precond_AAt == "jacobi" else 0
+        return o
+
+
+@dataclass
+class CyclicLanczosResult:
+    """CyclicLanczosSolver.hpp:149-167 plus counters and phase times."""
+    right_null: np.ndarray
+    left_null: np.ndarray
+    restarts: int
+    krylov_steps_total: int
+    frob_norm_est: float
+    converged_to_target: bool
+    cg_solves: int
+    cg_maxit_hits: int
+    spmv: int = 0
+    spmtv: int = 0
+    cg_iters: int = 0
+    times: dict = field(default_factory=dict)
+
+
+def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
+    """find_nullspace_cyclic_lanczos_ex (CyclicLanczosSolver.hpp:170-174).
+
+    Returns right_null as an (n x k) array of orthonormal columns.
+    """
+    if not isinstance(A, HipCsrMatrixAdapter):
+        raise ValueError("find_nullspace_cyclic_lanczos: backend mismatch (HIP matrix required)")
+    o = opt._c()
+    kcap = max(int(opt.k), int(opt.nullity_known), 1)
+    R = np.zeros((kcap, A.n))
+    Lb = np.zeros((kcap, A.m)) if opt.return_left_space else None
+    y0 = None
+    if opt.initial_right_space:
+        y0 = np.ascontiguousarray(opt.initial_right_space[0], dtype=np.float64)
+        if y0.shape != (A.n,):
+            raise ValueError("initial_right_space[0] size mismatch")
+    res = _CyclicRes()
+    _check(lib().kh_cyclic_lanczos(A.h, vf.h, C.byref(o), PREC[precision],
+                                   None if y0 is None else _ptr(y0), _ptr(R), kcap,
+                                   None if Lb is None else _ptr(Lb), kcap, C.byref(res)))
+    times = {k: getattr(res, "t_" + k) for k in ("gk", "tiny", "cand", "polish", "fallback", "rng")}
+    return CyclicLanczosResult(
+        right_null=R[:res.n_right].T.copy(),
+        left_null=(Lb[:res.n_left].T.copy() if Lb is not None else np.zeros((A.m, 0))),
+        restarts=res.restarts, krylov_steps_total=res.krylov_steps_total,
+        frob_norm_est=res.frob_norm_est, converged_to_target=bool(res.converged_to_target),
+        cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
+        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times)
+
+
+def find_nullspace_cyclic_lanczos(A, vf, opt, precision="fp64"):
+    """Backwards-compatible wrapper (CyclicLanczosSolver.hpp:176-178)."""
+    return find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision).right_null
+
+
+@dataclass
+class LOBPCGOptions:
+    """LOBPCGSolver.hpp:71-115 (same names and defaults)."""
+    nev: int = 1
+    block_size: int = 0
+    max_iters: int = 1000
+    tol_abs: float = 1e-12
+    tol_rel: float = 1e-10
+    soft_locking: bool = True
+    restart: float = 0.9
+    gs_tol: float = 1e-14
+    gs_reorth: bool = True
+    M_inv: Optional[str] = None
+    initial_block: List[np.ndarray] = field(default_factory=list)
+    frob_samples: int = 4
+    max_tiny_dim: int = 0
+    nullity_known: int = -1
+    rank_known: int = -1
+    verbose: bool = False
+    progress_every: int = 100
+
+    def _c(self):
+        if self.M_inv is not None:
+            raise ValueError("M_inv is not supported on the HIP backend")
+        o = _LobpcgOpts()
+        for f in fields(self):
+            if f.name in ("M_inv", "initial_block"):
+                continue
+            v = getattr(self, f.name)
+            setattr(o, f.name, int(v) if isinstance(v, bool) else v)
+        return o
+
+
+@dataclass
+class LOBPCGResult:
+    """LOBPCGSolver.hpp:117-139."""
+    right_null: np.ndarray
+    theta: np.ndarray
+    residuals_Av: np.ndarray
+    iters: int
+    locked: int
+    frob_norm_est: float
+    accept_threshold: float
+    best_unaccepted_residual: float
+    block_size: int
+    converged_to_target: bool
+    spmv: int = 0
+    spmtv: int = 0
+    times: dict = field(default_factory=dict)
+
+
+def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
+    """find_nullspace_lobpcg_ex (LOBPCGSolver.hpp:142-144)."""
+    if not isinstance(A, HipCsrMatrixAdapter):
+        raise ValueError("find_nullspace_lobpcg: backend mismatch (HIP matrix required)")
+    o = opt._c()
+    kcap = max(int(opt.nev), int(opt.nullity_known), 1)
+    if opt.rank_known >= 0:
+        kcap = max(kcap, A.n - int(opt.rank_known))
+    kcap = max(1, min(kcap, A.n))
+    R = np.zeros((kcap, A.n))
+    th = np.zeros(kcap)
+    rs = np.zeros(kcap)
+    ib = None
+    if opt.initial_block:
+        ib = np.ascontiguousarray(np.stack(opt.initial_block, axis=0), dtype=np.float64)
+        if ib.shape[1] != A.n:
+            raise ValueError("initial_block size mismatch")
+    res = _LobpcgRes()
+    _check(lib().kh_lobpcg(A.h, vf.h, C.byref(o), PREC[precision],
+                           None if ib is None else _ptr(ib), 0 if ib is None else ib.shape[0],
+                           _ptr(R), kcap, _ptr(th), _ptr(rs), C.byref(res)))
+    n = res.n_right
+    return LOBPCGResult(
+        right_null=R[:n].T.copy(), theta=th[:n].copy(), residuals_Av=rs[:n].copy(),
+        iters=res.iters, locked=res.locked, frob_norm_est=res.frob_norm_est,
+        accept_threshold=res.accept_threshold,
+        best_unaccepted_residual=res.best_unaccepted_residual, block_size=res.block_size,
+        converged_to_target=bool(res.converged_to_target), spmv=res.spmv, spmtv=res.spmtv,
+        times={"gram": res.t_gram, "sparse": res.t_sparse, "host": res.t_host})
+
+
+def find_nullspace_lobpcg(A, vf, opt, precision="fp64"):
+    """Backwards-compatible wrapper (LOBPCGSolver.hpp:147-149)."""
+    return find_nullspace_lobpcg_ex(A, vf, opt, precision).right_null

@@ -1,0 +1,56 @@
+"""Build libkrans_hip.so (all HIP/C++ sources of krans_amd/csrc) for gfx950.
+
+Usage: python -m krans_amd.build  (or krans_amd.build.build()).
+Objects and the library go to krans_amd/_build/ (in-tree, git-ignored).
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libkrans_hip.so")
+ARCH = os.environ.get("KRANS_HIP_ARCH", "gfx950")
+
+CXXFLAGS = [
+    "-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result",
+    f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off (hipcc only; the host pass warns)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def sources():
+    return sorted(f for f in os.listdir(SRC) if f.endswith((".hip", ".cpp")))
+
+
+def _compile(f):
+    src = os.path.join(SRC, f)
+    obj = os.path.join(OUT, f + ".o")
+    cmd = ["hipcc", *CXXFLAGS, "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(OUT, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(_compile, sources()))
+    tmp = LIB + ".tmp"
+    cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,486 @@
+// C ABI of libkrans_hip.so (declared in include/krans_hip.h).
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "../../include/krans_hip.h"
+#include "dense.hpp"
+#include "solver.hpp"
+
+struct kh_context_s {
+  std::unique_ptr<kh::Context> ctx;
+};
+struct kh_matrix_s {
+  kh_context *owner = nullptr;
+  std::unique_ptr<kh::DeviceCsr> A;
+};
+struct kh_rng_s {
+  kh::RefStream s;
+  explicit kh_rng_s(unsigned seed) : s(seed) {}
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F> int guard(F &&fn) {
+  try {
+    fn();
+    return KH_OK;
+  } catch (const std::invalid_argument &e) {
+    g_err = e.what();
+    return KH_EINVAL;
+  } catch (const kh::DeviceOOM &e) {
+    g_err = e.what();
+    return KH_ENOMEM;
+  } catch (const std::bad_alloc &e) {
+    g_err = e.what();
+    return KH_ENOMEM;
+  } catch (const kh::DeviceError &e) {
+    g_err = e.what();
+    return KH_EDEVICE;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return KH_ERUNTIME;
+  } catch (...) {
+    g_err = "unknown error";
+    return KH_ERUNTIME;
+  }
+}
+
+void need(bool cond, const char *msg) {
+  if (!cond)
+    throw std::invalid_argument(msg);
+}
+
+kh::Prec prec_of(int p) {
+  if (p == KH_PREC_FP64)
+    return kh::Prec::F64;
+  if (p == KH_PREC_MIXED)
+    return kh::Prec::F32;
+  throw std::invalid_argument("precision must be KH_PREC_FP64 or KH_PREC_MIXED");
+}
+
+kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
+  kh::CyclicParams P;
+  P.k = o.k;
+  P.ncv = o.ncv;
+  P.max_restarts = o.max_restarts;
+  P.tol_abs = o.tol_abs;
+  P.tol_rel = o.tol_rel;
+  need(o.reorth >= 0 && o.reorth <= 2, "reorth must be 0, 1 or 2");
+  P.reorth = o.reorth;
+  P.use_harmonic = o.use_harmonic != 0;
+  P.target_tau = o.target_tau;
+  P.nullity_known = o.nullity_known;
+  P.rank_known = o.rank_known;
+  P.cg_maxit = o.cg_maxit;
+  P.cg_rel_tol = o.cg_rel_tol;
+  P.cg_breakdown_tol = o.cg_breakdown_tol;
+  P.projector_passes = o.projector_passes;
+  P.grad_max_iters = o.grad_max_iters;
+  P.frob_samples = o.frob_samples;
+  P.prescreen_factor = o.prescreen_factor;
+  P.stagnation_restarts = o.stagnation_restarts;
+  P.return_left_space = o.return_left_space != 0;
+  P.verbose = o.verbose != 0;
+  P.jacobi_AAt = o.precond_jacobi != 0;
+  P.krylov = prec_of(precision);
+  P.polish = P.krylov;
+  need(P.k >= 0 && P.ncv >= 0 && P.max_restarts >= 0 && P.cg_maxit >= 0,
+       "negative size option");
+  return P;
+}
+
+void put_basis(kh::Context &ctx, const kh::DeviceBasis &B, double *out, int64_t cap) {
+  if (!out)
+    return;
+  const int64_t k = std::min(B.count, cap);
+  if (k > 0)
+    KH_CHECK(hipMemcpyAsync(out, B.buf.p, (std::size_t)(k * B.len) * 8,
+                            hipMemcpyDeviceToHost, ctx.stream));
+  ctx.sync();
+}
+}  // namespace
+
+extern "C" {
+
+const char *kh_last_error(void) { return g_err.c_str(); }
+int kh_version(void) { return 1; }
+
+int kh_device_count(int *n) {
+  return guard([&] {
+    need(n, "null out");
+    KH_CHECK(hipGetDeviceCount(n));
+  });
+}
+
+int kh_context_create(int device, kh_context **out) {
+  return guard([&] {
+    need(out, "null out");
+    auto c = std::make_unique<kh_context_s>();
+    c->ctx = std::make_unique<kh::Context>(device);
+    *out = c.release();
+  });
+}
+
+int kh_context_destroy(kh_context *ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int kh_context_sync(kh_context *ctx) {
+  return guard([&] {
+    need(ctx, "null context");
+    ctx->ctx->sync();
+  });
+}
+
+int kh_matrix_create_csr(kh_context *ctx, int64_t m, int64_t n, const int32_t *rowptr,
+                         const int32_t *col, const double *val, kh_matrix **out) {
+  return guard([&] {
+    need(ctx && out, "null argument");
+    auto M = std::make_unique<kh_matrix_s>();
+    M->owner = ctx;
+    M->A.reset(kh::csr_create(*ctx->ctx, m, n, rowptr, col, val));
+    *out = M.release();
+  });
+}
+
+int kh_matrix_destroy(kh_matrix *A) {
+  return guard([&] { delete A; });
+}
+
+int kh_matrix_info(const kh_matrix *A, int64_t *m, int64_t *n, int64_t *nnz, double *fro) {
+  return guard([&] {
+    need(A, "null matrix");
+    if (m)
+      *m = A->A->m;
+    if (n)
+      *n = A->A->n;
+    if (nnz)
+      *nnz = A->A->nnz;
+    if (fro)
+      *fro = A->A->fro;
+  });
+}
+
+int kh_matrix_download_csr(const kh_matrix *A, int32_t *rowptr, int32_t *col, double *val) {
+  return guard([&] {
+    need(A, "null matrix");
+    auto &c = *A->owner->ctx;
+    const auto &D = *A->A;
+    if (rowptr)
+      KH_CHECK(hipMemcpyAsync(rowptr, D.rp.p, (std::size_t)(D.m + 1) * 4,
+                              hipMemcpyDeviceToHost, c.stream));
+    if (col && D.nnz)
+      KH_CHECK(hipMemcpyAsync(col, D.ci.p, (std::size_t)D.nnz * 4, hipMemcpyDeviceToHost,
+                              c.stream));
+    if (val && D.nnz)
+      KH_CHECK(hipMemcpyAsync(val, D.va.p, (std::size_t)D.nnz * 8, hipMemcpyDeviceToHost,
+                              c.stream));
+    c.sync();
+  });
+}
+
+int kh_matrix_download_csc(const kh_matrix *A, int32_t *colptr, int32_t *rowidx, double *val,
+                           int32_t *perm) {
+  return guard([&] {
+    need(A, "null matrix");
+    auto &c = *A->owner->ctx;
+    const auto &D = *A->A;
+    if (colptr)
+      KH_CHECK(hipMemcpyAsync(colptr, D.cp.p, (std::size_t)(D.n + 1) * 4,
+                              hipMemcpyDeviceToHost, c.stream));
+    if (rowidx && D.nnz)
+      KH_CHECK(hipMemcpyAsync(rowidx, D.ri.p, (std::size_t)D.nnz * 4, hipMemcpyDeviceToHost,
+                              c.stream));
+    if (val && D.nnz)
+      KH_CHECK(hipMemcpyAsync(val, D.vt.p, (std::size_t)D.nnz * 8, hipMemcpyDeviceToHost,
+                              c.stream));
+    if (perm && D.nnz)
+      KH_CHECK(hipMemcpyAsync(perm, D.perm.p, (std::size_t)D.nnz * 4, hipMemcpyDeviceToHost,
+                              c.stream));
+    c.sync();
+  });
+}
+
+int kh_matrix_multiply(kh_matrix *A, int transpose, int precision, const double *x,
+                       double *y) {
+  return guard([&] {
+    need(A && x && y, "null argument");
+    auto &c = *A->owner->ctx;
+    const auto &D = *A->A;
+    const kh::Prec p = prec_of(precision);
+    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    kh::DBuf dx((std::size_t)std::max<int64_t>(1, nin) * 8),
+        dy((std::size_t)std::max<int64_t>(1, nout) * 8);
+    KH_CHECK(hipMemcpyAsync(dx.p, x, (std::size_t)nin * 8, hipMemcpyHostToDevice, c.stream));
+    if (p == kh::Prec::F64) {
+      kh::spmv(c, D, transpose != 0, p, dx.p, dy.p);
+    } else {
+      kh::DBuf x32((std::size_t)std::max<int64_t>(1, nin) * 4),
+          y32((std::size_t)std::max<int64_t>(1, nout) * 4);
+      kh::convert(c, kh::Prec::F64, dx.p, kh::Prec::F32, x32.p, nin);
+      kh::spmv(c, D, transpose != 0, p, x32.p, y32.p);
+      kh::convert(c, kh::Prec::F32, y32.p, kh::Prec::F64, dy.p, nout);
+      c.sync();
+    }
+    KH_CHECK(hipMemcpyAsync(y, dy.p, (std::size_t)nout * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int kh_rng_create(uint32_t seed, kh_rng **out) {
+  return guard([&] {
+    need(out, "null out");
+    *out = new kh_rng_s(seed);
+  });
+}
+
+int kh_rng_destroy(kh_rng *r) {
+  return guard([&] { delete r; });
+}
+
+int kh_rng_normal(kh_rng *r, int64_t n, double *out) {
+  return guard([&] {
+    need(r && (out || n == 0) && n >= 0, "bad argument");
+    r->s.fill(out, (std::size_t)n);
+  });
+}
+
+int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
+                      int precision, const double *y0, double *right, int64_t right_cap,
+                      double *left, int64_t left_cap, kh_cyclic_result *res) {
+  return guard([&] {
+    need(A && rng && opt && res, "null argument");
+    auto &c = *A->owner->ctx;
+    kh::CyclicParams P = to_params(*opt, precision);
+    P.y0_host = y0;
+    kh::CsrOp op(*A->A, false);
+    kh::CyclicStats S;
+    kh::DeviceBasis R = kh::cyclic_right(c, op, rng->s, P, S);
+    std::memset(res, 0, sizeof(*res));
+    res->n_right = R.count;
+    put_basis(c, R, right, right_cap);
+    kh::CyclicStats SL;
+    if (P.return_left_space && A->A->m > 0 && A->A->n > 0) {
+      // CyclicLanczosSolver.cpp:626-659: a second run on A^T with the listed
+      // options copied and no preconditioner.
+      kh::CyclicParams L = P;
+      L.y0_host = nullptr;
+      L.jacobi_AAt = false;
+      L.nullity_known = -1;
+      if (P.rank_known >= 0) {
+        const int left_nullity = (int)A->A->m - P.rank_known;
+        L.nullity_known = std::max(0, std::min(left_nullity, (int)P.k));
+      }
+      kh::CsrOp opt_t(*A->A, true);
+      kh::DeviceBasis Lb = kh::cyclic_right(c, opt_t, rng->s, L, SL);
+      res->n_left = Lb.count;
+      put_basis(c, Lb, left, left_cap);
+    }
+    res->restarts = S.restarts;
+    res->krylov_steps_total = S.krylov_steps_total;
+    res->frob_norm_est = S.frob_norm_est;
+    res->converged_to_target = S.converged_to_target;
+    res->cg_solves = S.cg_solves + SL.cg_solves;
+    res->cg_maxit_hits = S.cg_maxit_hits + SL.cg_maxit_hits;
+    res->spmv = S.spmv + SL.spmtv;
+    res->spmtv = S.spmtv + SL.spmv;
+    res->cg_iters = S.cg_iters + SL.cg_iters;
+    res->t_gk = S.t.gk + SL.t.gk;
+    res->t_tiny = S.t.tiny + SL.t.tiny;
+    res->t_cand = S.t.cand + SL.t.cand;
+    res->t_polish = S.t.polish + SL.t.polish;
+    res->t_fallback = S.t.fallback + SL.t.fallback;
+    res->t_rng = S.t.rng + SL.t.rng;
+  });
+}
+
+int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt, int precision,
+              const double *initial_block, int64_t n_initial, double *right, int64_t right_cap,
+              double *theta, double *residuals, kh_lobpcg_result *res) {
+  return guard([&] {
+    need(A && rng && opt && res, "null argument");
+    auto &c = *A->owner->ctx;
+    kh::LobpcgParams P;
+    P.nev = opt->nev;
+    P.block_size = opt->block_size;
+    P.max_iters = opt->max_iters;
+    P.tol_abs = opt->tol_abs;
+    P.tol_rel = opt->tol_rel;
+    P.soft_locking = opt->soft_locking != 0;
+    P.gs_reorth = opt->gs_reorth != 0;
+    P.restart = opt->restart;
+    P.gs_tol = opt->gs_tol;
+    P.frob_samples = opt->frob_samples;
+    P.nullity_known = opt->nullity_known;
+    P.rank_known = opt->rank_known;
+    P.verbose = opt->verbose != 0;
+    P.max_tiny_dim = opt->max_tiny_dim;
+    P.progress_every = opt->progress_every;
+    P.block = prec_of(precision);
+    P.initial_block_host = initial_block;
+    P.n_initial = initial_block ? n_initial : 0;
+    kh::CsrOp op(*A->A, false);
+    kh::LobpcgStats S;
+    kh::DeviceBasis R = kh::lobpcg(c, op, rng->s, P, S);
+    std::memset(res, 0, sizeof(*res));
+    res->n_right = R.count;
+    put_basis(c, R, right, right_cap);
+    for (int64_t i = 0; i < R.count && i < right_cap; ++i) {
+      if (theta)
+        theta[i] = S.theta[(std::size_t)i];
+      if (residuals)
+        residuals[i] = S.residuals_Av[(std::size_t)i];
+    }
+    res->iters = S.iters;
+    res->locked = S.locked;
+    res->frob_norm_est = S.frob_norm_est;
+    res->accept_threshold = S.accept_threshold;
+    res->best_unaccepted_residual = S.best_unaccepted_residual;
+    res->block_size = S.block_size;
+    res->converged_to_target = S.converged_to_target;
+    res->spmv = S.spmv;
+    res->spmtv = S.spmtv;
+    res->t_gram = S.t_gram;
+    res->t_sparse = S.t_sparse;
+    res->t_host = S.t_host;
+  });
+}
+
+int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out) {
+  return guard([&] {
+    need(ctx && out, "null argument");
+    KH_CHECK(hipSetDevice(ctx->ctx->device));
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      throw kh::DeviceOOM("hipMalloc: out of device memory");
+    }
+    KH_CHECK(e);
+    *out = p;
+  });
+}
+
+int kh_dev_free(kh_context *ctx, void *p) {
+  return guard([&] {
+    need(ctx, "null context");
+    ctx->ctx->sync();
+    KH_CHECK(hipFree(p));
+  });
+}
+
+int kh_h2d(kh_context *ctx, void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    need(ctx, "null context");
+    KH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->ctx->stream));
+    ctx->ctx->sync();
+  });
+}
+
+int kh_d2h(kh_context *ctx, void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    need(ctx, "null context");
+    KH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->ctx->stream));
+    ctx->ctx->sync();
+  });
+}
+
+int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s, const void *x, void *y) {
+  return guard([&] {
+    need(A && x && y && s >= 1, "bad argument");
+    auto &c = *A->owner->ctx;
+    const auto &D = *A->A;
+    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    if (s == 1)
+      kh::spmv(c, D, transpose != 0, prec_of(precision), x, y);
+    else
+      kh::spmm(c, D, transpose != 0, prec_of(precision), s, x, nin, y, nout, nullptr);
+    c.sync();
+  });
+}
+
+int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b, int64_t len,
+               double *out) {
+  return guard([&] {
+    need(ctx && a && b && out, "null argument");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    kh::dot(c, p, a, p, b, len, c.dscal, false);
+    KH_CHECK(hipMemcpyAsync(out, c.dscal, 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
+                double *h_out, double *norm_out) {
+  return guard([&] {
+    need(ctx && V && w && s >= 0 && s <= 4096, "bad argument");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    double *h1 = c.dscal + 16, *h2 = h1 + 4096, *nrm = c.dscal + 8;
+    kh::gemv_t(c, p, V, len, s, p, w, len, h1);
+    kh::gemv_n_sub(c, p, V, len, s, h1, p, w, len, nullptr);
+    kh::gemv_t(c, p, V, len, s, p, w, len, h2);
+    kh::gemv_n_sub(c, p, V, len, s, h2, p, w, len, nrm);
+    std::vector<double> a((std::size_t)s), b((std::size_t)s);
+    KH_CHECK(hipMemcpyAsync(a.data(), h1, (std::size_t)s * 8, hipMemcpyDeviceToHost, c.stream));
+    KH_CHECK(hipMemcpyAsync(b.data(), h2, (std::size_t)s * 8, hipMemcpyDeviceToHost, c.stream));
+    double nn = 0;
+    KH_CHECK(hipMemcpyAsync(&nn, nrm, 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (h_out)
+      for (int j = 0; j < s; ++j)
+        h_out[j] = a[(std::size_t)j] + b[(std::size_t)j];
+    if (norm_out)
+      *norm_out = nn;
+  });
+}
+
+int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, int k,
+                int64_t len, double *S, double *G) {
+  return guard([&] {
+    need(ctx && Z && AZ && S && G && k >= 1, "bad argument");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    kh::DBuf d((std::size_t)k * k * 2 * 8);
+    kh::gemm_tn(c, p, Z, len, k, p, Z, len, k, len, d.as<double>());
+    kh::gemm_tn(c, p, Z, len, k, p, AZ, len, k, len, d.as<double>() + (std::size_t)k * k);
+    KH_CHECK(hipMemcpyAsync(S, d.p, (std::size_t)k * k * 8, hipMemcpyDeviceToHost, c.stream));
+    KH_CHECK(hipMemcpyAsync(G, d.as<double>() + (std::size_t)k * k, (std::size_t)k * k * 8,
+                            hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps, double *avg_ms) {
+  return guard([&] {
+    need(A && avg_ms && reps >= 1, "bad argument");
+    auto &c = *A->owner->ctx;
+    const auto &D = *A->A;
+    const kh::Prec p = prec_of(precision);
+    const std::size_t e = p == kh::Prec::F64 ? 8 : 4;
+    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    kh::DBuf x((std::size_t)std::max<int64_t>(1, nin) * e),
+        y((std::size_t)std::max<int64_t>(1, nout) * e);
+    kh::fill_zero(c, p, nin, x.p);
+    hipEvent_t e0, e1;
+    KH_CHECK(hipEventCreate(&e0));
+    KH_CHECK(hipEventCreate(&e1));
+    kh::spmv(c, D, transpose != 0, p, x.p, y.p);
+    KH_CHECK(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r)
+      kh::spmv(c, D, transpose != 0, p, x.p, y.p);
+    KH_CHECK(hipEventRecord(e1, c.stream));
+    KH_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    KH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+  });
+}
+
+}  // extern "C"

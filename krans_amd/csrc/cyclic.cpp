@@ -1,0 +1,822 @@
+// Device-resident cyclic Lanczos (Golub-Kahan) null-space driver.
+//
+// Follows find_right_nullspace (src/Solvers/CyclicLanczosSolver.cpp:291-616)
+// step for step: the same thresholds, restart/stagnation/fallback logic,
+// counters and order of random draws. What differs is where and how the
+// vector work runs:
+//  * all vectors live in HBM; only the scalars the control flow branches on
+//    (beta_j, candidate norms and residuals, CG stop state) come back;
+//  * reorthogonalisation is classical Gram-Schmidt applied twice (CGS2, one
+//    V^T w then one w -= V h per pass) instead of modified Gram-Schmidt;
+//  * the candidates of a restart are processed in windows: the best-ranked
+//    one alone, then the rest as one batch whose projector polishes run as a
+//    multi-column CG (each column keeps its own scalar recurrence). The
+//    host then replays the reference's sequential accept/stall/best logic
+//    over the window; candidates after an acceptance are redone against the
+//    enlarged basis, so every candidate sees exactly the basis it would see
+//    in the sequential loop.
+//  * with P.krylov == F32 the Golub-Kahan vectors, products and
+//    reorthogonalisation run in FP32 (FP64 inner products); candidates,
+//    the locked basis and the polish residuals stay FP64.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <numeric>
+
+#include "dense.hpp"
+#include "solver.hpp"
+
+namespace kh {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+inline double since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+inline std::size_t esize(Prec p) { return p == Prec::F64 ? 8 : 4; }
+
+// Read n device doubles into out (through the pinned slab mirror).
+void readback(Context &ctx, const double *d, int n, double *out) {
+  if (n <= 0)
+    return;
+  if (n > Context::kScalars)
+    throw std::length_error("readback too large");
+  KH_CHECK(hipMemcpyAsync(ctx.hscal, d, sizeof(double) * (std::size_t)n,
+                          hipMemcpyDeviceToHost, ctx.stream));
+  KH_CHECK(hipStreamSynchronize(ctx.stream));
+  std::copy(ctx.hscal, ctx.hscal + n, out);
+}
+
+void upload_doubles(Context &ctx, const double *h, int n, double *d) {
+  if (n <= 0)
+    return;
+  if (n > Context::kScalars)
+    throw std::length_error("upload too large");
+  // stage through pinned memory; the sync keeps the staging buffer stable
+  std::copy(h, h + n, ctx.hscal);
+  KH_CHECK(hipMemcpyAsync(d, ctx.hscal, sizeof(double) * (std::size_t)n,
+                          hipMemcpyHostToDevice, ctx.stream));
+  KH_CHECK(hipStreamSynchronize(ctx.stream));
+}
+
+// Upload a host vector of FP64 (any length) via pageable memory.
+void upload_vec(Context &ctx, const double *h, int64_t n, double *d) {
+  KH_CHECK(hipMemcpyAsync(d, h, sizeof(double) * (std::size_t)n, hipMemcpyHostToDevice,
+                          ctx.stream));
+  KH_CHECK(hipStreamSynchronize(ctx.stream));
+}
+
+struct Group {  // a block of vectors to orthogonalise against
+  const void *V;
+  int64_t ld;
+  int count;
+  Prec p;
+};
+
+class CyclicDriver {
+public:
+  CyclicDriver(Context &ctx, const DeviceOp &op, RefStream &rng, const CyclicParams &P,
+               CyclicStats &S)
+      : c_(ctx), A_(op), rng_(rng), P_(P), S_(S), m_(op.rows()), n_(op.cols()) {}
+
+  DeviceBasis run();
+
+private:
+  Context &c_;
+  const DeviceOp &A_;
+  RefStream &rng_;
+  const CyclicParams &P_;
+  CyclicStats &S_;
+  int64_t m_, n_;
+  Prec kp_ = Prec::F64;  // Krylov precision
+
+  // device state
+  DeviceBasis basis_;   // FP64 locked basis (n x k_target)
+  DBuf basis32_;        // FP32 shadow of the basis (mixed mode)
+  DBuf Y_, X_, wx_, wy_;
+  DBuf y0_, best_v_, tmp_m_, tmp_n_;
+  DBuf small_;          // device scalars of this solve
+  double *bnorm_ = nullptr, *h_ = nullptr, *wsc_ = nullptr, *C_ = nullptr, *H_ = nullptr;
+  int64_t ycap_ = 0, xcap_ = 0;
+  int64_t k_target_ = 0;
+  double accept_thr_ = 0, prescreen_thr_ = 0;
+  std::vector<double> hbuf_;
+  // window blocks
+  DBuf Wv_, AV_, Vc_, AVc_, Vs_, Rs_, Vp_, AVp_, Wm_, Tn_, Wm64_;
+  int wcap_ = 0;
+  CgBatch cg_;
+
+  void alloc();
+  void ensure_window(int w);
+  void random_vec(double *d);
+  double norm_dev(Prec p, const void *x, int64_t len, double *slot);
+  void gs_basis_vec(double *v);           // 2 CGS passes vs basis (FP64)
+  void gs_basis_block(double *W, int q);  // same on q columns
+  void cgs_pass(void *w, Prec pw, int64_t len, const std::vector<Group> &groups,
+                double *nrm_out);
+  double estimate_fro();
+  void push_basis(const double *v);
+  // candidate window processing
+  struct ColOut {
+    bool valid = false, hit = false;
+    double nv = 0, res_raw = 0, res = 0;
+    int passes = 0;
+  };
+  void polish_batch(const std::vector<int> &cols, std::vector<ColOut> &out);
+  bool fallback_one();
+  void cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
+                int64_t maxit, double rel_tol, std::vector<CgState> &st);
+  void upload_mask(const std::vector<int> &mask);
+  DBuf mask_;
+  std::vector<int> slot_of_;
+};
+
+void CyclicDriver::alloc() {
+  kp_ = P_.krylov;
+  const int64_t ncv = std::max<int64_t>(std::max<int64_t>(2, P_.ncv), k_target_ + 2);
+  ycap_ = ncv / 2 + 2;
+  xcap_ = ncv / 2 + 2;
+  basis_.len = n_;
+  basis_.cap = k_target_;
+  basis_.count = 0;
+  basis_.buf = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 8);
+  if (kp_ == Prec::F32)
+    basis32_ = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 4);
+  Y_ = DBuf((std::size_t)ycap_ * n_ * esize(kp_));
+  X_ = DBuf((std::size_t)xcap_ * m_ * esize(kp_));
+  wx_ = DBuf((std::size_t)m_ * esize(kp_));
+  wy_ = DBuf((std::size_t)n_ * esize(kp_));
+  y0_ = DBuf((std::size_t)n_ * 8);
+  best_v_ = DBuf((std::size_t)n_ * 8);
+  tmp_m_ = DBuf((std::size_t)m_ * 8);
+  tmp_n_ = DBuf((std::size_t)n_ * 8);
+  const std::size_t hsz = (std::size_t)(k_target_ + ycap_ + xcap_ + 8);
+  // layout: bnorm[ncv+8] | h[4*hsz] | wsc[4096] | C[ycap*(ncv+2)] | H[...]
+  const std::size_t n_b = (std::size_t)ncv + 8, n_h = 4 * hsz, n_w = 4096,
+                    n_c = (std::size_t)ycap_ * (std::size_t)(ncv + 2);
+  const std::size_t n_H = (std::size_t)std::max<int64_t>(1, k_target_) * 512;
+  small_ = DBuf((n_b + n_h + n_w + n_c + n_H) * 8);
+  bnorm_ = small_.as<double>();
+  h_ = bnorm_ + n_b;
+  wsc_ = h_ + n_h;
+  C_ = wsc_ + n_w;
+  H_ = C_ + n_c;
+  hbuf_.resize((std::size_t)std::max(n_, m_));
+}
+
+void CyclicDriver::ensure_window(int w) {
+  if (w <= wcap_)
+    return;
+  wcap_ = std::max(w, 8);
+  const std::size_t cn = (std::size_t)wcap_ * n_ * 8, cm = (std::size_t)wcap_ * m_ * 8;
+  Wv_ = DBuf(cn);
+  AV_ = DBuf(cm);
+  Vc_ = DBuf(cn);
+  AVc_ = DBuf(cm);
+  Vs_ = DBuf(cn);
+  Rs_ = DBuf(cm);
+  Vp_ = DBuf(cn);
+  AVp_ = DBuf(cm);
+  Tn_ = DBuf(cn);
+  Wm64_ = DBuf(cm);
+  if (P_.polish == Prec::F32)
+    Wm_ = DBuf((std::size_t)wcap_ * m_ * 4 * 2);  // FP32 w and rhs
+}
+
+void CyclicDriver::random_vec(double *d) {
+  auto t0 = Clock::now();
+  rng_.fill(hbuf_.data(), (std::size_t)n_);
+  upload_vec(c_, hbuf_.data(), n_, d);
+  S_.t.rng += since(t0);
+}
+
+double CyclicDriver::norm_dev(Prec p, const void *x, int64_t len, double *slot) {
+  dot(c_, p, x, p, x, len, slot, true);
+  double v;
+  readback(c_, slot, 1, &v);
+  return v;
+}
+
+// :324-330 gs_against_basis (2 passes), as CGS2 on the locked basis.
+void CyclicDriver::gs_basis_vec(double *v) {
+  const int nb = (int)basis_.count;
+  if (nb == 0)
+    return;
+  for (int pass = 0; pass < 2; ++pass) {
+    gemv_t(c_, Prec::F64, basis_.col(0), n_, nb, Prec::F64, v, n_, h_);
+    gemv_n_sub(c_, Prec::F64, basis_.col(0), n_, nb, h_, Prec::F64, v, n_);
+  }
+}
+
+void CyclicDriver::gs_basis_block(double *W, int q) {
+  const int nb = (int)basis_.count;
+  if (nb == 0 || q == 0)
+    return;
+  if ((std::size_t)nb * q > (std::size_t)std::max<int64_t>(1, k_target_) * 512)
+    throw std::length_error("candidate window too large for the coefficient buffer");
+  for (int pass = 0; pass < 2; ++pass) {
+    gemm_tn(c_, Prec::F64, basis_.col(0), n_, nb, Prec::F64, W, n_, q, n_, H_);
+    gemm_nn_sub(c_, Prec::F64, basis_.col(0), n_, nb, H_, q, Prec::F64, W, n_, n_);
+  }
+}
+
+// One classical Gram-Schmidt pass of w against the groups: every coefficient
+// is computed from the same w, then all projections are removed.
+void CyclicDriver::cgs_pass(void *w, Prec pw, int64_t len, const std::vector<Group> &groups,
+                            double *nrm_out) {
+  std::size_t off = 0;
+  std::vector<std::size_t> offs;
+  for (const auto &g : groups) {
+    offs.push_back(off);
+    if (g.count > 0)
+      gemv_t(c_, g.p, g.V, g.ld, g.count, pw, w, len, h_ + off);
+    off += (std::size_t)g.count;
+  }
+  int last = -1;
+  for (int i = (int)groups.size() - 1; i >= 0; --i)
+    if (groups[(std::size_t)i].count > 0) {
+      last = i;
+      break;
+    }
+  for (std::size_t i = 0; i < groups.size(); ++i) {
+    const auto &g = groups[i];
+    if (g.count <= 0)
+      continue;
+    gemv_n_sub(c_, g.p, g.V, g.ld, g.count, h_ + offs[i], pw, w, len,
+               ((int)i == last) ? nrm_out : nullptr);
+  }
+  if (last < 0 && nrm_out)
+    dot(c_, pw, w, pw, w, len, nrm_out, true);
+}
+
+// :115-127
+double CyclicDriver::estimate_fro() {
+  const std::size_t samples = (std::size_t)std::max(1, P_.frob_samples);
+  double sum2 = 0.0;
+  for (std::size_t s = 0; s < samples; ++s) {
+    random_vec(tmp_n_.as<double>());
+    A_.mv(c_, Prec::F64, tmp_n_.as<double>(), tmp_m_.as<double>());
+    ++S_.spmv;
+    const double nn = norm_dev(Prec::F64, tmp_m_.as<double>(), m_, wsc_);
+    sum2 += nn * nn;
+  }
+  return std::sqrt(sum2 / (double)samples);
+}
+
+void CyclicDriver::push_basis(const double *v) {
+  const int64_t j = basis_.count;
+  KH_CHECK(hipMemcpyAsync(basis_.col(j), v, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
+                          c_.stream));
+  if (kp_ == Prec::F32)
+    convert(c_, Prec::F64, v, Prec::F32, basis32_.as<float>() + j * n_, n_);
+  basis_.count = j + 1;
+}
+
+void CyclicDriver::upload_mask(const std::vector<int> &mask) {
+  if (mask_.bytes < mask.size() * sizeof(int))
+    mask_ = DBuf(std::max<std::size_t>(mask.size(), 64) * sizeof(int));
+  KH_CHECK(hipMemcpyAsync(mask_.p, mask.data(), mask.size() * sizeof(int),
+                          hipMemcpyHostToDevice, c_.stream));
+  KH_CHECK(hipStreamSynchronize(c_.stream));
+}
+
+// (Preconditioned) CG on (A A^T) W = RHS for q columns, every column the
+// recurrence of cg_normal_eq_prec (CyclicLanczosSolver.cpp:48-113). The host
+// polls the device-resident states once per chunk of iterations.
+void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
+                            int64_t maxit, double rel_tol, std::vector<CgState> &st) {
+  cg_.ensure(q, m_, n_, pc);
+  cg_init(c_, cg_, rhs, w, diag, rel_tol, maxit);
+  st.resize((std::size_t)q);
+  int64_t iters_done = 0;
+  int chunk = 4;
+  while (true) {
+    KH_CHECK(hipMemcpyAsync(st.data(), cg_.st.p, sizeof(CgState) * (std::size_t)q,
+                            hipMemcpyDeviceToHost, c_.stream));
+    KH_CHECK(hipStreamSynchronize(c_.stream));
+    int running = 0;
+    for (int c = 0; c < q; ++c)
+      running += st[(std::size_t)c].done == 0;
+    if (running == 0 || iters_done >= maxit)
+      break;
+    const int it = (int)std::min<int64_t>(chunk, maxit - iters_done);
+    const int *act = cg_.act.as<int>();
+    for (int k = 0; k < it; ++k) {
+      A_.mtm(c_, pc, q, cg_.pv.p, m_, cg_.tmp_n.p, n_, act);
+      A_.mm(c_, pc, q, cg_.tmp_n.p, n_, cg_.Ap.p, m_, act);
+      cg_after_products(c_, cg_, w, diag, maxit, P_.cg_breakdown_tol);
+    }
+    iters_done += it;
+    chunk = std::min(chunk * 2, 16);
+  }
+  for (int c = 0; c < q; ++c) {
+    S_.cg_iters += st[(std::size_t)c].k;
+    S_.spmv += st[(std::size_t)c].k;
+    S_.spmtv += st[(std::size_t)c].k;
+  }
+}
+
+// Batched projector polish of the window columns `cols` (indices into the
+// window block Wv/AV): per column the pass loop of
+// CyclicLanczosSolver.cpp:509-542. Slot i holds the current v of cols[i] in
+// Vc(:,i) and A v in AVc(:,i).
+void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut> &out) {
+  const int q = (int)cols.size();
+  double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
+  double *Vc = Vc_.as<double>(), *AVc = AVc_.as<double>();
+  double *Vs = Vs_.as<double>(), *Rs = Rs_.as<double>();
+  double *Vp = Vp_.as<double>(), *AVp = AVp_.as<double>();
+  double *Tn = Tn_.as<double>(), *Wm64 = Wm64_.as<double>();
+  auto dcopy = [&](double *dst, const double *src, int64_t len) {
+    KH_CHECK(hipMemcpyAsync(dst, src, (std::size_t)len * 8, hipMemcpyDeviceToDevice,
+                            c_.stream));
+  };
+  for (int i = 0; i < q; ++i) {
+    const int c = cols[(std::size_t)i];
+    dcopy(Vc + (int64_t)i * n_, Wv + (int64_t)c * n_, n_);
+    dcopy(AVc + (int64_t)i * m_, AV + (int64_t)c * m_, m_);
+  }
+  std::vector<double> res(q);
+  std::vector<int> active(q, 1);
+  for (int i = 0; i < q; ++i)
+    res[(std::size_t)i] = out[(std::size_t)cols[(std::size_t)i]].res_raw;
+  const double *diag = P_.jacobi_AAt ? A_.diag_AAt() : nullptr;
+  if (P_.jacobi_AAt && !diag)
+    throw std::invalid_argument("Jacobi preconditioner needs a CSR operator");
+  std::vector<CgState> st;
+  for (int pass = 0; pass < std::max(1, P_.projector_passes); ++pass) {
+    std::vector<int> act;
+    for (int i = 0; i < q; ++i)
+      if (active[(std::size_t)i] && res[(std::size_t)i] > accept_thr_)
+        act.push_back(i);
+    const int qa = (int)act.size();
+    if (qa == 0)
+      break;
+    const bool dense_act = (qa == q);
+    double *Vw = Vc, *Rw = AVc;
+    if (!dense_act) {
+      Vw = Vs;
+      Rw = Rs;
+      for (int a = 0; a < qa; ++a) {
+        const int i = act[(std::size_t)a];
+        dcopy(Vw + (int64_t)a * n_, Vc + (int64_t)i * n_, n_);
+        dcopy(Rw + (int64_t)a * m_, AVc + (int64_t)i * m_, m_);
+      }
+    }
+    if (P_.polish == Prec::F64) {
+      cg_solve(qa, Rw, Wm64, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
+    } else {
+      // FP32 CG on the rounded FP64 residual; the FP64 residual of the
+      // updated v below decides whether the pass is kept.
+      float *W32 = Wm_.as<float>();
+      float *R32 = W32 + (int64_t)wcap_ * m_;
+      convert(c_, Prec::F64, Rw, Prec::F32, R32, (int64_t)qa * m_);
+      cg_solve(qa, R32, W32, Prec::F32, diag, P_.cg_maxit, std::max(P_.cg_rel_tol, 1e-6),
+               st);
+      convert(c_, Prec::F32, W32, Prec::F64, Wm64, (int64_t)qa * m_);
+    }
+    for (int a = 0; a < qa; ++a) {
+      ColOut &o = out[(std::size_t)cols[(std::size_t)act[(std::size_t)a]]];
+      ++o.passes;
+      if (st[(std::size_t)a].k >= P_.cg_maxit)
+        o.hit = true;
+    }
+    // v' = v - A^T w; GS vs basis; normalise; ||A v'||
+    A_.mtm(c_, Prec::F64, qa, Wm64, m_, Tn, n_, nullptr);
+    S_.spmtv += qa;
+    sub_cols(c_, Prec::F64, n_, qa, Vw, n_, Tn, n_, Vp, n_, nullptr);
+    gs_basis_block(Vp, qa);
+    col_norms(c_, Prec::F64, Vp, n_, qa, n_, wsc_);
+    std::vector<double> np(qa);
+    readback(c_, wsc_, qa, np.data());
+    std::vector<double> sc(qa, 0.0);
+    std::vector<int> ok(qa, 0);
+    for (int a = 0; a < qa; ++a) {
+      const double v = np[(std::size_t)a];
+      if (std::isfinite(v) && v >= 1e-14) {
+        ok[(std::size_t)a] = 1;
+        sc[(std::size_t)a] = 1.0 / v;
+      }
+    }
+    upload_doubles(c_, sc.data(), qa, wsc_ + 1024);
+    scale_cols(c_, Prec::F64, n_, qa, Vp, n_, wsc_ + 1024, true);
+    upload_mask(ok);
+    A_.mm(c_, Prec::F64, qa, Vp, n_, AVp, m_, mask_.as<int>());
+    for (int a = 0; a < qa; ++a)
+      S_.spmv += ok[(std::size_t)a];
+    col_norms(c_, Prec::F64, AVp, m_, qa, m_, wsc_ + 2048);
+    std::vector<double> rp(qa);
+    readback(c_, wsc_ + 2048, qa, rp.data());
+    for (int a = 0; a < qa; ++a) {
+      const int i = act[(std::size_t)a];
+      if (!ok[(std::size_t)a] || !(rp[(std::size_t)a] < res[(std::size_t)i])) {
+        active[(std::size_t)i] = 0;
+        continue;
+      }
+      dcopy(Vc + (int64_t)i * n_, Vp + (int64_t)a * n_, n_);
+      dcopy(AVc + (int64_t)i * m_, AVp + (int64_t)a * m_, m_);
+      res[(std::size_t)i] = rp[(std::size_t)a];
+    }
+    if (P_.verbose)
+      for (int a = 0; a < qa; ++a)
+        std::printf("[Cyclic-HIP] polish pass=%d cg_its=%d ||A v'||=%.3e\n", pass,
+                    st[(std::size_t)a].k, rp[(std::size_t)a]);
+  }
+  for (int i = 0; i < q; ++i)
+    out[(std::size_t)cols[(std::size_t)i]].res = res[(std::size_t)i];
+}
+
+// :140-214
+bool CyclicDriver::fallback_one() {
+  double *v = best_v_.as<double>();
+  double *tm = tmp_m_.as<double>(), *tn = tmp_n_.as<double>();
+  const int nb0 = (int)basis_.count;
+  auto gs1 = [&]() {  // single pass vs basis
+    if (basis_.count == 0)
+      return;
+    gemv_t(c_, Prec::F64, basis_.col(0), n_, (int)basis_.count, Prec::F64, v, n_, h_);
+    gemv_n_sub(c_, Prec::F64, basis_.col(0), n_, (int)basis_.count, h_, Prec::F64, v, n_);
+  };
+  (void)nb0;
+  random_vec(v);
+  gs1();
+  double nv = norm_dev(Prec::F64, v, n_, wsc_);
+  if (nv < 1e-14)
+    return false;
+  axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nv, v);
+  A_.mv(c_, Prec::F64, v, tm);
+  ++S_.spmv;
+  ensure_window(1);
+  double *w = Wm64_.as<double>();
+  std::vector<CgState> st;
+  cg_solve(1, tm, w, Prec::F64, nullptr, 500, 1e-12, st);
+  const int its = st[0].k;
+  const double rnorm = st[0].rnorm;
+  const double ntm = norm_dev(Prec::F64, tm, m_, wsc_);
+  if (its > 0 || rnorm < ntm) {
+    A_.mtv(c_, Prec::F64, w, tn);
+    ++S_.spmtv;
+    axpby(c_, Prec::F64, n_, -1.0, tn, 1.0, v);
+  }
+  gs1();
+  nv = norm_dev(Prec::F64, v, n_, wsc_);
+  if (nv >= 1e-14)
+    axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nv, v);
+  A_.mv(c_, Prec::F64, v, tm);
+  ++S_.spmv;
+  double res = norm_dev(Prec::F64, tm, m_, wsc_);
+  if (res <= accept_thr_) {
+    push_basis(v);
+    return true;
+  }
+  const double fn = estimate_fro();
+  const double L2 = std::max(1e-30, fn * fn);
+  const double tau = 0.9 / L2;
+  for (int it = 0; it < P_.grad_max_iters && res > accept_thr_; ++it) {
+    A_.mv(c_, Prec::F64, v, tm);
+    A_.mtv(c_, Prec::F64, tm, tn);
+    S_.spmv++;
+    S_.spmtv++;
+    axpby(c_, Prec::F64, n_, -tau, tn, 1.0, v);
+    gs1();
+    nv = norm_dev(Prec::F64, v, n_, wsc_);
+    if (nv < 1e-14)
+      return false;
+    axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nv, v);
+    A_.mv(c_, Prec::F64, v, tm);
+    ++S_.spmv;
+    res = norm_dev(Prec::F64, tm, m_, wsc_);
+  }
+  if (res <= accept_thr_) {
+    push_basis(v);
+    return true;
+  }
+  return false;
+}
+
+DeviceBasis CyclicDriver::run() {
+  basis_.len = n_;
+  if (n_ == 0 || m_ == 0)
+    return std::move(basis_);
+  k_target_ = (P_.nullity_known >= 0) ? P_.nullity_known : P_.k;
+  if (k_target_ == 0) {
+    S_.converged_to_target = true;
+    return std::move(basis_);
+  }
+  if (P_.rank_known >= 0 && n_ <= P_.rank_known)
+    return std::move(basis_);
+  const int64_t ncv = std::max<int64_t>(std::max<int64_t>(2, P_.ncv), k_target_ + 2);
+  alloc();
+  const Prec kp = kp_;
+  const std::size_t ke = esize(kp);
+
+  S_.frob_norm_est = estimate_fro();
+  accept_thr_ = std::max(P_.tol_abs, P_.tol_rel * std::max(1.0, S_.frob_norm_est));
+  prescreen_thr_ = std::max(0.0, P_.prescreen_factor) * std::max(1.0, S_.frob_norm_est);
+
+  double *y0 = y0_.as<double>();
+  if (P_.y0_host)
+    upload_vec(c_, P_.y0_host, n_, y0);
+  else
+    random_vec(y0);
+
+  auto random_right = [&]() {
+    random_vec(y0);
+    gs_basis_vec(y0);
+  };
+
+  char *Yb = static_cast<char *>(Y_.p), *Xb = static_cast<char *>(X_.p);
+  auto Ycol = [&](int64_t j) { return (void *)(Yb + (std::size_t)j * n_ * ke); };
+  auto Xcol = [&](int64_t j) { return (void *)(Xb + (std::size_t)j * m_ * ke); };
+  const void *basisK = kp == Prec::F64 ? (const void *)basis_.col(0) : basis32_.p;
+
+  std::vector<double> alpha, beta;
+  int64_t restarts_since_accept = 0;
+  for (int64_t restart = 0; restart < P_.max_restarts; ++restart) {
+    S_.restarts = restart;
+    alpha.clear();
+    beta.clear();
+    auto t0 = Clock::now();
+    // ---- start vector z0 = [0; y0] orthogonal to the locked vectors
+    gs_basis_vec(y0);
+    double ny = norm_dev(Prec::F64, y0, n_, wsc_);
+    if (!std::isfinite(ny) || ny < 1e-14) {
+      random_right();
+      ny = norm_dev(Prec::F64, y0, n_, wsc_);
+      if (!(ny >= 1e-14))
+        break;
+    }
+    axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / ny, y0);
+    convert(c_, Prec::F64, y0, kp, Ycol(0), n_);
+    int64_t nyc = 1, nxc = 0, local_steps = 0;
+    double beta_last = 0.0;
+    for (int64_t j = 0; j < ncv; ++j) {
+      const bool from_y = nyc > nxc;
+      void *w = from_y ? wx_.p : wy_.p;
+      const int64_t len = from_y ? m_ : n_;
+      if (from_y) {
+        A_.mv(c_, kp, Ycol(nyc - 1), w, beta.empty() ? nullptr : Xcol(nxc - 1),
+              beta.empty() ? nullptr : bnorm_ + (j - 1));
+        ++S_.spmv;
+      } else {
+        A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
+        ++S_.spmtv;
+      }
+      const int passes = P_.reorth;
+      std::vector<Group> groups;
+      if (from_y)
+        groups.push_back({Xb, m_, (int)nxc, kp});
+      else {
+        groups.push_back({basisK, n_, (int)basis_.count, kp});
+        groups.push_back({Yb, n_, (int)nyc, kp});
+      }
+      for (int pass = 0; pass < passes; ++pass)
+        cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+      if (passes <= 0)
+        dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
+      double bj;
+      readback(c_, bnorm_ + j, 1, &bj);
+      alpha.push_back(0.0);
+      beta_last = bj;
+      if (!std::isfinite(bj) || bj < 1e-14 || nyc + nxc >= ncv)
+        break;
+      beta.push_back(bj);
+      axpby(c_, kp, len, 1.0 / bj, w, 0.0, from_y ? Xcol(nxc) : Ycol(nyc));
+      if (from_y)
+        ++nxc;
+      else
+        ++nyc;
+      ++local_steps;
+    }
+    S_.krylov_steps_total += local_steps;
+    S_.t.gk += since(t0);
+    const int64_t l = (int64_t)alpha.size();
+    if (l == 0 || !std::isfinite(beta_last))
+      break;
+
+    // ---- tiny problem & interior selection (:428-453)
+    t0 = Clock::now();
+    std::vector<double> ev;
+    dense::Mat V;
+    std::vector<int> ord;
+    bool finite = true;
+    if (P_.use_harmonic) {
+      dense::Mat Tb((int)l + 1, (int)l);
+      for (int64_t i = 0; i < l; ++i) {
+        Tb((int)i, (int)i) = alpha[(std::size_t)i] - P_.target_tau;
+        if (i + 1 < l) {
+          Tb((int)i, (int)i + 1) = beta[(std::size_t)i];
+          Tb((int)i + 1, (int)i) = beta[(std::size_t)i];
+        }
+      }
+      Tb((int)l, (int)l - 1) = beta_last;
+      dense::svd_right(Tb, ev, V);
+      ord.resize(ev.size());
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+    } else {
+      dense::Mat T((int)l, (int)l);
+      for (int64_t i = 0; i < l; ++i) {
+        T((int)i, (int)i) = alpha[(std::size_t)i];
+        if (i + 1 < l) {
+          T((int)i, (int)i + 1) = beta[(std::size_t)i];
+          T((int)i + 1, (int)i) = beta[(std::size_t)i];
+        }
+      }
+      finite = dense::sym_eig(T, ev, V);
+      ord.resize(ev.size());
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return std::abs(ev[a] - P_.target_tau) < std::abs(ev[b] - P_.target_tau);
+      });
+    }
+    for (double x : ev)
+      finite = finite && std::isfinite(x);
+    for (double x : V.a)
+      finite = finite && std::isfinite(x);
+    S_.t.tiny += since(t0);
+    if (!finite) {
+      if (P_.verbose)
+        std::printf("[Cyclic-HIP] non-finite projected problem, random restart\n");
+      random_right();
+      continue;
+    }
+
+    // ---- candidates (:455-567), processed in windows
+    t0 = Clock::now();
+    const int64_t test_cap = std::min<int64_t>(l, std::max<int64_t>(k_target_, 8));
+    const int ny_used = (int)((l + 1) / 2);
+    std::vector<int> cl;
+    for (int jbest : ord) {
+      double ce = 0.0;
+      for (int64_t i = 0; i < l; i += 2)
+        ce += V((int)i, jbest) * V((int)i, jbest);
+      if (ce >= 1e-20)
+        cl.push_back(jbest);
+    }
+    double best_res = std::numeric_limits<double>::infinity();
+    bool have_best_v = false, accepted_cycle = false, polish_stalled = false;
+    int64_t tested = 0;
+    std::size_t pos = 0;
+    bool single_next = true;
+    bool done_all = false;
+    while (pos < cl.size() && tested < test_cap) {
+      int wn = single_next ? 1 : (int)std::min<int64_t>((int64_t)(cl.size() - pos), test_cap - tested);
+      wn = std::min(wn, 64);
+      ensure_window(wn);
+      // coefficient block C (ny_used x wn)
+      std::vector<double> Ch((std::size_t)ny_used * wn, 0.0);
+      for (int c = 0; c < wn; ++c) {
+        const int jb = cl[pos + (std::size_t)c];
+        for (int64_t i = 0; i < l; i += 2)
+          Ch[(std::size_t)c * ny_used + (std::size_t)(i / 2)] = V((int)i, jb);
+      }
+      upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
+      double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
+      gemm_nn(c_, kp, Yb, n_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
+      gs_basis_block(Wv, wn);
+      col_norms(c_, Prec::F64, Wv, n_, wn, n_, wsc_);
+      std::vector<double> nv(wn);
+      readback(c_, wsc_, wn, nv.data());
+      std::vector<ColOut> out((std::size_t)wn);
+      std::vector<double> sc(wn, 0.0);
+      std::vector<int> valid(wn, 0);
+      for (int c = 0; c < wn; ++c) {
+        out[(std::size_t)c].nv = nv[(std::size_t)c];
+        if (std::isfinite(nv[(std::size_t)c]) && nv[(std::size_t)c] >= 1e-8) {
+          out[(std::size_t)c].valid = true;
+          valid[(std::size_t)c] = 1;
+          sc[(std::size_t)c] = 1.0 / nv[(std::size_t)c];
+        }
+      }
+      upload_doubles(c_, sc.data(), wn, wsc_ + 1024);
+      scale_cols(c_, Prec::F64, n_, wn, Wv, n_, wsc_ + 1024, true);
+      upload_mask(valid);
+      A_.mm(c_, Prec::F64, wn, Wv, n_, AV, m_, mask_.as<int>());
+      col_norms(c_, Prec::F64, AV, m_, wn, m_, wsc_ + 2048);
+      std::vector<double> rr(wn);
+      readback(c_, wsc_ + 2048, wn, rr.data());
+      std::vector<int> pol;
+      for (int c = 0; c < wn; ++c) {
+        ColOut &o = out[(std::size_t)c];
+        if (!o.valid)
+          continue;
+        ++S_.spmv;
+        o.res_raw = o.res = rr[(std::size_t)c];
+        if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
+            o.res_raw > accept_thr_)
+          pol.push_back(c);
+      }
+      slot_of_.assign((std::size_t)wn, -1);
+      for (std::size_t i = 0; i < pol.size(); ++i)
+        slot_of_[(std::size_t)pol[i]] = (int)i;
+      if (!pol.empty()) {
+        auto tp = Clock::now();
+        polish_batch(pol, out);
+        S_.t.polish += since(tp);
+      }
+      // replay the sequential accept / stall / best logic over the window
+      std::size_t consumed = (std::size_t)wn;
+      bool cut = false;
+      for (int c = 0; c < wn; ++c) {
+        ColOut &o = out[(std::size_t)c];
+        if (!o.valid)
+          continue;
+        ++tested;
+        if (!std::isfinite(o.res_raw))
+          continue;
+        if (o.res_raw > prescreen_thr_ || polish_stalled) {
+          if (o.res_raw < best_res) {
+            best_res = o.res_raw;
+            KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)c * n_, (std::size_t)n_ * 8,
+                                    hipMemcpyDeviceToDevice, c_.stream));
+            have_best_v = true;
+          }
+          continue;
+        }
+        const int slot = slot_of_[(std::size_t)c];
+        const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)c * n_;
+        S_.cg_solves += o.passes;
+        if (o.hit)
+          S_.cg_maxit_hits += 1;  // one counted hit per column and pass set
+        if (P_.verbose)
+          std::printf("[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
+                      (long long)restart, o.res_raw, o.res, accept_thr_);
+        if (o.res <= accept_thr_) {
+          push_basis(vfin);
+          accepted_cycle = true;
+          if (basis_.count >= k_target_) {
+            S_.converged_to_target = true;
+            done_all = true;
+            break;
+          }
+          consumed = (std::size_t)c + 1;
+          cut = true;
+          break;
+        }
+        polish_stalled = polish_stalled || o.hit;
+        if (o.res < best_res) {
+          best_res = o.res;
+          KH_CHECK(hipMemcpyAsync(best_v_.p, vfin, (std::size_t)n_ * 8,
+                                  hipMemcpyDeviceToDevice, c_.stream));
+          have_best_v = true;
+        }
+      }
+      if (done_all)
+        break;
+      pos += consumed;
+      single_next = cut;
+    }
+    S_.t.cand += since(t0);
+    if (done_all) {
+      c_.sync();
+      return std::move(basis_);
+    }
+    // ---- stagnation guard (:569-581)
+    if (!accepted_cycle) {
+      ++restarts_since_accept;
+      if (P_.stagnation_restarts > 0 && restarts_since_accept >= P_.stagnation_restarts) {
+        if (P_.verbose)
+          std::printf("[Cyclic-HIP] stopping due to stagnation\n");
+        break;
+      }
+    } else {
+      restarts_since_accept = 0;
+    }
+    // ---- restart vector (:584)
+    if (have_best_v)
+      KH_CHECK(hipMemcpyAsync(y0, best_v_.p, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
+                              c_.stream));
+    else
+      random_right();
+  }
+
+  // ---- fallback (:587-602)
+  if (basis_.count < k_target_) {
+    auto t0 = Clock::now();
+    int failures = 0;
+    while (basis_.count < k_target_ && failures < 8) {
+      if (fallback_one())
+        failures = 0;
+      else
+        ++failures;
+    }
+    S_.converged_to_target = basis_.count >= k_target_;
+    S_.t.fallback += since(t0);
+  }
+  c_.sync();
+  return std::move(basis_);
+}
+
+}  // namespace
+
+DeviceBasis cyclic_right(Context &ctx, const DeviceOp &op, RefStream &rng,
+                         const CyclicParams &P, CyclicStats &S) {
+  CyclicDriver d(ctx, op, rng, P, S);
+  return d.run();
+}
+
+}  // namespace kh

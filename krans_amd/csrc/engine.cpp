@@ -1,0 +1,203 @@
+// Device context, buffers and the device CSR/CSC build of the KraNS-MI
+// engine. The CSC twin is produced by a stable counting sort over the rows of
+// the canonical CSR, so within every column the rows ascend and perm[q] is
+// the CSR position of CSC entry q: the same structure as Eigen's
+// A_.transpose() of EigenSparseMatrixAdapter.hpp:20-24.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+namespace kh {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess)
+    return;
+  std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory)
+    throw DeviceOOM(msg);
+  throw DeviceError(msg);
+}
+
+Context::Context(int dev) : device(dev) {
+  KH_CHECK(hipSetDevice(dev));
+  KH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  KH_CHECK(hipMalloc((void **)&dscal, sizeof(double) * kScalars));
+  KH_CHECK(hipMemset(dscal, 0, sizeof(double) * kScalars));
+  KH_CHECK(hipHostMalloc((void **)&hscal, sizeof(double) * kScalars, hipHostMallocDefault));
+  part_cap = (std::size_t)1 << 24;  // 16 Mi doubles = 128 MiB of reduction partials
+  KH_CHECK(hipMalloc((void **)&dpart, sizeof(double) * part_cap));
+  hipDeviceProp_t prop;
+  KH_CHECK(hipGetDeviceProperties(&prop, dev));
+  cu_count = prop.multiProcessorCount;
+}
+
+Context::~Context() {
+  if (stream)
+    (void)hipStreamSynchronize(stream);
+  if (dpart)
+    (void)hipFree(dpart);
+  if (dscal)
+    (void)hipFree(dscal);
+  if (hscal)
+    (void)hipHostFree(hscal);
+  if (stream)
+    (void)hipStreamDestroy(stream);
+}
+
+void Context::sync() { KH_CHECK(hipStreamSynchronize(stream)); }
+
+void Context::fetch_scalars(int i0, int n) {
+  if (i0 < 0 || n < 0 || i0 + n > kScalars)
+    throw std::out_of_range("scalar slab index out of range");
+  KH_CHECK(hipMemcpyAsync(hscal + i0, dscal + i0, sizeof(double) * (std::size_t)n,
+                          hipMemcpyDeviceToHost, stream));
+  KH_CHECK(hipStreamSynchronize(stream));
+}
+
+double *Context::partials(std::size_t need) {
+  if (need > part_cap)
+    throw std::length_error("reduction partial buffer too small for this batch");
+  return dpart;
+}
+
+DBuf::DBuf(std::size_t b) : bytes(b) {
+  if (b == 0)
+    return;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory)
+      throw DeviceOOM("hipMalloc(" + std::to_string(b) + " bytes): out of device memory");
+    hip_check(e, "hipMalloc");
+  }
+}
+
+DBuf::~DBuf() {
+  if (p)
+    (void)hipFree(p);
+}
+
+DBuf &DBuf::operator=(DBuf &&o) noexcept {
+  if (this != &o) {
+    if (p)
+      (void)hipFree(p);
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+static int pick_lanes(double avg, int maxlen) {
+  // Short rows: one lane per row keeps the reference's summation order.
+  if (avg <= 12.0 && maxlen <= 256)
+    return 1;
+  if (avg <= 48.0)
+    return 4;
+  if (avg <= 192.0)
+    return 16;
+  return 64;
+}
+
+template <class T> static DBuf upload(Context &ctx, const T *h, std::size_t count) {
+  DBuf d(count * sizeof(T) + (count == 0 ? sizeof(T) : 0));
+  if (count)
+    KH_CHECK(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+  return d;
+}
+
+DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
+                      const int32_t *col, const double *val) {
+  if (m < 0 || n < 0)
+    throw std::invalid_argument("csr_create: negative dimension");
+  if (m > INT32_MAX || n > INT32_MAX)
+    throw std::invalid_argument("csr_create: dimension exceeds int32 indexing");
+  if (!rowptr)
+    throw std::invalid_argument("csr_create: null rowptr");
+  if (rowptr[0] != 0)
+    throw std::invalid_argument("csr_create: rowptr[0] != 0");
+  const int64_t nnz = rowptr[m];
+  if (nnz < 0)
+    throw std::invalid_argument("csr_create: negative nnz");
+  if (nnz > 0 && (!col || !val))
+    throw std::invalid_argument("csr_create: null col/val");
+  int max_row = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    const int32_t a = rowptr[i], b = rowptr[i + 1];
+    if (b < a)
+      throw std::invalid_argument("csr_create: rowptr not monotone");
+    max_row = std::max(max_row, b - a);
+    for (int32_t p = a; p < b; ++p) {
+      if (col[p] < 0 || col[p] >= n)
+        throw std::invalid_argument("csr_create: column index out of range");
+      if (p > a && col[p] <= col[p - 1])
+        throw std::invalid_argument(
+            "csr_create: columns must be strictly ascending within a row (canonical CSR)");
+    }
+  }
+  auto *A = new DeviceCsr;
+  try {
+    A->m = m;
+    A->n = n;
+    A->nnz = nnz;
+    // CSC twin: stable counting sort over rows
+    std::vector<int32_t> cp((std::size_t)n + 1, 0), ri((std::size_t)nnz), perm((std::size_t)nnz);
+    std::vector<double> vt((std::size_t)nnz);
+    for (int64_t p = 0; p < nnz; ++p)
+      cp[(std::size_t)col[p] + 1]++;
+    int max_col = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      max_col = std::max(max_col, cp[(std::size_t)j + 1]);
+      cp[(std::size_t)j + 1] += cp[(std::size_t)j];
+    }
+    std::vector<int32_t> next(cp.begin(), cp.end() - 1);
+    double fro2 = 0.0;
+    std::vector<double> dAAt((std::size_t)m, 0.0);
+    for (int64_t i = 0; i < m; ++i) {
+      double rs = 0.0;
+      for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+        const int32_t q = next[(std::size_t)col[p]]++;
+        ri[(std::size_t)q] = (int32_t)i;
+        vt[(std::size_t)q] = val[p];
+        perm[(std::size_t)q] = p;
+        rs += val[p] * val[p];
+      }
+      dAAt[(std::size_t)i] = rs;
+      fro2 += rs;
+    }
+    A->fro = std::sqrt(fro2);
+    A->max_row = max_row;
+    A->max_col = max_col;
+    A->lpr_rows = pick_lanes(m ? (double)nnz / (double)m : 0.0, max_row);
+    A->lpr_cols = pick_lanes(n ? (double)nnz / (double)n : 0.0, max_col);
+    std::vector<float> va32((std::size_t)nnz), vt32((std::size_t)nnz);
+    for (int64_t p = 0; p < nnz; ++p) {
+      va32[(std::size_t)p] = (float)val[p];
+      vt32[(std::size_t)p] = (float)vt[(std::size_t)p];
+    }
+    A->rp = upload(ctx, rowptr, (std::size_t)m + 1);
+    A->ci = upload(ctx, col, (std::size_t)nnz);
+    A->va = upload(ctx, val, (std::size_t)nnz);
+    A->va32 = upload(ctx, va32.data(), (std::size_t)nnz);
+    A->cp = upload(ctx, cp.data(), (std::size_t)n + 1);
+    A->ri = upload(ctx, ri.data(), (std::size_t)nnz);
+    A->vt = upload(ctx, vt.data(), (std::size_t)nnz);
+    A->vt32 = upload(ctx, vt32.data(), (std::size_t)nnz);
+    A->perm = upload(ctx, perm.data(), (std::size_t)nnz);
+    A->diag_AAt = upload(ctx, dAAt.data(), (std::size_t)m);
+    A->diag_AAt_host = std::move(dAAt);
+    ctx.sync();  // host staging vectors die at scope exit
+  } catch (...) {
+    delete A;
+    throw;
+  }
+  return A;
+}
+
+}  // namespace kh

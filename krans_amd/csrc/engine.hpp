@@ -1,0 +1,189 @@
+// Internal C++ API of the KraNS-MI engine: device context, device CSR with its
+// CSC twin, device buffers and the launchers of the HIP kernels in
+// kernels.hip. Host code above this layer (solvers, adapters, C ABI) never
+// touches hip* calls except through these helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace kh {
+
+// ---------------------------------------------------------------------------
+// Errors: every HIP failure becomes a DeviceError; the C ABI maps it to a
+// status code, the C++ adapters re-throw the reference's exception types.
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct DeviceOOM : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what);
+#define KH_CHECK(expr) ::kh::hip_check((expr), #expr)
+
+enum class Prec : int { F64 = 0, F32 = 1 };
+
+// ---------------------------------------------------------------------------
+// Context: one device, one stream, a slab of device scalars and a pinned
+// host mirror used to read scalars back with one small copy.
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *dscal = nullptr;     // device scalar slab (kScalars doubles)
+  double *hscal = nullptr;     // pinned host mirror
+  double *dpart = nullptr;     // reduction partials
+  std::size_t part_cap = 0;    // doubles
+  int cu_count = 0;
+  static constexpr int kScalars = 1 << 16;
+  explicit Context(int dev);
+  ~Context();
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  void sync();
+  // copy `n` device scalars starting at slab index `i0` into hscal[i0..]
+  void fetch_scalars(int i0, int n);
+  double *partials(std::size_t need);
+};
+
+// Device allocation with an owner; moves only.
+struct DBuf {
+  void *p = nullptr;
+  std::size_t bytes = 0;
+  DBuf() = default;
+  explicit DBuf(std::size_t b);
+  ~DBuf();
+  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Device sparse matrix: canonical CSR (int32 pointers/indices, columns
+// ascending within each row) and its CSC twin (rows ascending within each
+// column) with the permutation perm[q] = CSR position of CSC entry q. Values
+// are held in FP64 and, for the mixed-precision path, rounded to FP32.
+struct DeviceCsr {
+  int64_t m = 0, n = 0, nnz = 0;
+  DBuf rp, ci, va, va32;  // CSR
+  DBuf cp, ri, vt, vt32;  // CSC twin
+  DBuf perm;
+  int lpr_rows = 1, lpr_cols = 1;  // lanes per row / per column (autotuned)
+  int max_row = 0, max_col = 0;
+  double fro = 0.0;  // exact ||A||_F from the CSR values
+  std::vector<double> diag_AAt_host;
+  DBuf diag_AAt;  // row sums of squares, for the Jacobi preconditioner
+};
+
+// Build from host canonical CSR (validated). Throws std::invalid_argument
+// on malformed input.
+DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n,
+                      const int32_t *rowptr, const int32_t *col,
+                      const double *val);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (kernels.hip). All run on ctx.stream, asynchronously.
+// Vectors are contiguous; multi-vector blocks are column-major with leading
+// dimension `ld` (elements).
+
+// y = A x (trans=false, CSR) or y = A^T x (trans=true, CSC twin).
+// With z != nullptr: y = A x - (*beta) z (beta on device).
+void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
+          void *y, const void *z = nullptr, const double *beta = nullptr);
+// Same operator on `s` columns at once (x: len_in x s, y: len_out x s).
+// Columns whose active flag is 0 are skipped (active may be nullptr).
+void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
+          const void *x, int64_t ldx, void *y, int64_t ldy,
+          const int *active);
+
+// out[0] = sum_i a_i b_i (FP64 accumulate, fixed reduction tree); with
+// do_sqrt: out[0] = sqrt(max(0,sum)).
+void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
+         int64_t len, double *out, bool do_sqrt = false);
+// out[j] = V(:,j) . w for j < s (FP64 accumulate).
+void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
+            const void *w, int64_t len, double *out);
+// w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
+void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
+                const double *h, Prec pw, void *w, int64_t len,
+                double *nrm2_out = nullptr);
+// out = V h (overwrite)
+void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
+            const double *h, Prec po, void *out, int64_t len);
+// Multi-column: OUT(:,c) = V * H(:,c) for c < q, H is s x q FP64 on device.
+void gemm_nn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
+             const double *H, int q, Prec po, void *OUT, int64_t ldo,
+             int64_t len);
+
+// H(:,c) = V^T W(:,c) for c < q; H is s x q (column-major) FP64 on device.
+void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
+             const void *W, int64_t ldw, int q, int64_t len, double *H);
+// W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.
+void gemm_nn_sub(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
+                 const double *H, int q, Prec pw, void *W, int64_t ldw,
+                 int64_t len, double *norms = nullptr);
+// out[c] = ||W(:,c)||_2 (FP64 accumulate, fixed tree).
+void col_norms(Context &ctx, Prec p, const void *W, int64_t ldw, int q,
+               int64_t len, double *out);
+// W(:,c) *= scale[c] (device scalars).
+void scale_cols(Context &ctx, Prec p, int64_t len, int q, void *W, int64_t ldw,
+                const double *scale, bool skip_zero);
+// OUT(:,c) = X(:,c) - Y(:,c) where mask[c] (mask may be null).
+void sub_cols(Context &ctx, Prec p, int64_t len, int q, const void *X,
+              int64_t ldx, const void *Y, int64_t ldy, void *OUT, int64_t ldo,
+              const int *mask);
+
+// y = alpha*x + beta*y (host scalars); x may be null (y = beta*y).
+void axpby(Context &ctx, Prec p, int64_t len, double alpha, const void *x,
+           double beta, void *y);
+void scale_dev(Context &ctx, Prec p, int64_t len, const double *num,
+               bool reciprocal, void *y);  // y *= (*num) or y *= 1/(*num)
+void copy(Context &ctx, Prec p, int64_t len, const void *x, void *y);
+void convert(Context &ctx, Prec from, const void *x, Prec to, void *y,
+             int64_t len);
+void fill_zero(Context &ctx, Prec p, int64_t len, void *y);
+
+// Jacobi preconditioner y = r ./ d (d>0) else r.
+void diag_solve(Context &ctx, Prec p, int64_t len, const double *d,
+                const void *r, void *y);
+
+// ---------------------------------------------------------------------------
+// CG on (A A^T) w = rhs with s independent right-hand sides (column-major,
+// length m each), FP64 or FP32 arithmetic. Every column follows exactly the
+// scalar recurrence of cg_normal_eq_prec, with its own alpha/beta and stop
+// flags kept on the device; the host launches iterations in chunks and
+// polls the per-column state once per chunk.
+struct CgState {  // per column, device resident
+  double rz, rnorm, tol, pAp, rz_new, rhsn;
+  int32_t k, done, reason, pad;
+};
+struct CgBatch {
+  int s = 0;
+  int64_t m = 0, n = 0;
+  Prec p = Prec::F64;
+  DBuf r, z, pv, Ap, tmp_n, st, act;
+  bool have_prec = false;
+  void ensure(int s_, int64_t m_, int64_t n_, Prec p_);
+};
+// Initialise: w = 0, r = rhs, z = M^{-1} r, p = z, per-column rho/tol.
+void cg_init(Context &ctx, CgBatch &B, const void *rhs, void *w,
+             const double *diag, double rel_tol, int64_t maxit);
+// The scalar part of one CG iteration, after the caller has formed
+// B.Ap = A (A^T B.pv) for the active columns (B.act): pAp, alpha, the w/r/z
+// updates, rho, beta and the new direction. No-ops for stopped columns.
+void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
+                       int64_t maxit, double breakdown_tol);
+
+// Partial-buffer size needed for the reductions of `len` elements.
+int reduce_blocks(int64_t len);
+
+}  // namespace kh

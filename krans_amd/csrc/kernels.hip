@@ -1,0 +1,1119 @@
+// HIP kernels of the KraNS-MI engine (gfx950 / CDNA4, wave64).
+//
+// Design rules (see engine.hpp for the launchers' contracts):
+//  * No atomics anywhere: every reduction is a fixed tree (per-thread
+//    sequential partial -> wave64 shuffle tree -> LDS across the block's 4
+//    waves -> per-block partial -> one finishing block summing the partials
+//    in index order). The block count depends only on the length, so results
+//    are bitwise reproducible run to run.
+//  * Sparse products are gathers: y = A x over the CSR rows, y = A^T x over
+//    the CSC twin's columns. With one lane per row (short rows) each output
+//    is summed sequentially in ascending inner index from 0.0 with no FMA
+//    contraction, which is the rounding of the reference's Eigen products
+//    (EigenSparseMatrixAdapter.hpp:34-57).
+//  * Scalars (alpha, beta, norms, CG state) stay on the device; kernels read
+//    them through pointers so the host can queue work without syncing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "engine.hpp"
+
+namespace kh {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    lds[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kWaves; ++w)
+      t += lds[w];
+  }
+  __syncthreads();
+  return t;
+}
+
+template <class T> struct Acc;
+template <> struct Acc<double> { using type = double; };
+template <> struct Acc<float> { using type = float; };
+
+inline int grid_for(int64_t len, int per_thread = 1) {
+  int64_t g = (len + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
+  if (g < 1)
+    g = 1;
+  if (g > 16384)
+    g = 16384;
+  return (int)g;
+}
+
+// ----------------------------------------------------------------- sparse --
+// One lane per row: sequential ascending sum, no contraction.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv_lane(int64_t rows, const int32_t *__restrict__ ptr,
+                const int32_t *__restrict__ idx, const T *__restrict__ val,
+                const T *__restrict__ x, T *__restrict__ y,
+                const T *__restrict__ z, const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  const T b = z ? (T)(*beta) : (T)0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < rows;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+    A acc = (A)0;
+    for (int32_t p = p0; p < p1; ++p)
+      acc += (A)val[p] * (A)x[idx[p]];
+    T out = (T)acc;
+    if (z)
+      out = out + (-b) * z[i];
+    y[i] = out;
+  }
+}
+
+// G lanes per row (G in 4,16,64): strided partial sums, shuffle tree.
+template <class T, int G>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv_group(int64_t rows, const int32_t *__restrict__ ptr,
+                 const int32_t *__restrict__ idx, const T *__restrict__ val,
+                 const T *__restrict__ x, T *__restrict__ y,
+                 const T *__restrict__ z, const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  const int sub = threadIdx.x % G;
+  const T b = z ? (T)(*beta) : (T)0;
+  const int64_t groups_per_grid = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t i = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; i < rows;
+       i += groups_per_grid) {
+    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+    A acc = (A)0;
+    for (int32_t p = p0 + sub; p < p1; p += G)
+      acc += (A)val[p] * (A)x[idx[p]];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+      acc += __shfl_down(acc, off, G);
+    if (sub == 0) {
+      T out = (T)acc;
+      if (z)
+        out = out + (-b) * z[i];
+      y[i] = out;
+    }
+  }
+}
+
+// Multi-RHS gather, one lane per row, columns c < s with active[c] != 0.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_spmm_lane(int64_t rows, const int32_t *__restrict__ ptr,
+                const int32_t *__restrict__ idx, const T *__restrict__ val,
+                int s, const T *__restrict__ x, int64_t ldx, T *__restrict__ y,
+                int64_t ldy, const int *__restrict__ active) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < rows;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+    for (int c = 0; c < s; ++c) {
+      if (active && !active[c])
+        continue;
+      const T *xc = x + (int64_t)c * ldx;
+      A acc = (A)0;
+      for (int32_t p = p0; p < p1; ++p)
+        acc += (A)val[p] * (A)xc[idx[p]];
+      y[i + (int64_t)c * ldy] = (T)acc;
+    }
+  }
+}
+
+template <class T, int G>
+__global__ void __launch_bounds__(kBlock)
+    k_spmm_group(int64_t rows, const int32_t *__restrict__ ptr,
+                 const int32_t *__restrict__ idx, const T *__restrict__ val,
+                 int s, const T *__restrict__ x, int64_t ldx,
+                 T *__restrict__ y, int64_t ldy, const int *__restrict__ active) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  const int sub = threadIdx.x % G;
+  const int64_t groups_per_grid = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t i = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; i < rows;
+       i += groups_per_grid) {
+    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+    for (int c = 0; c < s; ++c) {
+      if (active && !active[c])
+        continue;
+      const T *xc = x + (int64_t)c * ldx;
+      A acc = (A)0;
+      for (int32_t p = p0 + sub; p < p1; p += G)
+        acc += (A)val[p] * (A)xc[idx[p]];
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1)
+        acc += __shfl_down(acc, off, G);
+      if (sub == 0)
+        y[i + (int64_t)c * ldy] = (T)acc;
+    }
+  }
+}
+
+// ------------------------------------------------------------- reductions --
+template <class TA, class TB>
+__global__ void __launch_bounds__(kBlock)
+    k_dot_partial(int64_t len, const TA *__restrict__ a, const TB *__restrict__ b,
+                  double *__restrict__ part) {
+  __shared__ double lds[kWaves];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += (double)a[i] * (double)b[i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = acc;
+}
+
+// out[col] = sum of part[col*nb .. col*nb+nb) in index order (one block per
+// column); optional sqrt(max(0, .)).
+__global__ void __launch_bounds__(kBlock)
+    k_finish(const double *__restrict__ part, int nb, double *__restrict__ out,
+             int do_sqrt) {
+  __shared__ double lds[kWaves];
+  const double *pc = part + (int64_t)blockIdx.x * nb;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    acc += pc[i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = do_sqrt ? sqrt(fmax(0.0, acc)) : acc;
+}
+
+// V^T w over column tiles of CT columns: blockIdx.y = tile.
+template <class TV, class TW, int CT>
+__global__ void __launch_bounds__(kBlock)
+    k_gemv_t_partial(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                     const TW *__restrict__ w, double *__restrict__ part, int nb) {
+  __shared__ double lds[CT][kWaves];
+  const int c0 = blockIdx.y * CT;
+  double acc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+    acc[c] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+      if (c0 + c < s)
+        acc[c] += (double)V[i + (int64_t)(c0 + c) * ld] * wi;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0)
+      lds[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < CT && c0 + (int)threadIdx.x < s) {
+    double t = 0.0;
+    for (int w2 = 0; w2 < kWaves; ++w2)
+      t += lds[threadIdx.x][w2];
+    part[(int64_t)(c0 + threadIdx.x) * nb + blockIdx.x] = t;
+  }
+}
+
+// w -= V h, optional partial of ||w_new||^2.
+template <class TV, class TW>
+__global__ void __launch_bounds__(kBlock)
+    k_gemv_n_sub(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                 const double *__restrict__ h, TW *__restrict__ w,
+                 double *__restrict__ part) {
+  __shared__ double hs[512];
+  __shared__ double lds[kWaves];
+  for (int j = threadIdx.x; j < s && j < 512; j += kBlock)
+    hs[j] = h[j];
+  __syncthreads();
+  double nacc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j)
+      acc += (double)V[i + (int64_t)j * ld] * (j < 512 ? hs[j] : h[j]);
+    const TW nw = (TW)((double)w[i] - acc);
+    w[i] = nw;
+    nacc += (double)nw * (double)nw;
+  }
+  if (part) {
+    nacc = block_sum(nacc, lds);
+    if (threadIdx.x == 0)
+      part[blockIdx.x] = nacc;
+  }
+}
+
+// OUT(:,c) = V H(:,c)  (overwrite), c < q.
+template <class TV, class TO>
+__global__ void __launch_bounds__(kBlock)
+    k_gemm_nn(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
+              const double *__restrict__ H, int q, TO *__restrict__ out,
+              int64_t ldo) {
+  const int c = blockIdx.y;
+  const double *hc = H + (int64_t)c * s;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j)
+      acc += (double)V[i + (int64_t)j * ldv] * hc[j];
+    out[i + (int64_t)c * ldo] = (TO)acc;
+  }
+}
+
+// Batched V^T W: blockIdx.z = column c of W; tiles of CT columns of V.
+// part layout: [(c * s + j) * nb + bx].
+template <class TV, class TW, int CT>
+__global__ void __launch_bounds__(kBlock)
+    k_gemm_tn_partial(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
+                      const TW *__restrict__ W, int64_t ldw,
+                      double *__restrict__ part, int nb) {
+  __shared__ double lds[CT][kWaves];
+  const int c0 = blockIdx.y * CT;
+  const int cw = blockIdx.z;
+  const TW *w = W + (int64_t)cw * ldw;
+  double acc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+    acc[c] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+      if (c0 + c < s)
+        acc[c] += (double)V[i + (int64_t)(c0 + c) * ldv] * wi;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0)
+      lds[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < CT && c0 + (int)threadIdx.x < s) {
+    double t = 0.0;
+    for (int w2 = 0; w2 < kWaves; ++w2)
+      t += lds[threadIdx.x][w2];
+    part[((int64_t)cw * s + c0 + threadIdx.x) * nb + blockIdx.x] = t;
+  }
+}
+
+// W(:,c) -= V H(:,c) for c < q (blockIdx.y = c); optional partial norms of
+// the updated columns, part layout [c * nb + bx].
+template <class TV, class TW>
+__global__ void __launch_bounds__(kBlock)
+    k_gemm_nn_sub(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
+                  const double *__restrict__ H, TW *__restrict__ W, int64_t ldw,
+                  double *__restrict__ part, int nb) {
+  __shared__ double hs[512];
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.y;
+  const double *hc = H + (int64_t)c * s;
+  for (int j = threadIdx.x; j < s && j < 512; j += kBlock)
+    hs[j] = hc[j];
+  __syncthreads();
+  TW *w = W + (int64_t)c * ldw;
+  double nacc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j)
+      acc += (double)V[i + (int64_t)j * ldv] * (j < 512 ? hs[j] : hc[j]);
+    const TW nw = (TW)((double)w[i] - acc);
+    w[i] = nw;
+    nacc += (double)nw * (double)nw;
+  }
+  if (part) {
+    nacc = block_sum(nacc, lds);
+    if (threadIdx.x == 0)
+      part[(int64_t)c * nb + blockIdx.x] = nacc;
+  }
+}
+
+// Column sums of squares of W (q columns), part layout [c * nb + bx].
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_col_sq(int64_t len, const T *__restrict__ W, int64_t ldw,
+             double *__restrict__ part, int nb) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.y;
+  const T *w = W + (int64_t)c * ldw;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += (double)w[i] * (double)w[i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    part[(int64_t)c * nb + blockIdx.x] = acc;
+}
+
+// W(:,c) *= scale[c] (device scalars); columns with scale 0 are left as is
+// when skip_zero is set.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_scale_cols(int64_t len, T *__restrict__ W, int64_t ldw,
+                 const double *__restrict__ scale, int skip_zero) {
+  const int c = blockIdx.y;
+  const double sc = scale[c];
+  if (skip_zero && sc == 0.0)
+    return;
+  const T st = (T)sc;
+  T *w = W + (int64_t)c * ldw;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    w[i] = w[i] * st;
+}
+
+// OUT(:,c) = X(:,c) - Y(:,c) for c < q with mask[c] != 0.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sub_cols(int64_t len, const T *__restrict__ X, int64_t ldx,
+               const T *__restrict__ Y, int64_t ldy, T *__restrict__ OUT,
+               int64_t ldo, const int *__restrict__ mask) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.y;
+  if (mask && !mask[c])
+    return;
+  const T *x = X + (int64_t)c * ldx;
+  const T *y = Y + (int64_t)c * ldy;
+  T *o = OUT + (int64_t)c * ldo;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    o[i] = x[i] + (T)(-1) * y[i];
+}
+
+// ---------------------------------------------------------------- level 1 --
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_axpby(int64_t len, double alpha, const T *__restrict__ x, double beta,
+            T *__restrict__ y) {
+#pragma clang fp contract(off)
+  const T a = (T)alpha, b = (T)beta;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    T v = (b == (T)0) ? (T)0 : y[i];
+    if (b != (T)1 && b != (T)0)
+      v = v * b;
+    if (x)
+      v = v + a * x[i];
+    y[i] = v;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_scale_dev(int64_t len, const double *__restrict__ num, int recip,
+                T *__restrict__ y) {
+  const double s = recip ? 1.0 / (*num) : (*num);
+  const T st = (T)s;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = y[i] * st;
+}
+
+template <class TI, class TO>
+__global__ void __launch_bounds__(kBlock)
+    k_convert(int64_t len, const TI *__restrict__ x, TO *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = (TO)x[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_diag_solve(int64_t len, const double *__restrict__ d, const T *__restrict__ r,
+                 T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double di = d[i];
+    y[i] = di > 0.0 ? (T)((double)r[i] / di) : r[i];
+  }
+}
+
+// ---------------------------------------------------------------------- CG --
+// Per-column recurrence of cg_normal_eq_prec
+// (CyclicLanczosSolver.cpp:48-113), one column per blockIdx.y.
+enum : int32_t { kRun = 0, kDone = 1 };
+
+// r = rhs, w = 0, z = M^{-1} r (or alias), p = z; partials of r.r and r.z.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_init(int64_t m, const T *__restrict__ rhs, T *__restrict__ w,
+              T *__restrict__ r, T *__restrict__ z, T *__restrict__ p,
+              const double *__restrict__ d, double *__restrict__ part_rr,
+              double *__restrict__ part_rz, int nb) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.y;
+  const int64_t o = (int64_t)c * m;
+  double arr = 0.0, arz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock) {
+    const T ri = rhs[o + i];
+    r[o + i] = ri;
+    w[o + i] = (T)0;
+    T zi = ri;
+    if (d) {
+      const double di = d[i];
+      zi = di > 0.0 ? (T)((double)ri / di) : ri;
+      z[o + i] = zi;
+    }
+    p[o + i] = zi;
+    arr += (double)ri * (double)ri;
+    arz += (double)ri * (double)zi;
+  }
+  arr = block_sum(arr, lds);
+  arz = block_sum(arz, lds);
+  if (threadIdx.x == 0) {
+    part_rr[(int64_t)c * nb + blockIdx.x] = arr;
+    part_rz[(int64_t)c * nb + blockIdx.x] = arz;
+  }
+}
+
+__global__ void k_cg_init_finish(int s, const double *__restrict__ part_rr,
+                                 const double *__restrict__ part_rz, int nb,
+                                 CgState *__restrict__ st, int *__restrict__ act,
+                                 double rel_tol, int64_t maxit) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock) {
+    a += part_rr[(int64_t)c * nb + i];
+    b += part_rz[(int64_t)c * nb + i];
+  }
+  a = block_sum(a, lds);
+  b = block_sum(b, lds);
+  if (threadIdx.x == 0) {
+    CgState S{};
+    S.k = 0;
+    S.done = kRun;
+    S.reason = 0;
+    const double rhsn = sqrt(fmax(0.0, a));
+    S.rhsn = rhsn;
+    S.rnorm = rhsn;
+    if (!(rhsn > 0.0) || !isfinite(rhsn)) {
+      S.done = kDone;
+      S.reason = 1;
+    } else {
+      const double rz = b;
+      if (!(rz > 0.0) || !isfinite(rz)) {
+        S.done = kDone;
+        S.reason = 1;
+      } else {
+        S.rz = rz;
+        S.rnorm = sqrt(rz);
+        S.tol = fmax(rel_tol * S.rnorm, 1e-30);
+        if (S.rnorm <= S.tol) {
+          S.done = kDone;
+          S.reason = 0;
+        } else if (maxit <= 0) {
+          S.done = kDone;
+          S.reason = 4;
+        }
+      }
+    }
+    st[c] = S;
+    act[c] = S.done == kRun;
+  }
+}
+
+// pAp partials per column: p . Ap
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_pap(int64_t m, const T *__restrict__ p, const T *__restrict__ Ap,
+             const int *__restrict__ act, double *__restrict__ part, int nb) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.y;
+  if (!act[c])
+    return;
+  const int64_t o = (int64_t)c * m;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock)
+    acc += (double)p[o + i] * (double)Ap[o + i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    part[(int64_t)c * nb + blockIdx.x] = acc;
+}
+
+__global__ void k_cg_alpha(const double *__restrict__ part, int nb,
+                           CgState *__restrict__ st, int *__restrict__ act,
+                           double breakdown_tol) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.x;
+  if (!act[c])
+    return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    a += part[(int64_t)c * nb + i];
+  a = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    st[c].pAp = a;
+    if (!isfinite(a) || fabs(a) <= breakdown_tol) {
+      st[c].done = kDone;
+      st[c].reason = 2;
+      act[c] = 0;
+    }
+  }
+}
+
+// w += alpha p; r -= alpha Ap; z = M^{-1} r; partial r.z
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_update(int64_t m, T *__restrict__ w, T *__restrict__ r,
+                T *__restrict__ z, const T *__restrict__ p,
+                const T *__restrict__ Ap, const double *__restrict__ d,
+                const CgState *__restrict__ st, const int *__restrict__ act,
+                double *__restrict__ part, int nb) {
+#pragma clang fp contract(off)
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.y;
+  if (!act[c])
+    return;
+  const double alpha = st[c].rz / st[c].pAp;
+  const T a = (T)alpha, na = (T)(-alpha);
+  const int64_t o = (int64_t)c * m;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock) {
+    w[o + i] = w[o + i] + a * p[o + i];
+    const T ri = r[o + i] + na * Ap[o + i];
+    r[o + i] = ri;
+    T zi = ri;
+    if (d) {
+      const double di = d[i];
+      zi = di > 0.0 ? (T)((double)ri / di) : ri;
+      z[o + i] = zi;
+    }
+    acc += (double)ri * (double)zi;
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    part[(int64_t)c * nb + blockIdx.x] = acc;
+}
+
+__global__ void k_cg_beta(const double *__restrict__ part, int nb,
+                          CgState *__restrict__ st, int *__restrict__ act,
+                          int64_t maxit) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.x;
+  if (!act[c])
+    return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock)
+    a += part[(int64_t)c * nb + i];
+  a = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    CgState S = st[c];
+    S.rz_new = a;
+    if (!isfinite(a)) {
+      S.done = kDone;
+      S.reason = 3;
+    } else {
+      S.rnorm = sqrt(fmax(0.0, a));
+      if (S.rnorm <= S.tol) {
+        S.k += 1;
+        S.done = kDone;
+        S.reason = 0;
+      } else {
+        S.pAp = a / fmax(1e-300, S.rz);  // beta, consumed by k_cg_dir
+        S.rz = a;
+        S.k += 1;
+        if (S.k >= maxit) {
+          S.done = kDone;
+          S.reason = 4;
+        }
+      }
+    }
+    // p must still be updated for a column that stopped at maxit? No: the
+    // reference leaves the loop, so p is dead; act gates k_cg_dir.
+    st[c] = S;
+    act[c] = (S.done == kRun) ? 1 : (S.reason == 4 ? 2 : 0);
+  }
+}
+
+// p = beta p + z (columns still running; act==2 marks a maxit stop after the
+// update of w and r, where p is no longer used).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cg_dir(int64_t m, T *__restrict__ p, const T *__restrict__ z,
+             const CgState *__restrict__ st, int *__restrict__ act) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.y;
+  if (act[c] != 1)
+    return;
+  const T b = (T)st[c].pAp;
+  const int64_t o = (int64_t)c * m;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock)
+    p[o + i] = p[o + i] * b + z[o + i];
+}
+
+__global__ void k_act_normalise(int s, int *__restrict__ act) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < s && act[c] != 1)
+    act[c] = 0;
+}
+
+}  // namespace
+
+// =============================================================== launchers ==
+int reduce_blocks(int64_t len) {
+  int64_t g = (len + 2047) / 2048;
+  if (g < 1)
+    g = 1;
+  if (g > 1024)
+    g = 1024;
+  return (int)g;
+}
+
+template <class T>
+static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
+                        const int32_t *idx, const T *val, int lpr, const T *x,
+                        T *y, const T *z, const double *beta) {
+  hipStream_t s = ctx.stream;
+  if (rows <= 0)
+    return;
+  switch (lpr) {
+  case 1:
+    hipLaunchKernelGGL(k_spmv_lane<T>, dim3(grid_for(rows)), dim3(kBlock), 0, s,
+                       rows, ptr, idx, val, x, y, z, beta);
+    break;
+  case 4:
+    hipLaunchKernelGGL((k_spmv_group<T, 4>), dim3(grid_for(rows * 4)), dim3(kBlock), 0,
+                       s, rows, ptr, idx, val, x, y, z, beta);
+    break;
+  case 16:
+    hipLaunchKernelGGL((k_spmv_group<T, 16>), dim3(grid_for(rows * 16)), dim3(kBlock),
+                       0, s, rows, ptr, idx, val, x, y, z, beta);
+    break;
+  default:
+    hipLaunchKernelGGL((k_spmv_group<T, 64>), dim3(grid_for(rows * 64)), dim3(kBlock),
+                       0, s, rows, ptr, idx, val, x, y, z, beta);
+    break;
+  }
+  KH_CHECK(hipGetLastError());
+}
+
+void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
+          void *y, const void *z, const double *beta) {
+  const int64_t rows = trans ? A.n : A.m;
+  const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
+  const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
+  const int lpr = trans ? A.lpr_cols : A.lpr_rows;
+  if (p == Prec::F64)
+    launch_spmv<double>(ctx, rows, ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(),
+                        lpr, (const double *)x, (double *)y, (const double *)z, beta);
+  else
+    launch_spmv<float>(ctx, rows, ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(),
+                       lpr, (const float *)x, (float *)y, (const float *)z, beta);
+}
+
+template <class T>
+static void launch_spmm(Context &ctx, int64_t rows, const int32_t *ptr,
+                        const int32_t *idx, const T *val, int lpr, int s,
+                        const T *x, int64_t ldx, T *y, int64_t ldy,
+                        const int *act) {
+  hipStream_t st = ctx.stream;
+  if (rows <= 0 || s <= 0)
+    return;
+  if (lpr == 1)
+    hipLaunchKernelGGL(k_spmm_lane<T>, dim3(grid_for(rows)), dim3(kBlock), 0, st, rows,
+                       ptr, idx, val, s, x, ldx, y, ldy, act);
+  else if (lpr == 4)
+    hipLaunchKernelGGL((k_spmm_group<T, 4>), dim3(grid_for(rows * 4)), dim3(kBlock), 0,
+                       st, rows, ptr, idx, val, s, x, ldx, y, ldy, act);
+  else if (lpr == 16)
+    hipLaunchKernelGGL((k_spmm_group<T, 16>), dim3(grid_for(rows * 16)), dim3(kBlock),
+                       0, st, rows, ptr, idx, val, s, x, ldx, y, ldy, act);
+  else
+    hipLaunchKernelGGL((k_spmm_group<T, 64>), dim3(grid_for(rows * 64)), dim3(kBlock),
+                       0, st, rows, ptr, idx, val, s, x, ldx, y, ldy, act);
+  KH_CHECK(hipGetLastError());
+}
+
+void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
+          const void *x, int64_t ldx, void *y, int64_t ldy, const int *active) {
+  const int64_t rows = trans ? A.n : A.m;
+  const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
+  const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
+  const int lpr = trans ? A.lpr_cols : A.lpr_rows;
+  if (p == Prec::F64)
+    launch_spmm<double>(ctx, rows, ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(),
+                        lpr, s, (const double *)x, ldx, (double *)y, ldy, active);
+  else
+    launch_spmm<float>(ctx, rows, ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(),
+                       lpr, s, (const float *)x, ldx, (float *)y, ldy, active);
+}
+
+void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
+         int64_t len, double *out, bool do_sqrt) {
+  const int nb = reduce_blocks(len);
+  double *part = ctx.partials(nb);
+  hipStream_t s = ctx.stream;
+  if (pa == Prec::F64 && pb == Prec::F64)
+    hipLaunchKernelGGL((k_dot_partial<double, double>), dim3(nb), dim3(kBlock), 0, s, len,
+                       (const double *)a, (const double *)b, part);
+  else if (pa == Prec::F32 && pb == Prec::F32)
+    hipLaunchKernelGGL((k_dot_partial<float, float>), dim3(nb), dim3(kBlock), 0, s, len,
+                       (const float *)a, (const float *)b, part);
+  else if (pa == Prec::F64)
+    hipLaunchKernelGGL((k_dot_partial<double, float>), dim3(nb), dim3(kBlock), 0, s, len,
+                       (const double *)a, (const float *)b, part);
+  else
+    hipLaunchKernelGGL((k_dot_partial<float, double>), dim3(nb), dim3(kBlock), 0, s, len,
+                       (const float *)a, (const double *)b, part);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, part, nb, out, do_sqrt ? 1 : 0);
+  KH_CHECK(hipGetLastError());
+}
+
+template <class TV, class TW>
+static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
+                          int64_t len, double *out) {
+  constexpr int CT = 8;
+  const int nb = reduce_blocks(len);
+  double *part = ctx.partials((std::size_t)nb * s);
+  dim3 grid(nb, (s + CT - 1) / CT);
+  hipLaunchKernelGGL((k_gemv_t_partial<TV, TW, CT>), grid, dim3(kBlock), 0, ctx.stream,
+                     len, V, ld, s, w, part, nb);
+  hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
+            const void *w, int64_t len, double *out) {
+  if (s <= 0)
+    return;
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemv_t(ctx, (const double *)V, ld, s, (const double *)w, len, out);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemv_t(ctx, (const float *)V, ld, s, (const float *)w, len, out);
+  else if (pv == Prec::F64)
+    launch_gemv_t(ctx, (const double *)V, ld, s, (const float *)w, len, out);
+  else
+    launch_gemv_t(ctx, (const float *)V, ld, s, (const double *)w, len, out);
+}
+
+template <class TV, class TW>
+static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
+                              const double *h, TW *w, int64_t len, double *nrm) {
+  const int nb = reduce_blocks(len);
+  double *part = nrm ? ctx.partials(nb) : nullptr;
+  hipLaunchKernelGGL((k_gemv_n_sub<TV, TW>), dim3(nb), dim3(kBlock), 0, ctx.stream, len,
+                     V, ld, s, h, w, part);
+  if (nrm)
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, part, nb, nrm, 1);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
+                const double *h, Prec pw, void *w, int64_t len, double *nrm2_out) {
+  if (s <= 0 && !nrm2_out)
+    return;
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm2_out);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm2_out);
+  else if (pv == Prec::F64)
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm2_out);
+  else
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm2_out);
+}
+
+template <class TV, class TO>
+static void launch_gemm_nn(Context &ctx, const TV *V, int64_t ldv, int s, const double *H,
+                           int q, TO *out, int64_t ldo, int64_t len) {
+  dim3 grid(grid_for(len, 4), q);
+  hipLaunchKernelGGL((k_gemm_nn<TV, TO>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
+                     s, H, q, out, ldo);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemm_nn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
+             const double *H, int q, Prec po, void *OUT, int64_t ldo, int64_t len) {
+  if (q <= 0)
+    return;
+  if (pv == Prec::F64 && po == Prec::F64)
+    launch_gemm_nn(ctx, (const double *)V, ldv, s, H, q, (double *)OUT, ldo, len);
+  else if (pv == Prec::F32 && po == Prec::F32)
+    launch_gemm_nn(ctx, (const float *)V, ldv, s, H, q, (float *)OUT, ldo, len);
+  else if (pv == Prec::F32)
+    launch_gemm_nn(ctx, (const float *)V, ldv, s, H, q, (double *)OUT, ldo, len);
+  else
+    launch_gemm_nn(ctx, (const double *)V, ldv, s, H, q, (float *)OUT, ldo, len);
+}
+
+void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
+            Prec po, void *out, int64_t len) {
+  gemm_nn(ctx, pv, V, ld, s, h, 1, po, out, len, len);
+}
+
+template <class TV, class TW>
+static void launch_gemm_tn(Context &ctx, const TV *V, int64_t ldv, int s, const TW *W,
+                           int64_t ldw, int q, int64_t len, double *H) {
+  constexpr int CT = 8;
+  const int nb = reduce_blocks(len);
+  double *part = ctx.partials((std::size_t)nb * s * q);
+  dim3 grid(nb, (s + CT - 1) / CT, q);
+  hipLaunchKernelGGL((k_gemm_tn_partial<TV, TW, CT>), grid, dim3(kBlock), 0, ctx.stream,
+                     len, V, ldv, s, W, ldw, part, nb);
+  hipLaunchKernelGGL(k_finish, dim3(s * q), dim3(kBlock), 0, ctx.stream, part, nb, H, 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
+             const void *W, int64_t ldw, int q, int64_t len, double *H) {
+  if (s <= 0 || q <= 0)
+    return;
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemm_tn(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemm_tn(ctx, (const float *)V, ldv, s, (const float *)W, ldw, q, len, H);
+  else if (pv == Prec::F64)
+    launch_gemm_tn(ctx, (const double *)V, ldv, s, (const float *)W, ldw, q, len, H);
+  else
+    launch_gemm_tn(ctx, (const float *)V, ldv, s, (const double *)W, ldw, q, len, H);
+}
+
+template <class TV, class TW>
+static void launch_gemm_nn_sub(Context &ctx, const TV *V, int64_t ldv, int s,
+                               const double *H, int q, TW *W, int64_t ldw, int64_t len,
+                               double *norms) {
+  const int nb = reduce_blocks(len);
+  double *part = norms ? ctx.partials((std::size_t)nb * q) : nullptr;
+  hipLaunchKernelGGL((k_gemm_nn_sub<TV, TW>), dim3(nb, q), dim3(kBlock), 0, ctx.stream,
+                     len, V, ldv, s, H, W, ldw, part, nb);
+  if (norms)
+    hipLaunchKernelGGL(k_finish, dim3(q), dim3(kBlock), 0, ctx.stream, part, nb, norms, 1);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemm_nn_sub(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
+                 const double *H, int q, Prec pw, void *W, int64_t ldw, int64_t len,
+                 double *norms) {
+  if (q <= 0)
+    return;
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemm_nn_sub(ctx, (const double *)V, ldv, s, H, q, (double *)W, ldw, len, norms);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemm_nn_sub(ctx, (const float *)V, ldv, s, H, q, (float *)W, ldw, len, norms);
+  else if (pv == Prec::F64)
+    launch_gemm_nn_sub(ctx, (const double *)V, ldv, s, H, q, (float *)W, ldw, len, norms);
+  else
+    launch_gemm_nn_sub(ctx, (const float *)V, ldv, s, H, q, (double *)W, ldw, len, norms);
+}
+
+void col_norms(Context &ctx, Prec p, const void *W, int64_t ldw, int q, int64_t len,
+               double *out) {
+  if (q <= 0)
+    return;
+  const int nb = reduce_blocks(len);
+  double *part = ctx.partials((std::size_t)nb * q);
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_col_sq<double>, dim3(nb, q), dim3(kBlock), 0, ctx.stream, len,
+                       (const double *)W, ldw, part, nb);
+  else
+    hipLaunchKernelGGL(k_col_sq<float>, dim3(nb, q), dim3(kBlock), 0, ctx.stream, len,
+                       (const float *)W, ldw, part, nb);
+  hipLaunchKernelGGL(k_finish, dim3(q), dim3(kBlock), 0, ctx.stream, part, nb, out, 1);
+  KH_CHECK(hipGetLastError());
+}
+
+void scale_cols(Context &ctx, Prec p, int64_t len, int q, void *W, int64_t ldw,
+                const double *scale, bool skip_zero) {
+  if (q <= 0 || len <= 0)
+    return;
+  dim3 grid(grid_for(len, 4), q);
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_scale_cols<double>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (double *)W, ldw, scale, skip_zero ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_scale_cols<float>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (float *)W, ldw, scale, skip_zero ? 1 : 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void sub_cols(Context &ctx, Prec p, int64_t len, int q, const void *X, int64_t ldx,
+              const void *Y, int64_t ldy, void *OUT, int64_t ldo, const int *mask) {
+  if (q <= 0 || len <= 0)
+    return;
+  dim3 grid(grid_for(len, 4), q);
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_sub_cols<double>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (const double *)X, ldx, (const double *)Y, ldy, (double *)OUT, ldo,
+                       mask);
+  else
+    hipLaunchKernelGGL(k_sub_cols<float>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (const float *)X, ldx, (const float *)Y, ldy, (float *)OUT, ldo,
+                       mask);
+  KH_CHECK(hipGetLastError());
+}
+
+void axpby(Context &ctx, Prec p, int64_t len, double alpha, const void *x, double beta,
+           void *y) {
+  if (len <= 0)
+    return;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(len)), dim3(kBlock), 0, ctx.stream,
+                       len, alpha, (const double *)x, beta, (double *)y);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_for(len)), dim3(kBlock), 0, ctx.stream,
+                       len, alpha, (const float *)x, beta, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void scale_dev(Context &ctx, Prec p, int64_t len, const double *num, bool reciprocal,
+               void *y) {
+  if (len <= 0)
+    return;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_scale_dev<double>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, num, reciprocal ? 1 : 0, (double *)y);
+  else
+    hipLaunchKernelGGL(k_scale_dev<float>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, num, reciprocal ? 1 : 0, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void copy(Context &ctx, Prec p, int64_t len, const void *x, void *y) {
+  if (len <= 0 || x == y)
+    return;
+  KH_CHECK(hipMemcpyAsync(y, x, (std::size_t)len * (p == Prec::F64 ? 8 : 4),
+                          hipMemcpyDeviceToDevice, ctx.stream));
+}
+
+void convert(Context &ctx, Prec from, const void *x, Prec to, void *y, int64_t len) {
+  if (len <= 0)
+    return;
+  if (from == to) {
+    copy(ctx, from, len, x, y);
+    return;
+  }
+  if (from == Prec::F64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, (const double *)x, (float *)y);
+  else
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, (const float *)x, (double *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void fill_zero(Context &ctx, Prec p, int64_t len, void *y) {
+  if (len <= 0)
+    return;
+  axpby(ctx, p, len, 0.0, nullptr, 0.0, y);
+}
+
+void diag_solve(Context &ctx, Prec p, int64_t len, const double *d, const void *r,
+                void *y) {
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_diag_solve<double>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, d, (const double *)r, (double *)y);
+  else
+    hipLaunchKernelGGL(k_diag_solve<float>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, d, (const float *)r, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+// --------------------------------------------------------------------- CG --
+void CgBatch::ensure(int s_, int64_t m_, int64_t n_, Prec p_) {
+  const std::size_t e = p_ == Prec::F64 ? 8 : 4;
+  const std::size_t need_m = (std::size_t)s_ * (std::size_t)m_ * e;
+  const std::size_t need_n = (std::size_t)s_ * (std::size_t)n_ * e;
+  if (r.bytes < need_m)
+    r = DBuf(need_m);
+  if (z.bytes < need_m)
+    z = DBuf(need_m);
+  if (pv.bytes < need_m)
+    pv = DBuf(need_m);
+  if (Ap.bytes < need_m)
+    Ap = DBuf(need_m);
+  if (tmp_n.bytes < need_n)
+    tmp_n = DBuf(need_n);
+  if (st.bytes < (std::size_t)s_ * sizeof(CgState))
+    st = DBuf((std::size_t)s_ * sizeof(CgState));
+  if (act.bytes < (std::size_t)s_ * sizeof(int))
+    act = DBuf((std::size_t)s_ * sizeof(int));
+  s = s_;
+  m = m_;
+  n = n_;
+  p = p_;
+}
+
+template <class T>
+static void cg_init_t(Context &ctx, CgBatch &B, const T *rhs, T *w, const double *d,
+                      double rel_tol, int64_t maxit) {
+  const int nb = reduce_blocks(B.m);
+  double *part = ctx.partials((std::size_t)nb * B.s * 2);
+  dim3 grid(nb, B.s);
+  T *z = d ? B.z.as<T>() : B.r.as<T>();
+  hipLaunchKernelGGL(k_cg_init<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, rhs, w,
+                     B.r.as<T>(), z, B.pv.as<T>(), d, part, part + (std::size_t)nb * B.s,
+                     nb);
+  hipLaunchKernelGGL(k_cg_init_finish, dim3(B.s), dim3(kBlock), 0, ctx.stream, B.s, part,
+                     part + (std::size_t)nb * B.s, nb, B.st.as<CgState>(), B.act.as<int>(),
+                     rel_tol, maxit);
+  KH_CHECK(hipGetLastError());
+}
+
+void cg_init(Context &ctx, CgBatch &B, const void *rhs, void *w, const double *diag,
+             double rel_tol, int64_t maxit) {
+  B.have_prec = diag != nullptr;
+  if (B.p == Prec::F64)
+    cg_init_t(ctx, B, (const double *)rhs, (double *)w, diag, rel_tol, maxit);
+  else
+    cg_init_t(ctx, B, (const float *)rhs, (float *)w, diag, rel_tol, maxit);
+}
+
+template <class T>
+static void cg_post_t(Context &ctx, CgBatch &B, T *w, const double *d, int64_t maxit,
+                      double bt) {
+  const int nb = reduce_blocks(B.m);
+  double *part = ctx.partials((std::size_t)nb * B.s);
+  dim3 grid(nb, B.s);
+  T *r = B.r.as<T>(), *p = B.pv.as<T>(), *Ap = B.Ap.as<T>();
+  T *z = d ? B.z.as<T>() : r;
+  int *act = B.act.as<int>();
+  CgState *st = B.st.as<CgState>();
+  hipLaunchKernelGGL(k_cg_pap<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, p, Ap, act, part,
+                     nb);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(B.s), dim3(kBlock), 0, ctx.stream, part, nb, st, act,
+                     bt);
+  hipLaunchKernelGGL(k_cg_update<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, w, r, z, p, Ap,
+                     d, st, act, part, nb);
+  hipLaunchKernelGGL(k_cg_beta, dim3(B.s), dim3(kBlock), 0, ctx.stream, part, nb, st, act,
+                     maxit);
+  hipLaunchKernelGGL(k_cg_dir<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, p, z, st, act);
+  hipLaunchKernelGGL(k_act_normalise, dim3((B.s + 63) / 64), dim3(64), 0, ctx.stream, B.s,
+                     act);
+  KH_CHECK(hipGetLastError());
+}
+
+void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
+                       int64_t maxit, double breakdown_tol) {
+  if (B.p == Prec::F64)
+    cg_post_t(ctx, B, (double *)w, diag, maxit, breakdown_tol);
+  else
+    cg_post_t(ctx, B, (float *)w, diag, maxit, breakdown_tol);
+}
+
+}  // namespace kh

@@ -1,0 +1,154 @@
+// Device-resident null-space solvers of the KraNS-MI engine: the operator
+// abstraction they run on, the reference-compatible random stream, options,
+// results and the two drivers (cyclic Lanczos / Golub-Kahan and LOBPCG).
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace kh {
+
+// ---------------------------------------------------------------------------
+// A linear operator on the device: A (rows x cols) with A x, A^T x and their
+// multi-column forms. `mv` optionally fuses y = A x - (*beta) z.
+struct DeviceOp {
+  virtual ~DeviceOp() = default;
+  virtual int64_t rows() const = 0;
+  virtual int64_t cols() const = 0;
+  virtual void mv(Context &c, Prec p, const void *x, void *y, const void *z = nullptr,
+                  const double *beta = nullptr) const = 0;
+  virtual void mtv(Context &c, Prec p, const void *x, void *y, const void *z = nullptr,
+                   const double *beta = nullptr) const = 0;
+  virtual void mm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y,
+                  int64_t ldy, const int *act) const = 0;
+  virtual void mtm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y,
+                   int64_t ldy, const int *act) const = 0;
+  // diag(A A^T) on the device (Jacobi preconditioner), or nullptr.
+  virtual const double *diag_AAt() const { return nullptr; }
+  virtual double fro() const { return 0.0; }
+};
+
+// CSR matrix (or its transpose view, which swaps the CSR and CSC roles).
+struct CsrOp final : DeviceOp {
+  const DeviceCsr &A;
+  bool t;
+  CsrOp(const DeviceCsr &a, bool transposed) : A(a), t(transposed) {}
+  int64_t rows() const override { return t ? A.n : A.m; }
+  int64_t cols() const override { return t ? A.m : A.n; }
+  void mv(Context &c, Prec p, const void *x, void *y, const void *z,
+          const double *beta) const override {
+    spmv(c, A, t, p, x, y, z, beta);
+  }
+  void mtv(Context &c, Prec p, const void *x, void *y, const void *z,
+           const double *beta) const override {
+    spmv(c, A, !t, p, x, y, z, beta);
+  }
+  void mm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y, int64_t ldy,
+          const int *act) const override {
+    spmm(c, A, t, p, s, x, ldx, y, ldy, act);
+  }
+  void mtm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y, int64_t ldy,
+           const int *act) const override {
+    spmm(c, A, !t, p, s, x, ldx, y, ldy, act);
+  }
+  const double *diag_AAt() const override { return t ? nullptr : A.diag_AAt.as<double>(); }
+  double fro() const override { return A.fro; }
+};
+
+// ---------------------------------------------------------------------------
+// The reference's random stream: std::mt19937(seed) feeding one
+// std::normal_distribution<double> whose cached second draw persists across
+// calls (StdVectorFactory.hpp:16-58, EigenVectorFactory.hpp:13-38). Vectors
+// are generated on the host, in the reference's order, and uploaded.
+struct RefStream {
+  std::mt19937 gen;
+  std::normal_distribution<double> dist;
+  explicit RefStream(unsigned seed) : gen(seed), dist(0.0, 1.0) {}
+  void fill(double *out, std::size_t n) {
+    for (std::size_t i = 0; i < n; ++i)
+      out[i] = dist(gen);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Options: field for field CyclicLanczosOptions (CyclicLanczosSolver.hpp:84-146)
+// minus the two non-POD members, which are passed separately.
+struct CyclicParams {
+  int64_t k = 1, ncv = 48, max_restarts = 80;
+  double tol_abs = 1e-12, tol_rel = 1e-10;
+  int reorth = 2;
+  bool use_harmonic = true;
+  double target_tau = 0.0;
+  int nullity_known = -1, rank_known = -1;
+  int64_t cg_maxit = 80;
+  double cg_rel_tol = 1e-10, cg_breakdown_tol = 1e-30;
+  int projector_passes = 1, grad_max_iters = 3000, frob_samples = 4;
+  double prescreen_factor = 0.8;
+  int64_t stagnation_restarts = 30;
+  bool return_left_space = false, verbose = false;
+  // engine choices (not in the reference struct)
+  bool jacobi_AAt = false;  // M^{-1} = diag(A A^T)^{-1} in the polish CG
+  Prec krylov = Prec::F64;  // Golub-Kahan vectors and products
+  Prec polish = Prec::F64;  // CG of the projector polish (F32 = refinement)
+  const double *y0_host = nullptr;  // initial_right_space[0], host, length n
+};
+
+struct PhaseTimes {
+  double gk = 0, tiny = 0, cand = 0, polish = 0, fallback = 0, rng = 0;
+};
+
+struct CyclicStats {
+  int64_t restarts = 0, krylov_steps_total = 0;
+  double frob_norm_est = 0.0;
+  bool converged_to_target = false;
+  int64_t cg_solves = 0, cg_maxit_hits = 0;
+  int64_t spmv = 0, spmtv = 0;    // operator applications (column count)
+  int64_t cg_iters = 0;           // CG iterations over all columns
+  PhaseTimes t;
+};
+
+// A device block of `count` FP64 vectors of length `len` (column-major).
+struct DeviceBasis {
+  int64_t len = 0, count = 0, cap = 0;
+  DBuf buf;
+  double *col(int64_t j) const { return buf.as<double>() + j * len; }
+};
+
+// Right null space of op; left space computed by the caller with the
+// transposed operator.
+DeviceBasis cyclic_right(Context &ctx, const DeviceOp &op, RefStream &rng,
+                         const CyclicParams &P, CyclicStats &S);
+
+// LOBPCGOptions (LOBPCGSolver.hpp:71-115).
+struct LobpcgParams {
+  int64_t nev = 1, block_size = 0, max_iters = 1000;
+  double tol_abs = 1e-12, tol_rel = 1e-10;
+  bool soft_locking = true;
+  double restart = 0.9, gs_tol = 1e-14;
+  bool gs_reorth = true;
+  int frob_samples = 4;
+  int64_t max_tiny_dim = 0;
+  int nullity_known = -1, rank_known = -1;
+  bool verbose = false;
+  int64_t progress_every = 100;
+  Prec block = Prec::F64;  // X/W/P blocks and their products
+  const double *initial_block_host = nullptr;  // n x n_initial, column-major
+  int64_t n_initial = 0;
+};
+
+struct LobpcgStats {
+  std::vector<double> theta, residuals_Av;
+  int64_t iters = 0, locked = 0, block_size = 0;
+  double frob_norm_est = 0, accept_threshold = 0, best_unaccepted_residual = 0;
+  bool converged_to_target = false;
+  int64_t spmv = 0, spmtv = 0;
+  double t_gram = 0, t_sparse = 0, t_host = 0;
+};
+
+DeviceBasis lobpcg(Context &ctx, const DeviceOp &op, RefStream &rng,
+                   const LobpcgParams &P, LobpcgStats &S);
+
+}  // namespace kh

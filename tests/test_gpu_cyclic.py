@@ -1,0 +1,195 @@
+"""GPU parity of the cyclic Lanczos hot path, through the C ABI.
+
+Checked against the CPU oracle (oracle/, an FP64 restatement of
+src/Solvers/CyclicLanczosSolver.cpp) on the same seeded inputs, and against
+the exact kernels of the generators. Tolerances are stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import krans_amd as K
+from krans_amd import workloads as W
+from conftest import principal_sine
+from helpers import csr_identity, csr_ones_2x2, csr_small_2x3, residuals, scaled, random_sparse
+
+pytestmark = pytest.mark.gpu
+
+
+def hip(A):
+    return K.HipCsrMatrixAdapter.from_csr(A)
+
+
+def ora(A):
+    return O.Operator.csr(A.m, A.n, A.rowptr, A.col, A.val)
+
+
+# ------------------------------------------------------------ structure ----
+@pytest.mark.parametrize("case", ["incidence", "random", "shiftdiff"])
+def test_csr_csc_twin_bit_exact(case):
+    if case == "incidence":
+        A = W.regular_incidence(4, 512, seed=7)
+    elif case == "random":
+        A = random_sparse(300, 200, 7, seed=3)
+    else:
+        A = W.shifted_difference(1000, 3, 0.5)
+    print(f"[case] csc twin {case} m={A.m} n={A.n} nnz={A.nnz}", flush=True)
+    H = hip(A)
+    rp, ci, va = H.download_csr()
+    assert np.array_equal(rp, A.rowptr) and np.array_equal(ci, A.col)
+    assert np.array_equal(va.view(np.uint64), A.val.view(np.uint64))
+    cp, ri, vt, perm = H.download_csc()
+    ocp, ori, ovt, operm = ora(A).csc(A.nnz)
+    assert np.array_equal(cp, ocp) and np.array_equal(ri, ori)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(vt.view(np.uint64), ovt.view(np.uint64))
+    # and against scipy's transpose (rows ascend within each column)
+    T = A.to_scipy().T.tocsr()
+    T.sort_indices()
+    assert np.array_equal(cp, T.indptr) and np.array_equal(ri, T.indices)
+    assert np.array_equal(vt, T.data)
+    assert np.array_equal(A.val[perm], vt)
+
+
+# ----------------------------------------------------------------- SpMV ----
+@pytest.mark.parametrize("case", ["incidence", "random", "shiftdiff"])
+def test_spmv_fp64_bit_exact(case):
+    if case == "incidence":
+        A = W.regular_incidence(4, 512, seed=7)
+    elif case == "random":
+        A = random_sparse(300, 200, 7, seed=3)
+    else:
+        A = W.shifted_difference(1000, 3, 0.5)
+    print(f"[case] spmv {case}", flush=True)
+    H, Op = hip(A), ora(A)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(A.n)
+    y = rng.standard_normal(A.m)
+    # one lane per row: the reference's sequential ascending order, no FMA
+    assert np.array_equal(H.multiply(x).view(np.uint64), Op.mv(x).view(np.uint64))
+    assert np.array_equal(H.multiply_transpose(y).view(np.uint64), Op.mtv(y).view(np.uint64))
+    # FP32 operator: within FP32 rounding of the row sums
+    S = A.to_scipy()
+    bound = 4 * np.finfo(np.float32).eps * (abs(S) @ np.abs(x) + 1e-30)
+    assert np.all(np.abs(H.multiply(x, "fp32") - S @ x) <= 8 * bound + 1e-30)
+
+
+def test_spmv_long_rows_tolerance():
+    # rows longer than the one-lane threshold use lane groups: not the
+    # sequential order, so compare within 1e-14 relative to sum |a||x|
+    A = random_sparse(64, 4000, 300, seed=5)
+    print("[case] spmv long rows", flush=True)
+    H, Op = hip(A), ora(A)
+    x = np.random.default_rng(2).standard_normal(A.n)
+    ref = Op.mv(x)
+    scale = abs(A.to_scipy()) @ np.abs(x)
+    assert np.all(np.abs(H.multiply(x) - ref) <= 1e-14 * scale)
+
+
+# ----------------------------------------------------- solver scenarios ----
+def test_quickstart_as_csr_fp64_matches_oracle():
+    """Config C1 with the stencil materialised as CSR (bit-identical products,
+    see the stencil/CSR test); same seed 42 factory, k=3."""
+    n, N, c = 1000, 3, 0.5
+    A = W.shifted_difference(n, N, c)
+    Kx = W.shifted_difference_kernel(n, N, c)
+    print("[case] quickstart cyclic fp64", flush=True)
+    H = hip(A)
+    vf = K.HipVectorFactory(n, 42)
+    r = K.find_nullspace_cyclic_lanczos_ex(H, vf, K.CyclicLanczosOptions(k=3))
+    U = r.right_null
+    assert U.shape[1] == 3 and r.converged_to_target
+    thr = max(1e-12, 1e-10 * max(1.0, r.frob_norm_est))
+    assert residuals(A, U).max() <= thr
+    # README.md:76-80: max dist(u, ker A) 7.6e-16 -- allow the 1e-8 of
+    # examples/quickstart.cpp:65
+    assert principal_sine(U, Kx) <= 1e-8
+    assert np.abs(U.T @ U - np.eye(3)).max() <= 1e-12
+    Uo, _, ro = O.cyclic(O.Operator.shiftdiff(n, N, c), O.Factory(42), O.cyclic_defaults(k=3))
+    # same random stream, same norm estimate to rounding
+    assert abs(r.frob_norm_est - ro.frob_norm_est) <= 1e-12 * ro.frob_norm_est
+    assert O.subspace_diff(U, Uo, 1e-12) <= 1e-8
+    print("gpu", r.restarts, r.krylov_steps_total, r.cg_solves, "oracle", ro.restarts,
+          ro.krylov_steps_total, ro.cg_solves)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_incidence_small_matches_exact_kernel(precision):
+    g, v, k = 8, 1024, 8
+    A = W.regular_incidence(g, v, seed=42)
+    Kx = W.component_indicators(g, v)
+    print(f"[case] incidence {g}x{v} {precision}", flush=True)
+    H = hip(A)
+    vf = K.HipVectorFactory(A.n, 42)
+    opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3)
+    r = K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision=precision)
+    U = r.right_null
+    assert U.shape[1] == k and r.converged_to_target
+    thr = 1e-12 * r.frob_norm_est
+    res = residuals(A, U)
+    assert res.max() <= thr, (res.max(), thr)
+    # ||A u|| <= thr and sigma_min(A) >= 0.5 bound the angle
+    assert principal_sine(U, Kx) <= 5e-9
+    assert np.abs(U.T @ U - np.eye(k)).max() <= 1e-10
+    Uo, _, ro = O.cyclic(ora(A), O.Factory(42), O.cyclic_defaults(
+        k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3))
+    assert Uo.shape[1] == k
+    assert principal_sine(U, Uo) <= 1e-8
+    print(precision, "gpu restarts", r.restarts, "cg", r.cg_solves, "| oracle", ro.restarts,
+          ro.cg_solves)
+
+
+def test_small_2x3_analytic():
+    # tests/test_cyclic_lanczos.cpp:144-190
+    A = csr_small_2x3()
+    print("[case] 2x3", flush=True)
+    opt = K.CyclicLanczosOptions(k=1, ncv=20, max_restarts=50, nullity_known=1, rank_known=2,
+                                 cg_maxit=40)
+    U = K.find_nullspace_cyclic_lanczos(hip(A), K.HipVectorFactory(3, 1), opt)
+    assert U.shape[1] == 1
+    assert abs(np.linalg.norm(U[:, 0]) - 1) <= 1e-12
+    assert residuals(A, U)[0] <= 1e-10
+    e = np.array([-1.0, -1.0, 1.0]) / np.sqrt(3)
+    assert principal_sine(U, e[:, None]) <= 1e-10
+
+
+def test_identity_has_no_null_space():
+    # tests/test_cyclic_lanczos.cpp:268-300
+    A = csr_identity(3)
+    print("[case] identity", flush=True)
+    opt = K.CyclicLanczosOptions(k=1, ncv=20, max_restarts=30, nullity_known=0, rank_known=3)
+    U = K.find_nullspace_cyclic_lanczos(hip(A), K.HipVectorFactory(3, 1), opt)
+    assert U.shape[1] == 0
+
+
+def test_rank1_left_and_right():
+    # tests/test_cyclic_lanczos.cpp:430-488
+    A = csr_ones_2x2()
+    print("[case] 2x2 rank-1", flush=True)
+    opt = K.CyclicLanczosOptions(k=1, ncv=20, max_restarts=50, nullity_known=1, rank_known=1,
+                                 return_left_space=True)
+    r = K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(2, 1), opt)
+    e = np.array([-1.0, 1.0]) / np.sqrt(2)
+    assert r.right_null.shape[1] == 1 and r.left_null.shape[1] == 1
+    assert principal_sine(r.right_null, e[:, None]) <= 1e-10
+    assert principal_sine(r.left_null, e[:, None]) <= 1e-10
+
+
+def test_scale_invariance_1e6():
+    # tests/test_cyclic_lanczos.cpp:490-542: the same subspace for A and 1e6 A
+    A = W.regular_incidence(2, 256, seed=3)
+    print("[case] scale 1e6", flush=True)
+    opt = K.CyclicLanczosOptions(k=2, nullity_known=2)
+    U1 = K.find_nullspace_cyclic_lanczos(hip(A), K.HipVectorFactory(A.n, 1), opt)
+    U2 = K.find_nullspace_cyclic_lanczos(hip(scaled(A, 1e6)), K.HipVectorFactory(A.n, 1), opt)
+    assert U1.shape[1] == 2 and U2.shape[1] == 2
+    assert principal_sine(U1, U2) <= 1e-8
+
+
+def test_backend_mismatch_raises():
+    A = csr_small_2x3()
+    with pytest.raises(ValueError):
+        K.find_nullspace_cyclic_lanczos(object(), K.HipVectorFactory(3, 1),
+                                        K.CyclicLanczosOptions())
+    with pytest.raises(ValueError):
+        hip(A).multiply(np.zeros(2))
