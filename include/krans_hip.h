@@ -143,6 +143,14 @@ int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out);
 int kh_dev_free(kh_context *ctx, void *p);
 int kh_h2d(kh_context *ctx, void *dst, const void *src, size_t bytes);
 int kh_d2h(kh_context *ctx, void *dst, const void *src, size_t bytes);
+/* y = alpha x + beta y on FP64 device vectors; x may be NULL (y = beta y,
+ * beta = 0 writes zeros) -- the level-1 work of VectorInterface
+ * (VectorInterface.hpp:11-27) for HipVector. */
+int kh_vec_axpby(kh_context *ctx, int64_t len, double alpha, const double *x,
+                 double beta, double *y);
+/* y = r ./ diag(A A^T) (rows with a zero diagonal copy r): the Jacobi
+ * preconditioner for CyclicLanczosOptions::precond_AAt. Device FP64. */
+int kh_matrix_jacobi_AAt(kh_matrix *A, const double *r, double *y);
 /* y = op(A) x on device vectors, s columns (ld = vector length) */
 int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s,
                 const void *x, void *y);

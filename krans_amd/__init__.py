@@ -107,6 +107,8 @@ SIGNATURES = {
     "kh_dev_free": (I32, [P, P]),
     "kh_h2d": (I32, [P, P, P, SZ]),
     "kh_d2h": (I32, [P, P, P, SZ]),
+    "kh_vec_axpby": (I32, [P, I64, D, P, D, P]),
+    "kh_matrix_jacobi_AAt": (I32, [P, P, P]),
     "kh_spmm_dev": (I32, [P, I32, I32, I32, P, P]),
     "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),

@@ -16,6 +16,7 @@ ARCH = os.environ.get("KRANS_HIP_ARCH", "gfx950")
 
 CXXFLAGS = [
     "-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result",
+    "-I" + os.path.join(os.path.dirname(HERE), "include"),
     f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off (hipcc only; the host pass warns)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -5,20 +5,9 @@
 #include <string>
 
 #include "../../include/krans_hip.h"
+#include "capi_internal.hpp"
 #include "dense.hpp"
 #include "solver.hpp"
-
-struct kh_context_s {
-  std::unique_ptr<kh::Context> ctx;
-};
-struct kh_matrix_s {
-  kh_context *owner = nullptr;
-  std::unique_ptr<kh::DeviceCsr> A;
-};
-struct kh_rng_s {
-  kh::RefStream s;
-  explicit kh_rng_s(unsigned seed) : s(seed) {}
-};
 
 namespace {
 thread_local std::string g_err;
@@ -385,6 +374,25 @@ int kh_d2h(kh_context *ctx, void *dst, const void *src, size_t bytes) {
     need(ctx, "null context");
     KH_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->ctx->stream));
     ctx->ctx->sync();
+  });
+}
+
+int kh_vec_axpby(kh_context *ctx, int64_t len, double alpha, const double *x, double beta,
+                 double *y) {
+  return guard([&] {
+    need(ctx && y && len >= 0, "bad argument");
+    auto &c = *ctx->ctx;
+    kh::axpby(c, kh::Prec::F64, len, alpha, x, beta, y);
+    c.sync();
+  });
+}
+
+int kh_matrix_jacobi_AAt(kh_matrix *A, const double *r, double *y) {
+  return guard([&] {
+    need(A && r && y, "null argument");
+    auto &c = *A->owner->ctx;
+    kh::diag_solve(c, kh::Prec::F64, A->A->m, A->A->diag_AAt.as<double>(), r, y);
+    c.sync();
   });
 }
 
