@@ -40,6 +40,10 @@ def parse():
     return ap.parse_args()
 
 
+PRESCREEN = 1e-4
+MAX_RESTARTS = 400
+
+
 def spmv_bytes(A, elt):
     """Algorithmic bytes of one CSR gather SpMV: values + indices + row
     pointers + x once + y once."""
@@ -60,7 +64,8 @@ def main():
     H = K.HipCsrMatrixAdapter.from_csr(A)
     k = args.graphs
     setup_s = time.time() - t0
-    opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3)
+    opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
+                                 max_restarts=MAX_RESTARTS)
     S = A.to_scipy()
 
     def check(U):
@@ -143,7 +148,8 @@ def main():
             "workload": "cyclic Lanczos k=64 on the oriented incidence of 64 random 4-regular "
                         "graphs x 16384 vertices (m=2097152, n=1048576, nnz=4194304)",
             "graphs": args.graphs, "verts": args.verts, "k": k,
-            "tol_rel": 1e-12, "tol_abs": 0.0, "prescreen_factor": 1e-3,
+            "tol_rel": 1e-12, "tol_abs": 0.0, "prescreen_factor": PRESCREEN,
+            "max_restarts": MAX_RESTARTS,
         },
         "latency_s": {"p50": round(pct(lat_m, 0.5), 4), "p99": round(pct(lat_m, 0.99), 4)},
         "fp64": {
@@ -177,19 +183,23 @@ def cpu_baseline(budget_s):
     from krans_amd import workloads as W
     g = 2
     t_total, vecs, runs = 0.0, 0, 0
+    # same prescreen threshold in absolute terms (0.2) at this smaller ||A||_F
+    pf = PRESCREEN * 2048.0 / np.sqrt(2.0 * 2 * g * 16384)
     while t_total < budget_s and runs < 8:
         A = W.regular_incidence(g, 16384, seed=42 + runs)
         op = O.Operator.csr(A.m, A.n, A.rowptr, A.col, A.val)
         t = time.perf_counter()
         U, _, _ = O.cyclic(op, O.Factory(42), O.cyclic_defaults(
-            k=g, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3))
+            k=g, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=pf, max_restarts=MAX_RESTARTS))
         t_total += time.perf_counter() - t
         vecs += U.shape[1]
         runs += 1
     return {"value": round(vecs / t_total, 4), "unit": "null vectors/s", "cores": 1,
             "kind": "port",
             "sample": f"{runs} oracle solves of the same generator with {g} graphs x 16384 "
-                      f"vertices, k={g}, same options ({t_total:.1f} s single-threaded)"}
+                      f"vertices, k={g}, same options, prescreen threshold 0.2 "
+                      f"({t_total:.1f} s single-threaded); per-vector CPU cost grows with "
+                      f"n and k, so this overstates the full-size CPU rate"}
 
 
 if __name__ == "__main__":

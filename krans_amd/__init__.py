@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libkrans_hip.so")
 
 KH_OK, KH_EINVAL, KH_ERUNTIME, KH_ENOMEM, KH_EDEVICE = 0, 1, 2, 3, 4
-PREC = {"fp64": 0, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "fp32": 1}
 
 
 class _CyclicOpts(C.Structure):
