@@ -95,8 +95,11 @@ private:
 
   // device state
   DeviceBasis basis_;   // FP64 locked basis (n x k_target)
-  DBuf basis32_;        // FP32 shadow of the basis (mixed mode)
-  DBuf Y_, X_, wx_, wy_;
+  // [Y reversed | basis] in Krylov precision: Golub-Kahan vector y_j sits in
+  // column ycap-1-j and locked vector b_i in column ycap+i, so the y-side
+  // reorthogonalisation set [basis, Y] is one contiguous block.
+  DBuf VB_;
+  DBuf X_, wx_, wy_;
   DBuf y0_, best_v_, tmp_m_, tmp_n_;
   DBuf small_;          // device scalars of this solve
   double *bnorm_ = nullptr, *h_ = nullptr, *wsc_ = nullptr, *C_ = nullptr, *H_ = nullptr;
@@ -143,9 +146,7 @@ void CyclicDriver::alloc() {
   basis_.cap = k_target_;
   basis_.count = 0;
   basis_.buf = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 8);
-  if (kp_ == Prec::F32)
-    basis32_ = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 4);
-  Y_ = DBuf((std::size_t)ycap_ * n_ * esize(kp_));
+  VB_ = DBuf((std::size_t)(ycap_ + std::max<int64_t>(1, k_target_)) * n_ * esize(kp_));
   X_ = DBuf((std::size_t)xcap_ * m_ * esize(kp_));
   wx_ = DBuf((std::size_t)m_ * esize(kp_));
   wy_ = DBuf((std::size_t)n_ * esize(kp_));
@@ -270,8 +271,8 @@ void CyclicDriver::push_basis(const double *v) {
   const int64_t j = basis_.count;
   KH_CHECK(hipMemcpyAsync(basis_.col(j), v, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
                           c_.stream));
-  if (kp_ == Prec::F32)
-    convert(c_, Prec::F64, v, Prec::F32, basis32_.as<float>() + j * n_, n_);
+  convert(c_, Prec::F64, v, kp_,
+          static_cast<char *>(VB_.p) + (std::size_t)(ycap_ + j) * n_ * esize(kp_), n_);
   basis_.count = j + 1;
 }
 
@@ -528,12 +529,11 @@ DeviceBasis CyclicDriver::run() {
     gs_basis_vec(y0);
   };
 
-  char *Yb = static_cast<char *>(Y_.p), *Xb = static_cast<char *>(X_.p);
-  auto Ycol = [&](int64_t j) { return (void *)(Yb + (std::size_t)j * n_ * ke); };
+  char *VBb = static_cast<char *>(VB_.p), *Xb = static_cast<char *>(X_.p);
+  auto Ycol = [&](int64_t j) { return (void *)(VBb + (std::size_t)(ycap_ - 1 - j) * n_ * ke); };
   auto Xcol = [&](int64_t j) { return (void *)(Xb + (std::size_t)j * m_ * ke); };
-  const void *basisK = kp == Prec::F64 ? (const void *)basis_.col(0) : basis32_.p;
 
-  std::vector<double> alpha, beta;
+  std::vector<double> alpha, beta, bh((std::size_t)ncv);
   int64_t restarts_since_accept = 0;
   for (int64_t restart = 0; restart < P_.max_restarts; ++restart) {
     S_.restarts = restart;
@@ -551,15 +551,20 @@ DeviceBasis CyclicDriver::run() {
     }
     axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / ny, y0);
     convert(c_, Prec::F64, y0, kp, Ycol(0), n_);
-    int64_t nyc = 1, nxc = 0, local_steps = 0;
-    double beta_last = 0.0;
+    // ---- Golub-Kahan build (:387-421), queued without host round trips:
+    // beta_j stays on the device (bnorm_[j]) where the next step's -beta z
+    // term and the w / beta store read it. The reference's stop test
+    // (non-finite or < 1e-14) is replayed on the host afterwards; steps
+    // queued after a stop only write columns that are then ignored.
+    const int nb = (int)basis_.count;
+    int64_t nyc = 1, nxc = 0;
     for (int64_t j = 0; j < ncv; ++j) {
       const bool from_y = nyc > nxc;
       void *w = from_y ? wx_.p : wy_.p;
       const int64_t len = from_y ? m_ : n_;
       if (from_y) {
-        A_.mv(c_, kp, Ycol(nyc - 1), w, beta.empty() ? nullptr : Xcol(nxc - 1),
-              beta.empty() ? nullptr : bnorm_ + (j - 1));
+        A_.mv(c_, kp, Ycol(nyc - 1), w, j == 0 ? nullptr : Xcol(nxc - 1),
+              j == 0 ? nullptr : bnorm_ + (j - 1));
         ++S_.spmv;
       } else {
         A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
@@ -569,26 +574,31 @@ DeviceBasis CyclicDriver::run() {
       std::vector<Group> groups;
       if (from_y)
         groups.push_back({Xb, m_, (int)nxc, kp});
-      else {
-        groups.push_back({basisK, n_, (int)basis_.count, kp});
-        groups.push_back({Yb, n_, (int)nyc, kp});
-      }
+      else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
+        groups.push_back({Ycol(nyc - 1), n_, (int)nyc + nb, kp});
       for (int pass = 0; pass < passes; ++pass)
         cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
       if (passes <= 0)
         dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
-      double bj;
-      readback(c_, bnorm_ + j, 1, &bj);
-      alpha.push_back(0.0);
-      beta_last = bj;
-      if (!std::isfinite(bj) || bj < 1e-14 || nyc + nxc >= ncv)
+      if (nyc + nxc >= ncv)
         break;
-      beta.push_back(bj);
-      axpby(c_, kp, len, 1.0 / bj, w, 0.0, from_y ? Xcol(nxc) : Ycol(nyc));
+      scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
       if (from_y)
         ++nxc;
       else
         ++nyc;
+    }
+    const int64_t queued = nyc + nxc;  // steps queued (the last one stores nothing)
+    readback(c_, bnorm_, (int)queued, bh.data());
+    double beta_last = 0.0;
+    int64_t local_steps = 0;
+    for (int64_t j = 0; j < queued; ++j) {
+      const double bj = bh[(std::size_t)j];
+      alpha.push_back(0.0);
+      beta_last = bj;
+      if (!std::isfinite(bj) || bj < 1e-14 || j + 1 >= ncv)
+        break;
+      beta.push_back(bj);
       ++local_steps;
     }
     S_.krylov_steps_total += local_steps;
@@ -672,11 +682,13 @@ DeviceBasis CyclicDriver::run() {
       for (int c = 0; c < wn; ++c) {
         const int jb = cl[pos + (std::size_t)c];
         for (int64_t i = 0; i < l; i += 2)
-          Ch[(std::size_t)c * ny_used + (std::size_t)(i / 2)] = V((int)i, jb);
+          Ch[(std::size_t)c * ny_used + (std::size_t)(ny_used - 1 - i / 2)] = V((int)i, jb);
       }
       upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
       double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
-      gemm_nn(c_, kp, Yb, n_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
+      // Y_{ny_used-1} .. Y_0 are contiguous (reversed), so the coefficients
+      // are stored in reverse order
+      gemm_nn(c_, kp, Ycol(ny_used - 1), n_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
       gs_basis_block(Wv, wn);
       col_norms(c_, Prec::F64, Wv, n_, wn, n_, wsc_);
       std::vector<double> nv(wn);

@@ -148,6 +148,9 @@ void axpby(Context &ctx, Prec p, int64_t len, double alpha, const void *x,
 void scale_dev(Context &ctx, Prec p, int64_t len, const double *num,
                bool reciprocal, void *y);  // y *= (*num) or y *= 1/(*num)
 void copy(Context &ctx, Prec p, int64_t len, const void *x, void *y);
+// y = x * (1 / *den) with den on the device
+void scale_copy_dev(Context &ctx, Prec p, int64_t len, const double *den, const void *x,
+                    void *y);
 void convert(Context &ctx, Prec from, const void *x, Prec to, void *y,
              int64_t len);
 void fill_zero(Context &ctx, Prec p, int64_t len, void *y);

@@ -431,6 +431,18 @@ __global__ void __launch_bounds__(kBlock)
     y[i] = y[i] * st;
 }
 
+// y = x * (1 / *den): the vnew = w / beta store of a Golub-Kahan step with
+// beta on the device.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_scale_copy_dev(int64_t len, const double *__restrict__ den, const T *__restrict__ x,
+                     T *__restrict__ y) {
+  const T s = (T)(1.0 / (*den));
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = x[i] * s;
+}
+
 template <class TI, class TO>
 __global__ void __launch_bounds__(kBlock)
     k_convert(int64_t len, const TI *__restrict__ x, TO *__restrict__ y) {
@@ -991,6 +1003,19 @@ void scale_dev(Context &ctx, Prec p, int64_t len, const double *num, bool recipr
   else
     hipLaunchKernelGGL(k_scale_dev<float>, dim3(grid_for(len)), dim3(kBlock), 0,
                        ctx.stream, len, num, reciprocal ? 1 : 0, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void scale_copy_dev(Context &ctx, Prec p, int64_t len, const double *den, const void *x,
+                    void *y) {
+  if (len <= 0)
+    return;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_scale_copy_dev<double>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, den, (const double *)x, (double *)y);
+  else
+    hipLaunchKernelGGL(k_scale_copy_dev<float>, dim3(grid_for(len)), dim3(kBlock), 0,
+                       ctx.stream, len, den, (const float *)x, (float *)y);
   KH_CHECK(hipGetLastError());
 }
 

@@ -2,9 +2,12 @@
 // abstraction they run on, the reference-compatible random stream, options,
 // results and the two drivers (cyclic Lanczos / Golub-Kahan and LOBPCG).
 #pragma once
+#include <condition_variable>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -62,15 +65,28 @@ struct CsrOp final : DeviceOp {
 // The reference's random stream: std::mt19937(seed) feeding one
 // std::normal_distribution<double> whose cached second draw persists across
 // calls (StdVectorFactory.hpp:16-58, EigenVectorFactory.hpp:13-38). Vectors
-// are generated on the host, in the reference's order, and uploaded.
-struct RefStream {
-  std::mt19937 gen;
-  std::normal_distribution<double> dist;
-  explicit RefStream(unsigned seed) : gen(seed), dist(0.0, 1.0) {}
-  void fill(double *out, std::size_t n) {
-    for (std::size_t i = 0; i < n; ++i)
-      out[i] = dist(gen);
-  }
+// are generated on the host, in the reference's order, and uploaded. A
+// background thread keeps the next values of the sequence ready (the
+// sequence does not depend on how requests are split), so generation
+// overlaps the device work.
+class RefStream {
+public:
+  explicit RefStream(unsigned seed);
+  ~RefStream();
+  RefStream(const RefStream &) = delete;
+  RefStream &operator=(const RefStream &) = delete;
+  void fill(double *out, std::size_t n);
+
+private:
+  void produce();
+  std::mt19937 gen_;
+  std::normal_distribution<double> dist_;
+  std::vector<double> ring_;
+  std::size_t head_ = 0, count_ = 0;
+  bool stop_ = false;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::thread th_;
 };
 
 // ---------------------------------------------------------------------------

@@ -193,3 +193,19 @@ def test_backend_mismatch_raises():
                                         K.CyclicLanczosOptions())
     with pytest.raises(ValueError):
         hip(A).multiply(np.zeros(2))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_twice_bitwise_identical(precision):
+    """No atomics and fixed reduction trees: two solves with the same seed
+    return bit-identical bases (a race / nondeterminism detector)."""
+    A = W.regular_incidence(4, 2048, seed=9)
+    print(f"[case] determinism {precision}", flush=True)
+    H = hip(A)
+    opt = K.CyclicLanczosOptions(k=4, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3)
+    r1 = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 5), opt, precision)
+    r2 = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 5), opt, precision)
+    assert r1.right_null.shape == r2.right_null.shape == (A.n, 4)
+    assert np.array_equal(r1.right_null.view(np.uint64), r2.right_null.view(np.uint64))
+    assert (r1.restarts, r1.cg_solves, r1.krylov_steps_total) == \
+        (r2.restarts, r2.cg_solves, r2.krylov_steps_total)
