@@ -38,6 +38,13 @@ inline double since(Clock::time_point t0) {
 
 inline std::size_t esize(Prec p) { return p == Prec::F64 ? 8 : 4; }
 
+// Leading dimension (elements) for a block of columns of length len: the
+// length rounded up to 256 bytes plus 17 x 256 bytes.
+inline int64_t pad_ld(int64_t len, std::size_t e) {
+  const int64_t q = 256 / (int64_t)e;
+  return (len + q - 1) / q * q + 17 * q;
+}
+
 // Read n device doubles into out (through the pinned slab mirror).
 void readback(Context &ctx, const double *d, int n, double *out) {
   if (n <= 0)
@@ -104,6 +111,7 @@ private:
   DBuf small_;          // device scalars of this solve
   double *bnorm_ = nullptr, *h_ = nullptr, *wsc_ = nullptr, *C_ = nullptr, *H_ = nullptr;
   int64_t ycap_ = 0, xcap_ = 0;
+  int64_t ldy_ = 0, ldx_ = 0;  // padded leading dimensions of VB_ and X_
   int64_t k_target_ = 0;
   double accept_thr_ = 0, prescreen_thr_ = 0;
   std::vector<double> hbuf_;
@@ -146,8 +154,12 @@ void CyclicDriver::alloc() {
   basis_.cap = k_target_;
   basis_.count = 0;
   basis_.buf = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 8);
-  VB_ = DBuf((std::size_t)(ycap_ + std::max<int64_t>(1, k_target_)) * n_ * esize(kp_));
-  X_ = DBuf((std::size_t)xcap_ * m_ * esize(kp_));
+  // Column strides are padded off powers of two (by 17 x 256 bytes) so that
+  // the same row of consecutive columns does not map to one HBM channel.
+  ldy_ = pad_ld(n_, esize(kp_));
+  ldx_ = pad_ld(m_, esize(kp_));
+  VB_ = DBuf((std::size_t)(ycap_ + std::max<int64_t>(1, k_target_)) * ldy_ * esize(kp_));
+  X_ = DBuf((std::size_t)xcap_ * ldx_ * esize(kp_));
   wx_ = DBuf((std::size_t)m_ * esize(kp_));
   wy_ = DBuf((std::size_t)n_ * esize(kp_));
   y0_ = DBuf((std::size_t)n_ * 8);
@@ -272,7 +284,7 @@ void CyclicDriver::push_basis(const double *v) {
   KH_CHECK(hipMemcpyAsync(basis_.col(j), v, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
                           c_.stream));
   convert(c_, Prec::F64, v, kp_,
-          static_cast<char *>(VB_.p) + (std::size_t)(ycap_ + j) * n_ * esize(kp_), n_);
+          static_cast<char *>(VB_.p) + (std::size_t)(ycap_ + j) * ldy_ * esize(kp_), n_);
   basis_.count = j + 1;
 }
 
@@ -530,8 +542,8 @@ DeviceBasis CyclicDriver::run() {
   };
 
   char *VBb = static_cast<char *>(VB_.p), *Xb = static_cast<char *>(X_.p);
-  auto Ycol = [&](int64_t j) { return (void *)(VBb + (std::size_t)(ycap_ - 1 - j) * n_ * ke); };
-  auto Xcol = [&](int64_t j) { return (void *)(Xb + (std::size_t)j * m_ * ke); };
+  auto Ycol = [&](int64_t j) { return (void *)(VBb + (std::size_t)(ycap_ - 1 - j) * ldy_ * ke); };
+  auto Xcol = [&](int64_t j) { return (void *)(Xb + (std::size_t)j * ldx_ * ke); };
 
   std::vector<double> alpha, beta, bh((std::size_t)ncv);
   int64_t restarts_since_accept = 0;
@@ -573,9 +585,9 @@ DeviceBasis CyclicDriver::run() {
       const int passes = P_.reorth;
       std::vector<Group> groups;
       if (from_y)
-        groups.push_back({Xb, m_, (int)nxc, kp});
+        groups.push_back({Xb, ldx_, (int)nxc, kp});
       else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
-        groups.push_back({Ycol(nyc - 1), n_, (int)nyc + nb, kp});
+        groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
       for (int pass = 0; pass < passes; ++pass)
         cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
       if (passes <= 0)
@@ -688,7 +700,7 @@ DeviceBasis CyclicDriver::run() {
       double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
       // Y_{ny_used-1} .. Y_0 are contiguous (reversed), so the coefficients
       // are stored in reverse order
-      gemm_nn(c_, kp, Ycol(ny_used - 1), n_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
+      gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
       gs_basis_block(Wv, wn);
       col_norms(c_, Prec::F64, Wv, n_, wn, n_, wsc_);
       std::vector<double> nv(wn);

@@ -15,6 +15,7 @@
 //    them through pointers so the host can queue work without syncing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -199,61 +200,144 @@ __global__ void __launch_bounds__(kBlock)
     out[blockIdx.x] = do_sqrt ? sqrt(fmax(0.0, acc)) : acc;
 }
 
-// V^T w over column tiles of CT columns: blockIdx.y = tile.
-template <class TV, class TW, int CT>
+// Reorthogonalisation products with memory-level parallelism: a block owns
+// kRB = 256*kR consecutive rows; thread t owns rows r0+t+k*256 (k < kR), so
+// for every column the block issues kR coalesced loads per thread before any
+// arithmetic. All s columns of the block's chunk are handled by the block
+// (w is read once); per column a fixed wave64 tree and a 4-wave LDS sum give
+// the block partial. part layout [c * nb + block].
+constexpr int kR = 8;
+constexpr int kRB = kBlock * kR;
+
+// Columns per step: enough independent loads in flight per thread (kR rows x
+// CU columns) to cover HBM latency with 2-4 blocks per CU.
+template <class TV> struct ColUnroll { static constexpr int value = sizeof(TV) == 4 ? 4 : 2; };
+
+template <class TV, class TW>
 __global__ void __launch_bounds__(kBlock)
-    k_gemv_t_partial(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
-                     const TW *__restrict__ w, double *__restrict__ part, int nb) {
-  __shared__ double lds[CT][kWaves];
-  const int c0 = blockIdx.y * CT;
-  double acc[CT];
+    k_vtw_partial(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                  const TW *__restrict__ w, double *__restrict__ part, int nb) {
+  constexpr int CU = ColUnroll<TV>::value;
+  extern __shared__ double red[];  // [s][kWaves]
+  const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
+  // Row indices are clamped into range (rows past the end get weight 0), so
+  // every load below is unconditional and the loads of a step issue
+  // back to back.
+  int64_t rr[kR];
+  double wv[kR];
 #pragma unroll
-  for (int c = 0; c < CT; ++c)
-    acc[c] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
-       i += (int64_t)gridDim.x * kBlock) {
-    const double wi = (double)w[i];
-#pragma unroll
-    for (int c = 0; c < CT; ++c)
-      if (c0 + c < s)
-        acc[c] += (double)V[i + (int64_t)(c0 + c) * ld] * wi;
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    rr[k] = r < len ? r : len - 1;
+    const double wr = (double)w[rr[k]];
+    wv[k] = r < len ? wr : 0.0;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int s_main = s - s % CU;
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    TV x[CU][kR];
 #pragma unroll
-  for (int c = 0; c < CT; ++c) {
-    const double v = wave_sum(acc[c]);
+    for (int u = 0; u < CU; ++u) {
+      const TV *vc = V + (int64_t)(c0 + u) * ld;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        x[u][k] = vc[rr[k]];
+    }
+    // keep the CU*kR loads of this step together ahead of their uses
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        acc += (double)x[u][k] * wv[k];
+      acc = wave_sum(acc);
+      if (lane == 0)
+        red[(c0 + u) * kWaves + wid] = acc;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const TV *vc = V + (int64_t)c * ld;
+    TV x[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      x[k] = vc[rr[k]];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      acc += (double)x[k] * wv[k];
+    acc = wave_sum(acc);
     if (lane == 0)
-      lds[c][wid] = v;
+      red[c * kWaves + wid] = acc;
   }
   __syncthreads();
-  if (threadIdx.x < CT && c0 + (int)threadIdx.x < s) {
+  for (int c = threadIdx.x; c < s; c += kBlock) {
     double t = 0.0;
     for (int w2 = 0; w2 < kWaves; ++w2)
-      t += lds[threadIdx.x][w2];
-    part[(int64_t)(c0 + threadIdx.x) * nb + blockIdx.x] = t;
+      t += red[c * kWaves + w2];
+    part[(int64_t)c * nb + blockIdx.x] = t;
   }
 }
 
-// w -= V h, optional partial of ||w_new||^2.
+// w -= V h over the same row ownership; optional partial of ||w_new||^2.
 template <class TV, class TW>
 __global__ void __launch_bounds__(kBlock)
-    k_gemv_n_sub(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
-                 const double *__restrict__ h, TW *__restrict__ w,
-                 double *__restrict__ part) {
-  __shared__ double hs[512];
+    k_vh_sub(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+             const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part) {
+  constexpr int CU = ColUnroll<TV>::value;
+  extern __shared__ double hs[];  // [s]
   __shared__ double lds[kWaves];
-  for (int j = threadIdx.x; j < s && j < 512; j += kBlock)
+  for (int j = threadIdx.x; j < s; j += kBlock)
     hs[j] = h[j];
   __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
+  int64_t rr[kR];
+  double acc[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    rr[k] = r < len ? r : len - 1;
+    acc[k] = 0.0;
+  }
+  const int s_main = s - s % CU;
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    TV x[CU][kR];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const TV *vc = V + (int64_t)(c0 + u) * ld;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        x[u][k] = vc[rr[k]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const double hc = hs[c0 + u];
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        acc[k] += (double)x[u][k] * hc;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const TV *vc = V + (int64_t)c * ld;
+    const double hc = hs[c];
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      acc[k] += (double)vc[rr[k]] * hc;
+  }
+  TW wold[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k)
+    wold[k] = w[rr[k]];
   double nacc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
-       i += (int64_t)gridDim.x * kBlock) {
-    double acc = 0.0;
-    for (int j = 0; j < s; ++j)
-      acc += (double)V[i + (int64_t)j * ld] * (j < 512 ? hs[j] : h[j]);
-    const TW nw = (TW)((double)w[i] - acc);
-    w[i] = nw;
-    nacc += (double)nw * (double)nw;
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    const TW nw = (TW)((double)wold[k] - acc[k]);
+    if (r < len) {
+      w[r] = nw;
+      nacc += (double)nw * (double)nw;
+    }
   }
   if (part) {
     nacc = block_sum(nacc, lds);
@@ -400,6 +484,155 @@ __global__ void __launch_bounds__(kBlock)
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
        i += (int64_t)gridDim.x * kBlock)
     o[i] = x[i] + (T)(-1) * y[i];
+}
+
+// ------------------------------------------------- tall-skinny GEMM tiles --
+// H(s x q) = V^T W over a 64 x 64 output tile (blockIdx.y: rows of H in
+// steps of 64, blockIdx.z: columns in steps of 64). Each block reduces a
+// contiguous chunk of rows into a private 64x64 partial (LDS-staged 32-row
+// slabs, 4x4 register micro-tiles per thread); k_ts_finish sums the partials
+// in block order. V and W are read once per tile pair.
+constexpr int kTS = 64, kSlab = 32;
+
+template <class TV, class TW>
+__global__ void __launch_bounds__(kBlock)
+    k_ts_tn_partial(int64_t len, int64_t rows_per_block, const TV *__restrict__ V,
+                    int64_t ldv, int s, const TW *__restrict__ W, int64_t ldw, int q,
+                    double *__restrict__ part) {
+  __shared__ double Vs[kSlab][kTS + 1];
+  __shared__ double Ws[kSlab][kTS + 1];
+  const int j0 = blockIdx.y * kTS, c0 = blockIdx.z * kTS;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      acc[a][b] = 0.0;
+  for (int64_t rb = r0; rb < r1; rb += kSlab) {
+    // load the slab: 32 rows x 64 columns of V and W (zero padded)
+    // clamped (always in-bounds) loads, zeroed afterwards: no per-load branch
+    TV va[kSlab * kTS / kBlock];
+    TW wa[kSlab * kTS / kBlock];
+#pragma unroll
+    for (int t = 0; t < kSlab * kTS / kBlock; ++t) {
+      const int e = threadIdx.x + t * kBlock;
+      const int r = e % kSlab, j = e / kSlab;
+      const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
+      const int jv = (j0 + j) < s ? (j0 + j) : (s - 1);
+      const int jw = (c0 + j) < q ? (c0 + j) : (q - 1);
+      va[t] = V[row + (int64_t)jv * ldv];
+      wa[t] = W[row + (int64_t)jw * ldw];
+    }
+#pragma unroll
+    for (int t = 0; t < kSlab * kTS / kBlock; ++t) {
+      const int e = threadIdx.x + t * kBlock;
+      const int r = e % kSlab, j = e / kSlab;
+      const bool inr = (rb + r) < r1;
+      Vs[r][j] = (inr && j0 + j < s) ? (double)va[t] : 0.0;
+      Ws[r][j] = (inr && c0 + j < q) ? (double)wa[t] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = 0; r < kSlab; ++r) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = Vs[r][tx * 4 + u];
+        b[u] = Ws[r][ty * 4 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          acc[u][v] += a[u] * b[v];
+    }
+    __syncthreads();
+  }
+  // partial tile layout: [(tile_y, tile_z, block)][64 x 64]
+  const int64_t tile = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  double *out = part + tile * (kTS * kTS);
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      out[(ty * 4 + v) * kTS + tx * 4 + u] = acc[u][v];
+}
+
+// H(j, c) = sum over blocks of the partial tiles, in block order.
+__global__ void __launch_bounds__(kBlock)
+    k_ts_finish(const double *__restrict__ part, int nblk, int ty_tiles, int s, int q,
+                double *__restrict__ H) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t per_tile = kTS * kTS;
+  const int64_t tiles = (int64_t)ty_tiles * ((q + kTS - 1) / kTS);
+  if (e >= tiles * per_tile)
+    return;
+  const int64_t t = e / per_tile, w = e % per_tile;
+  const int tyi = (int)(t % ty_tiles), tzi = (int)(t / ty_tiles);
+  const int jj = (int)(w % kTS), cc = (int)(w / kTS);
+  const int j = tyi * kTS + jj, c = tzi * kTS + cc;
+  if (j >= s || c >= q)
+    return;
+  const double *p = part + ((int64_t)tzi * ty_tiles + tyi) * nblk * per_tile + w;
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b)
+    acc += p[(int64_t)b * per_tile];
+  H[j + (int64_t)c * s] = acc;
+}
+
+// OUT(:, c) (= or -=) V H(:, c) for a tile of kQT output columns
+// (blockIdx.y); H staged in LDS, one row per thread, FP64 accumulation.
+constexpr int kQT = 32;
+template <class TV, class TO, bool SUB>
+__global__ void __launch_bounds__(kBlock)
+    k_ts_nn(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
+            const double *__restrict__ H, int q, TO *__restrict__ OUT, int64_t ldo) {
+  constexpr int JU = 8;  // V columns loaded per step (unconditional, clamped row)
+  __shared__ double Hs[64][kQT];
+  const int c0 = blockIdx.y * kQT;
+  const int qc = (q - c0) < kQT ? (q - c0) : kQT;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i - threadIdx.x < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int64_t ic = i < len ? i : len - 1;
+    double acc[kQT];
+#pragma unroll
+    for (int c = 0; c < kQT; ++c)
+      acc[c] = 0.0;
+    for (int jb = 0; jb < s; jb += 64) {
+      const int js = (s - jb) < 64 ? (s - jb) : 64;
+      __syncthreads();
+      for (int e = threadIdx.x; e < 64 * kQT; e += kBlock) {
+        const int j = e / kQT, c = e % kQT;
+        Hs[j][c] = (j < js && c < qc) ? H[(jb + j) + (int64_t)(c0 + c) * s] : 0.0;
+      }
+      __syncthreads();
+      for (int j0 = 0; j0 < js; j0 += JU) {
+        TV v[JU];
+#pragma unroll
+        for (int u = 0; u < JU; ++u) {
+          const int j = (j0 + u) < js ? (j0 + u) : (js - 1);  // Hs is 0 past js
+          v[u] = V[ic + (int64_t)(jb + j) * ldv];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < JU; ++u) {
+          const double vu = (double)v[u];
+#pragma unroll
+          for (int c = 0; c < kQT; ++c)
+            acc[c] += vu * Hs[(j0 + u) & 63][c];
+        }
+      }
+    }
+    if (i < len) {
+      for (int c = 0; c < qc; ++c) {
+        TO *o = OUT + (int64_t)(c0 + c) * ldo;
+        o[i] = SUB ? (TO)((double)o[i] - acc[c]) : (TO)acc[c];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- level 1 --
@@ -798,16 +1031,23 @@ void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
   KH_CHECK(hipGetLastError());
 }
 
+inline int vtw_blocks(int64_t len) {
+  int64_t g = (len + kRB - 1) / kRB;
+  return (int)(g < 1 ? 1 : g);
+}
+
 template <class TV, class TW>
 static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
                           int64_t len, double *out) {
-  constexpr int CT = 8;
-  const int nb = reduce_blocks(len);
-  double *part = ctx.partials((std::size_t)nb * s);
-  dim3 grid(nb, (s + CT - 1) / CT);
-  hipLaunchKernelGGL((k_gemv_t_partial<TV, TW, CT>), grid, dim3(kBlock), 0, ctx.stream,
-                     len, V, ld, s, w, part, nb);
-  hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
+  const int nb = vtw_blocks(len);
+  for (int c0 = 0; c0 < s; c0 += 512) {  // LDS holds 512 x 4 partials
+    const int sc = std::min(512, s - c0);
+    double *part = ctx.partials((std::size_t)nb * sc);
+    hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock),
+                       (std::size_t)sc * kWaves * sizeof(double), ctx.stream, len,
+                       V + (int64_t)c0 * ld, ld, sc, w, part, nb);
+    hipLaunchKernelGGL(k_finish, dim3(sc), dim3(kBlock), 0, ctx.stream, part, nb, out + c0, 0);
+  }
   KH_CHECK(hipGetLastError());
 }
 
@@ -828,10 +1068,11 @@ void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
 template <class TV, class TW>
 static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
                               const double *h, TW *w, int64_t len, double *nrm) {
-  const int nb = reduce_blocks(len);
+  const int nb = vtw_blocks(len);
   double *part = nrm ? ctx.partials(nb) : nullptr;
-  hipLaunchKernelGGL((k_gemv_n_sub<TV, TW>), dim3(nb), dim3(kBlock), 0, ctx.stream, len,
-                     V, ld, s, h, w, part);
+  hipLaunchKernelGGL((k_vh_sub<TV, TW>), dim3(nb), dim3(kBlock),
+                     (std::size_t)std::max(s, 1) * sizeof(double), ctx.stream, len, V, ld, s, h,
+                     w, part);
   if (nrm)
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, part, nb, nrm, 1);
   KH_CHECK(hipGetLastError());
@@ -854,9 +1095,15 @@ void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
 template <class TV, class TO>
 static void launch_gemm_nn(Context &ctx, const TV *V, int64_t ldv, int s, const double *H,
                            int q, TO *out, int64_t ldo, int64_t len) {
-  dim3 grid(grid_for(len, 4), q);
-  hipLaunchKernelGGL((k_gemm_nn<TV, TO>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
-                     s, H, q, out, ldo);
+  if (q == 1) {
+    dim3 grid(grid_for(len, 4), 1);
+    hipLaunchKernelGGL((k_gemm_nn<TV, TO>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
+                       s, H, q, out, ldo);
+  } else {
+    dim3 grid(grid_for(len, 2), (q + kQT - 1) / kQT);
+    hipLaunchKernelGGL((k_ts_nn<TV, TO, false>), grid, dim3(kBlock), 0, ctx.stream, len, V,
+                       ldv, s, H, q, out, ldo);
+  }
   KH_CHECK(hipGetLastError());
 }
 
@@ -882,13 +1129,17 @@ void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const doubl
 template <class TV, class TW>
 static void launch_gemm_tn(Context &ctx, const TV *V, int64_t ldv, int s, const TW *W,
                            int64_t ldw, int q, int64_t len, double *H) {
-  constexpr int CT = 8;
-  const int nb = reduce_blocks(len);
-  double *part = ctx.partials((std::size_t)nb * s * q);
-  dim3 grid(nb, (s + CT - 1) / CT, q);
-  hipLaunchKernelGGL((k_gemm_tn_partial<TV, TW, CT>), grid, dim3(kBlock), 0, ctx.stream,
-                     len, V, ldv, s, W, ldw, part, nb);
-  hipLaunchKernelGGL(k_finish, dim3(s * q), dim3(kBlock), 0, ctx.stream, part, nb, H, 0);
+  const int ty = (s + kTS - 1) / kTS, tz = (q + kTS - 1) / kTS;
+  int nblk = (int)std::min<int64_t>(512, std::max<int64_t>(1, (len + 2047) / 2048));
+  int64_t rpb = (len + nblk - 1) / nblk;
+  rpb = (rpb + kSlab - 1) / kSlab * kSlab;
+  nblk = (int)std::max<int64_t>(1, (len + rpb - 1) / rpb);
+  double *part = ctx.partials((std::size_t)nblk * ty * tz * kTS * kTS);
+  hipLaunchKernelGGL((k_ts_tn_partial<TV, TW>), dim3(nblk, ty, tz), dim3(kBlock), 0,
+                     ctx.stream, len, rpb, V, ldv, s, W, ldw, q, part);
+  const int64_t outs = (int64_t)ty * tz * kTS * kTS;
+  hipLaunchKernelGGL(k_ts_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, ctx.stream, part, nblk, ty, s, q, H);
   KH_CHECK(hipGetLastError());
 }
 
@@ -896,6 +1147,13 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
              const void *W, int64_t ldw, int q, int64_t len, double *H) {
   if (s <= 0 || q <= 0)
     return;
+  if (q <= 4) {  // narrow: column by column (no 64-wide tile padding)
+    const std::size_t ew = pw == Prec::F64 ? 8 : 4;
+    for (int c = 0; c < q; ++c)
+      gemv_t(ctx, pv, V, ldv, s, pw, static_cast<const char *>(W) + (std::size_t)c * ldw * ew,
+             len, H + (std::size_t)c * s);
+    return;
+  }
   if (pv == Prec::F64 && pw == Prec::F64)
     launch_gemm_tn(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H);
   else if (pv == Prec::F32 && pw == Prec::F32)
@@ -910,13 +1168,24 @@ template <class TV, class TW>
 static void launch_gemm_nn_sub(Context &ctx, const TV *V, int64_t ldv, int s,
                                const double *H, int q, TW *W, int64_t ldw, int64_t len,
                                double *norms) {
-  const int nb = reduce_blocks(len);
-  double *part = norms ? ctx.partials((std::size_t)nb * q) : nullptr;
-  hipLaunchKernelGGL((k_gemm_nn_sub<TV, TW>), dim3(nb, q), dim3(kBlock), 0, ctx.stream,
-                     len, V, ldv, s, H, W, ldw, part, nb);
-  if (norms)
-    hipLaunchKernelGGL(k_finish, dim3(q), dim3(kBlock), 0, ctx.stream, part, nb, norms, 1);
+  if (q <= 4) {  // narrow: column by column
+    for (int c = 0; c < q; ++c)
+      launch_gemv_n_sub(ctx, V, ldv, s, H + (std::size_t)c * s, W + (int64_t)c * ldw, len,
+                        norms ? norms + c : nullptr);
+    return;
+  }
+  dim3 grid(grid_for(len, 2), (q + kQT - 1) / kQT);
+  hipLaunchKernelGGL((k_ts_nn<TV, TW, true>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
+                     s, H, q, W, ldw);
   KH_CHECK(hipGetLastError());
+  if (norms) {
+    const int nb = reduce_blocks(len);
+    double *part = ctx.partials((std::size_t)nb * q);
+    hipLaunchKernelGGL(k_col_sq<TW>, dim3(nb, q), dim3(kBlock), 0, ctx.stream, len,
+                       (const TW *)W, ldw, part, nb);
+    hipLaunchKernelGGL(k_finish, dim3(q), dim3(kBlock), 0, ctx.stream, part, nb, norms, 1);
+    KH_CHECK(hipGetLastError());
+  }
 }
 
 void gemm_nn_sub(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,

@@ -1,0 +1,101 @@
+"""Kernel-level parity through the C ABI: reductions, CGS2 and the batched
+Gram products against numpy FP64 references (the oracle's arithmetic for
+these is plain sums; tolerances are stated per test)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import krans_amd as K
+
+pytestmark = pytest.mark.gpu
+
+
+class Dev:
+    """A device copy of a numpy array (FP64 or FP32)."""
+
+    def __init__(self, ctx, a):
+        self.ctx = ctx
+        self.a = np.asfortranarray(a)
+        self.p = C.c_void_p()
+        K._check(K.lib().kh_dev_alloc(ctx.h, max(self.a.nbytes, 8), C.byref(self.p)))
+        K._check(K.lib().kh_h2d(ctx.h, self.p, self.a.ctypes.data_as(C.c_void_p), self.a.nbytes))
+
+    def get(self):
+        out = np.empty_like(self.a, order="F")
+        K._check(K.lib().kh_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.p, out.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            K.lib().kh_dev_free(self.ctx.h, self.p)
+        except Exception:
+            pass
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return K.default_context()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 2_000_003])
+def test_dot_fp64_and_fp32(ctx, n):
+    print(f"[case] dot n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    da, db = Dev(ctx, a), Dev(ctx, b)
+    out = C.c_double()
+    K._check(K.lib().kh_dot_dev(ctx.h, 0, da.p, db.p, n, C.byref(out)))
+    ref = float(np.dot(a, b))
+    assert abs(out.value - ref) <= 1e-14 * np.abs(a) @ np.abs(b) + 1e-300
+    a32, b32 = a.astype(np.float32), b.astype(np.float32)
+    da32, db32 = Dev(ctx, a32), Dev(ctx, b32)
+    K._check(K.lib().kh_dot_dev(ctx.h, 1, da32.p, db32.p, n, C.byref(out)))
+    ref32 = float(np.dot(a32.astype(np.float64), b32.astype(np.float64)))
+    # FP32 inputs, FP64 accumulation
+    assert abs(out.value - ref32) <= 1e-14 * np.abs(a) @ np.abs(b) + 1e-300
+    # determinism: same bits twice
+    o2 = C.c_double()
+    K._check(K.lib().kh_dot_dev(ctx.h, 1, da32.p, db32.p, n, C.byref(o2)))
+    assert o2.value == out.value
+
+
+@pytest.mark.parametrize("s,n", [(1, 5000), (33, 100_000), (96, 300_000)])
+def test_cgs2_orthogonalises(ctx, s, n):
+    print(f"[case] cgs2 s={s} n={n}", flush=True)
+    rng = np.random.default_rng(s)
+    V, _ = np.linalg.qr(rng.standard_normal((n, s)))
+    w = rng.standard_normal(n) + V @ rng.standard_normal(s)
+    dV, dw = Dev(ctx, V), Dev(ctx, w)
+    h = np.zeros(s)
+    nrm = C.c_double()
+    K._check(K.lib().kh_cgs2_dev(ctx.h, 0, dV.p, s, dw.p, n, h.ctypes.data_as(C.c_void_p),
+                                 C.byref(nrm)))
+    w2 = dw.get()
+    ref = w - V @ (V.T @ w)
+    assert np.abs(V.T @ w2).max() <= 1e-13 * np.linalg.norm(w)
+    assert np.allclose(h, V.T @ w, rtol=1e-12, atol=1e-12 * np.linalg.norm(w))
+    assert np.linalg.norm(w2 - ref) <= 1e-12 * np.linalg.norm(w)
+    assert abs(nrm.value - np.linalg.norm(w2)) <= 1e-13 * nrm.value
+
+
+@pytest.mark.parametrize("k,n,prec", [(3, 1000, 0), (40, 50_000, 0), (96, 200_000, 0),
+                                      (96, 200_000, 1)])
+def test_gram_blocks(ctx, k, n, prec):
+    print(f"[case] gram k={k} n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(k)
+    Z = rng.standard_normal((n, k))
+    AZ = rng.standard_normal((n, k))
+    if prec == 1:
+        Z, AZ = Z.astype(np.float32), AZ.astype(np.float32)
+    dZ, dA = Dev(ctx, Z), Dev(ctx, AZ)
+    S = np.zeros((k, k), order="F")
+    G = np.zeros((k, k), order="F")
+    K._check(K.lib().kh_gram_dev(ctx.h, prec, dZ.p, dA.p, k, n, S.ctypes.data_as(C.c_void_p),
+                                 G.ctypes.data_as(C.c_void_p)))
+    Z64, A64 = Z.astype(np.float64), AZ.astype(np.float64)
+    Sr, Gr = Z64.T @ Z64, Z64.T @ A64
+    scale = np.sqrt(np.outer(np.sum(Z64 * Z64, 0), np.sum(A64 * A64, 0)))
+    tol = 1e-13 if prec == 0 else 1e-6
+    assert np.abs(S - Sr).max() <= tol * scale.max()
+    assert np.abs(G - Gr).max() <= tol * scale.max()
