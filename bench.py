@@ -81,16 +81,26 @@ def main():
         sine = float(np.linalg.norm(R, 2))
         return float(res.max() / A.fro()), sine
 
-    def run(precision, steps, warmup):
+    trace = {"launches": 0, "bytes": 0.0, "ms": 0.0}
+
+    def run(precision, steps, warmup, traced=False):
         lat, found, chk, info = [], [], None, None
         seed = 42
         for i in range(warmup + steps):
             vf = K.HipVectorFactory(A.n, seed)
             H.ctx.sync()
+            timed = i >= warmup
+            if traced and timed:
+                H.ctx.trace_begin("vtw")
             t = time.perf_counter()
             r = K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision=precision)
             H.ctx.sync()
             dt = time.perf_counter() - t
+            if traced and timed:
+                n_l, b_l, ms_l = H.ctx.trace_end()
+                trace["launches"] += n_l
+                trace["bytes"] += b_l
+                trace["ms"] += ms_l
             if args.verbose:
                 print(f"[{precision}] solve {i}: {dt:.3f}s found={r.right_null.shape[1]} "
                       f"restarts={r.restarts} cg={r.cg_solves} times={r.times}",
@@ -104,12 +114,16 @@ def main():
         return lat, found, chk, info
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else args.steps
-    lat_m, found_m, chk_m, info_m = run("mixed", args.steps, args.warmup)
-    lat_d, found_d, chk_d, info_d = run("fp64", fp64_steps, args.warmup)
+    copy_probe(K, H.ctx)
+    lat_m, found_m, chk_m, info_m = run("mixed", args.steps, args.warmup, traced=True)
+    if fp64_steps > 0:
+        lat_d, found_d, chk_d, info_d = run("fp64", fp64_steps, args.warmup)
+    else:
+        lat_d, found_d, chk_d, info_d = [], [], (None, None), None
 
     total_m = sum(lat_m)
     value = sum(found_m) / total_m
-    fp64_value = sum(found_d) / sum(lat_d)
+    fp64_value = sum(found_d) / sum(lat_d) if lat_d else None
 
     def pct(xs, q):
         xs = sorted(xs)
@@ -120,13 +134,16 @@ def main():
         hi = min(lo + 1, len(xs) - 1)
         return xs[lo] + (xs[hi] - xs[lo]) * (pos - lo)
 
-    # roofline of the dominant sparse kernel class: FP32 CSR gather SpMV on
-    # this matrix (HIP events on the library stream)
+    # roofline of the dominant kernel of the timed mixed solves: the FP32
+    # reorthogonalisation product V^T w (k_vtw_partial<float,float>), timed
+    # per launch with HIP events on the library stream during the timed solves
+    achieved = trace["bytes"] / (trace["ms"] * 1e-3) / 1e9 if trace["ms"] > 0 else None
+    avg_bytes = trace["bytes"] / max(1, trace["launches"])
+    avg_ms = trace["ms"] / max(1, trace["launches"])
+    peak = 8000.0
     ms_spmv = H.time_spmv(transpose=False, precision="fp32", reps=200)
     ms_spmtv = H.time_spmv(transpose=True, precision="fp32", reps=200)
-    b_spmv = spmv_bytes(A, 4)
-    achieved = b_spmv / (ms_spmv * 1e-3) / 1e9
-    peak = 8000.0
+    traffic = load_traffic("k_vtw_partial<float, float>")
 
     cpu = None
     if not args.no_cpu:
@@ -153,28 +170,60 @@ def main():
             "max_restarts": MAX_RESTARTS,
         },
         "latency_s": {"p50": round(pct(lat_m, 0.5), 4), "p99": round(pct(lat_m, 0.99), 4)},
-        "fp64": {
+        "fp64": None if not lat_d else {
             "value": round(fp64_value, 4), "steps": fp64_steps,
             "latency_s": {"p50": round(pct(lat_d, 0.5), 4), "p99": round(pct(lat_d, 0.99), 4)},
             "max_res_rel": chk_d[0], "sine_vs_exact": chk_d[1],
             "restarts": info_d.restarts, "cg_solves": info_d.cg_solves,
         },
-        "mixed_over_fp64": round(value / fp64_value, 3),
+        "mixed_over_fp64": round(value / fp64_value, 3) if fp64_value else None,
         "accuracy": {"max_res_rel": chk_m[0], "sine_vs_exact": chk_m[1],
                      "vectors": found_m[0], "restarts": info_m.restarts,
                      "cg_solves": info_m.cg_solves, "phase_s": info_m.times},
         "roofline": {
-            "kernel": "k_spmv_lane<float> (CSR gather y=Ax)",
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": peak, "unit": "GB/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
-            "bytes_per_launch": b_spmv, "avg_ms": round(ms_spmv, 5),
-            "spmtv_avg_ms": round(ms_spmtv, 5),
-            "spmtv_GBps": round(spmv_bytes(A, 4) / (ms_spmtv * 1e-3) / 1e9, 1),
+            "kernel": "k_vtw_partial<float, float> (reorthogonalisation V^T w, FP32 in, FP64 acc)",
+            "bound": "hbm",
+            "achieved": None if achieved is None else round(achieved, 1),
+            "peak": peak, "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / peak, 4),
+            "traffic": traffic,
+            "launches": trace["launches"], "avg_bytes_per_launch": round(avg_bytes),
+            "avg_us_per_launch": round(1e3 * avg_ms, 3),
+            "spmv_fp32": {"avg_us": round(1e3 * ms_spmv, 2),
+                          "GBps": round(spmv_bytes(A, 4) / (ms_spmv * 1e-3) / 1e9, 1),
+                          "spmtv_avg_us": round(1e3 * ms_spmtv, 2),
+                          "spmtv_GBps": round(spmv_bytes(A, 4) / (ms_spmtv * 1e-3) / 1e9, 1)},
         },
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line))
+
+
+def copy_probe(K, ctx, mib=256):
+    """One plain copy of a known size (y = x, FP64), the reference point for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the profiled kernels."""
+    import ctypes as C
+    n = mib * (1 << 20) // 8
+    x, y = C.c_void_p(), C.c_void_p()
+    K._check(K.lib().kh_dev_alloc(ctx.h, n * 8, C.byref(x)))
+    K._check(K.lib().kh_dev_alloc(ctx.h, n * 8, C.byref(y)))
+    K._check(K.lib().kh_vec_axpby(ctx.h, n, 0.0, None, 0.0, x))
+    K._check(K.lib().kh_vec_axpby(ctx.h, n, 1.0, x, 0.0, y))
+    K._check(K.lib().kh_dev_free(ctx.h, x))
+    K._check(K.lib().kh_dev_free(ctx.h, y))
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/traffic_*.json, made by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs of this script, scaled by the same counters on a plain copy)."""
+    path = os.path.join(ROOT, "profiles", "traffic_vtw_fp32.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(kernel, {}).get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(budget_s):

@@ -168,6 +168,13 @@ int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s,
  * S and G k x k FP64 out (host). */
 int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ,
                 int k, int64_t len, double *S, double *G);
+/* Kernel timing with HIP events on the context's stream: between begin and
+ * end every launch of the class (1 = reorthogonalisation V^T w products,
+ * 2 = w -= V h updates, 3 = sparse products) is bracketed by an event pair;
+ * end returns the launch count, their summed algorithmic bytes (each array
+ * touched once) and summed device time in ms. */
+int kh_trace_begin(kh_context *ctx, int kernel_class);
+int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms);
 /* Average device time (ms) of `reps` launches of y = op(A) x, measured with
  * HIP events on the library stream. */
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps,

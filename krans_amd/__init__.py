@@ -114,6 +114,8 @@ SIGNATURES = {
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
     "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),
+    "kh_trace_begin": (I32, [P, I32]),
+    "kh_trace_end": (I32, [P, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
 }
 
 _lib = None
@@ -175,6 +177,18 @@ class HipContext:
 
     def sync(self):
         _check(lib().kh_context_sync(self.h))
+
+    TRACE_CLASSES = {"vtw": 1, "vh_sub": 2, "spmv": 3}
+
+    def trace_begin(self, kernel_class):
+        """Time every launch of a kernel class with HIP events (see kh_trace_begin)."""
+        _check(lib().kh_trace_begin(self.h, self.TRACE_CLASSES[kernel_class]))
+
+    def trace_end(self):
+        """Returns (launches, algorithmic bytes, device ms) since trace_begin."""
+        n, b, ms = I64(), D(), D()
+        _check(lib().kh_trace_end(self.h, C.byref(n), C.byref(b), C.byref(ms)))
+        return n.value, b.value, ms.value
 
 
 _default_ctx = None

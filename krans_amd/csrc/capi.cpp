@@ -463,6 +463,36 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
   });
 }
 
+int kh_trace_begin(kh_context *ctx, int kernel_class) {
+  return guard([&] {
+    need(ctx && kernel_class >= 0 && kernel_class <= 3, "bad argument");
+    auto &c = *ctx->ctx;
+    c.sync();
+    c.trace_reset();
+    c.trace_class = kernel_class;
+  });
+}
+
+int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms) {
+  return guard([&] {
+    need(ctx && launches && bytes && ms, "null argument");
+    auto &c = *ctx->ctx;
+    c.sync();
+    double tb = 0.0, tm = 0.0;
+    for (auto &r : c.trace) {
+      float e = 0.0f;
+      KH_CHECK(hipEventElapsedTime(&e, r.a, r.b));
+      tb += r.bytes;
+      tm += (double)e;
+    }
+    *launches = (int64_t)c.trace.size();
+    *bytes = tb;
+    *ms = tm;
+    c.trace_class = kh::kTraceNone;
+    c.trace_reset();
+  });
+}
+
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps, double *avg_ms) {
   return guard([&] {
     need(A && avg_ms && reps >= 1, "bad argument");

@@ -34,9 +34,31 @@ Context::Context(int dev) : device(dev) {
   cu_count = prop.multiProcessorCount;
 }
 
+hipEvent_t Context::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  KH_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Context::trace_reset() {
+  for (auto &r : trace) {
+    event_pool.push_back(r.a);
+    event_pool.push_back(r.b);
+  }
+  trace.clear();
+}
+
 Context::~Context() {
   if (stream)
     (void)hipStreamSynchronize(stream);
+  trace_reset();
+  for (hipEvent_t e : event_pool)
+    (void)hipEventDestroy(e);
   if (dpart)
     (void)hipFree(dpart);
   if (dscal)

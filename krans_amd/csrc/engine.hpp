@@ -31,7 +31,21 @@ enum class Prec : int { F64 = 0, F32 = 1 };
 // ---------------------------------------------------------------------------
 // Context: one device, one stream, a slab of device scalars and a pinned
 // host mirror used to read scalars back with one small copy.
+// Optional per-launch timing of one kernel class with HIP events on the
+// context's stream (for roofline reporting): algorithmic bytes of each traced
+// launch and its event pair.
+enum TraceClass : int { kTraceNone = 0, kTraceVtW = 1, kTraceVhSub = 2, kTraceSpmv = 3 };
+struct TraceRec {
+  double bytes;
+  hipEvent_t a, b;
+};
+
 struct Context {
+  int trace_class = kTraceNone;
+  std::vector<TraceRec> trace;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  void trace_reset();
   int device = 0;
   hipStream_t stream = nullptr;
   double *dscal = nullptr;     // device scalar slab (kScalars doubles)

@@ -922,6 +922,27 @@ __global__ void k_act_normalise(int s, int *__restrict__ act) {
 }  // namespace
 
 // =============================================================== launchers ==
+// Records an event pair around the launches of a traced class.
+struct TraceScope {
+  Context &c;
+  bool on;
+  hipEvent_t a = nullptr;
+  double bytes;
+  TraceScope(Context &ctx, int cls, double b) : c(ctx), on(ctx.trace_class == cls), bytes(b) {
+    if (on) {
+      a = c.take_event();
+      KH_CHECK(hipEventRecord(a, c.stream));
+    }
+  }
+  ~TraceScope() {
+    if (on) {
+      hipEvent_t b2 = c.take_event();
+      (void)hipEventRecord(b2, c.stream);
+      c.trace.push_back({bytes, a, b2});
+    }
+  }
+};
+
 int reduce_blocks(int64_t len) {
   int64_t g = (len + 2047) / 2048;
   if (g < 1)
@@ -962,6 +983,10 @@ static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
 void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
           void *y, const void *z, const double *beta) {
   const int64_t rows = trans ? A.n : A.m;
+  const double e = p == Prec::F64 ? 8.0 : 4.0;
+  TraceScope tr(ctx, kTraceSpmv,
+                (e + 4.0) * A.nnz + 4.0 * (rows + 1) + e * (trans ? A.m : A.n) +
+                    e * rows * (z ? 2.0 : 1.0));
   const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
   const int lpr = trans ? A.lpr_cols : A.lpr_rows;
@@ -1043,6 +1068,8 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
   for (int c0 = 0; c0 < s; c0 += 512) {  // LDS holds 512 x 4 partials
     const int sc = std::min(512, s - c0);
     double *part = ctx.partials((std::size_t)nb * sc);
+    TraceScope tr(ctx, kTraceVtW,
+                  (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc);
     hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock),
                        (std::size_t)sc * kWaves * sizeof(double), ctx.stream, len,
                        V + (int64_t)c0 * ld, ld, sc, w, part, nb);
@@ -1070,9 +1097,13 @@ static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
                               const double *h, TW *w, int64_t len, double *nrm) {
   const int nb = vtw_blocks(len);
   double *part = nrm ? ctx.partials(nb) : nullptr;
+  {
+  TraceScope tr(ctx, kTraceVhSub,
+                (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * s);
   hipLaunchKernelGGL((k_vh_sub<TV, TW>), dim3(nb), dim3(kBlock),
                      (std::size_t)std::max(s, 1) * sizeof(double), ctx.stream, len, V, ld, s, h,
                      w, part);
+  }
   if (nrm)
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, part, nb, nrm, 1);
   KH_CHECK(hipGetLastError());
