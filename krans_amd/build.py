@@ -47,9 +47,29 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_examples()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+EXAMPLES = {"quickstart_hip": os.path.join(os.path.dirname(HERE), "tests", "cxx",
+                                           "quickstart_hip.cpp")}
+
+
+def build_examples():
+    """C++ programs written against the KraNS API + HIP adapters (plain g++,
+    linked to libkrans_hip.so): the drop-in check of the C++ boundary."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    for name, src in EXAMPLES.items():
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(OUT, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-I", inc, src, "-o", out, "-L", OUT, "-lkrans_hip",
+               f"-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -79,3 +79,17 @@ def test_lobpcg_inconsistent_hints_raise():
     with pytest.raises(ValueError):
         K.find_nullspace_lobpcg(hip(A), K.HipVectorFactory(3, 1),
                                 K.LOBPCGOptions(nev=1, nullity_known=5))
+
+
+def test_cxx_quickstart_drop_in():
+    """The reference's quick start, compiled against the unchanged KraNS C++
+    API with the HIP adapters swapped in (tests/cxx/quickstart_hip.cpp)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(K.LIB_PATH), "quickstart_hip")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    print("[case] c++ quickstart", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cyclic   3" in r.stdout and "lobpcg   3" in r.stdout
