@@ -138,6 +138,34 @@ int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt,
               double *right, int64_t right_cap, double *theta,
               double *residuals, kh_lobpcg_result *res);
 
+/* --- serving: continuous batching of null-space requests -----------------
+ * No counterpart in the reference (which solves one matrix per call). A
+ * server owns `slots` solver slots (one device stream each) on `device`;
+ * requests are admitted in arrival order when a slot is free and their
+ * device footprint (matrix + Krylov state) fits in `hbm_budget_bytes`
+ * (<= 0: 85% of the free HBM) beside the running ones. Each request is
+ * find_nullspace_cyclic_lanczos_ex with its own seeded stream; its result
+ * does not depend on the other requests. */
+typedef struct kh_server_s kh_server;
+int kh_server_create(int device, int slots, double hbm_budget_bytes, kh_server **out);
+int kh_server_destroy(kh_server *s);
+/* Host CSR, copied at submit. */
+int kh_server_submit_cyclic(kh_server *s, int64_t m, int64_t n, const int32_t *rowptr,
+                            const int32_t *col, const double *val,
+                            const kh_cyclic_options *opt, int precision, uint32_t seed,
+                            int64_t *id);
+/* A resident device matrix (must outlive the request). */
+int kh_server_submit_resident(kh_server *s, kh_matrix *A, const kh_cyclic_options *opt,
+                              int precision, uint32_t seed, int64_t *id);
+/* Block until request `id` finishes; basis n x cap column-major. */
+int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cyclic_result *res,
+                   double *latency_s, double *queued_s);
+/* Completed requests, returned vectors, latency percentiles (s), wall time
+ * from creation (or the last reset) to the last completion. */
+int kh_server_stats(kh_server *s, int64_t *done, int64_t *vectors, double *p50, double *p99,
+                    double *wall_s);
+int kh_server_reset_clock(kh_server *s);
+
 /* --- device buffers and kernel entry points (tests, benchmarks) ---------- */
 int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out);
 int kh_dev_free(kh_context *ctx, void *p);

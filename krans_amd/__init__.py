@@ -103,6 +103,17 @@ SIGNATURES = {
                                 C.POINTER(_CyclicRes)]),
     "kh_lobpcg": (I32, [P, P, C.POINTER(_LobpcgOpts), I32, P, I64, P, I64, P, P,
                         C.POINTER(_LobpcgRes)]),
+    "kh_server_create": (I32, [I32, I32, D, C.POINTER(P)]),
+    "kh_server_destroy": (I32, [P]),
+    "kh_server_submit_cyclic": (I32, [P, I64, I64, P, P, P, C.POINTER(_CyclicOpts), I32,
+                                      C.c_uint32, C.POINTER(I64)]),
+    "kh_server_submit_resident": (I32, [P, P, C.POINTER(_CyclicOpts), I32, C.c_uint32,
+                                        C.POINTER(I64)]),
+    "kh_server_wait": (I32, [P, I64, P, I64, C.POINTER(_CyclicRes), C.POINTER(D),
+                             C.POINTER(D)]),
+    "kh_server_stats": (I32, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(D), C.POINTER(D),
+                              C.POINTER(D)]),
+    "kh_server_reset_clock": (I32, [P]),
     "kh_dev_alloc": (I32, [P, SZ, C.POINTER(P)]),
     "kh_dev_free": (I32, [P, P]),
     "kh_h2d": (I32, [P, P, P, SZ]),
@@ -490,3 +501,68 @@ def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
 def find_nullspace_lobpcg(A, vf, opt, precision="fp64"):
     """Backwards-compatible wrapper (LOBPCGSolver.hpp:147-149)."""
     return find_nullspace_lobpcg_ex(A, vf, opt, precision).right_null
+
+
+class NullspaceServer:
+    """Continuous-batching null-space server (kh_server_*): requests are
+    admitted in arrival order when one of `slots` solver slots (one device
+    stream each) is free and their HBM footprint fits the budget."""
+
+    def __init__(self, slots=4, hbm_budget_bytes=0.0, device=0):
+        h = P()
+        _check(lib().kh_server_create(device, slots, float(hbm_budget_bytes), C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, A, opt, precision="fp64", seed=42):
+        """A: object with m, n, rowptr, col, val (host CSR) or a
+        HipCsrMatrixAdapter (resident). Returns a request id."""
+        o = opt._c()
+        rid = I64()
+        if isinstance(A, HipCsrMatrixAdapter):
+            _check(lib().kh_server_submit_resident(self.h, A.h, C.byref(o), PREC[precision],
+                                                   seed, C.byref(rid)))
+            n = A.n
+        else:
+            rp = np.ascontiguousarray(A.rowptr, dtype=np.int32)
+            ci = np.ascontiguousarray(A.col, dtype=np.int32)
+            va = np.ascontiguousarray(A.val, dtype=np.float64)
+            _check(lib().kh_server_submit_cyclic(self.h, A.m, A.n, _ptr(rp), _ptr(ci), _ptr(va),
+                                                 C.byref(o), PREC[precision], seed,
+                                                 C.byref(rid)))
+            n = A.n
+        self._pending[rid.value] = (n, max(int(opt.k), int(opt.nullity_known), 1))
+        return rid.value
+
+    def wait(self, rid):
+        """Returns (basis n x k, CyclicLanczosResult-like dict, latency_s, queued_s)."""
+        n, kcap = self._pending.pop(rid)
+        R = np.zeros((kcap, n))
+        res = _CyclicRes()
+        lat, q = D(), D()
+        _check(lib().kh_server_wait(self.h, rid, _ptr(R), kcap, C.byref(res), C.byref(lat),
+                                    C.byref(q)))
+        info = {f[0]: getattr(res, f[0]) for f in _CyclicRes._fields_}
+        return R[:res.n_right].T.copy(), info, lat.value, q.value
+
+    def stats(self):
+        done, vec = I64(), I64()
+        p50, p99, wall = D(), D(), D()
+        _check(lib().kh_server_stats(self.h, C.byref(done), C.byref(vec), C.byref(p50),
+                                     C.byref(p99), C.byref(wall)))
+        return {"done": done.value, "vectors": vec.value, "p50_s": p50.value,
+                "p99_s": p99.value, "wall_s": wall.value}
+
+    def reset_clock(self):
+        _check(lib().kh_server_reset_clock(self.h))

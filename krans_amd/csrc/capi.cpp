@@ -7,7 +7,12 @@
 #include "../../include/krans_hip.h"
 #include "capi_internal.hpp"
 #include "dense.hpp"
+#include "server.hpp"
 #include "solver.hpp"
+
+struct kh_server_s {
+  std::unique_ptr<kh::Server> srv;
+};
 
 namespace {
 thread_local std::string g_err;
@@ -338,6 +343,99 @@ int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt, int preci
   });
 }
 
+int kh_server_create(int device, int slots, double hbm_budget_bytes, kh_server **out) {
+  return guard([&] {
+    need(out, "null out");
+    auto s = std::make_unique<kh_server_s>();
+    s->srv = std::make_unique<kh::Server>(device, slots, hbm_budget_bytes);
+    *out = s.release();
+  });
+}
+
+int kh_server_destroy(kh_server *s) {
+  return guard([&] { delete s; });
+}
+
+int kh_server_submit_cyclic(kh_server *s, int64_t m, int64_t n, const int32_t *rowptr,
+                            const int32_t *col, const double *val, const kh_cyclic_options *opt,
+                            int precision, uint32_t seed, int64_t *id) {
+  return guard([&] {
+    need(s && opt && id, "null argument");
+    *id = s->srv->submit(m, n, rowptr, col, val, to_params(*opt, precision), seed);
+  });
+}
+
+int kh_server_submit_resident(kh_server *s, kh_matrix *A, const kh_cyclic_options *opt,
+                              int precision, uint32_t seed, int64_t *id) {
+  return guard([&] {
+    need(s && A && opt && id, "null argument");
+    *id = s->srv->submit_resident(A->A.get(), to_params(*opt, precision), seed);
+  });
+}
+
+int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cyclic_result *res,
+                   double *latency_s, double *queued_s) {
+  return guard([&] {
+    need(s && res, "null argument");
+    kh::Server::Result r = s->srv->wait(id);
+    std::memset(res, 0, sizeof(*res));
+    if (r.status == 1)
+      throw std::invalid_argument(r.error);
+    if (r.status == 3)
+      throw kh::DeviceOOM(r.error);
+    if (r.status != 0)
+      throw std::runtime_error(r.error);
+    res->n_right = r.count;
+    if (right)
+      std::memcpy(right, r.basis.data(),
+                  (std::size_t)(std::min(r.count, cap) * r.n) * sizeof(double));
+    res->restarts = r.stats.restarts;
+    res->krylov_steps_total = r.stats.krylov_steps_total;
+    res->frob_norm_est = r.stats.frob_norm_est;
+    res->converged_to_target = r.stats.converged_to_target;
+    res->cg_solves = r.stats.cg_solves;
+    res->cg_maxit_hits = r.stats.cg_maxit_hits;
+    res->spmv = r.stats.spmv;
+    res->spmtv = r.stats.spmtv;
+    res->cg_iters = r.stats.cg_iters;
+    res->t_gk = r.stats.t.gk;
+    res->t_tiny = r.stats.t.tiny;
+    res->t_cand = r.stats.t.cand;
+    res->t_polish = r.stats.t.polish;
+    res->t_fallback = r.stats.t.fallback;
+    res->t_rng = r.stats.t.rng;
+    if (latency_s)
+      *latency_s = r.latency;
+    if (queued_s)
+      *queued_s = r.queued;
+  });
+}
+
+int kh_server_stats(kh_server *s, int64_t *done, int64_t *vectors, double *p50, double *p99,
+                    double *wall_s) {
+  return guard([&] {
+    need(s, "null server");
+    kh::Server::Stats st = s->srv->stats();
+    if (done)
+      *done = st.done;
+    if (vectors)
+      *vectors = st.vectors;
+    if (p50)
+      *p50 = st.p50;
+    if (p99)
+      *p99 = st.p99;
+    if (wall_s)
+      *wall_s = st.wall;
+  });
+}
+
+int kh_server_reset_clock(kh_server *s) {
+  return guard([&] {
+    need(s, "null server");
+    s->srv->reset_clock();
+  });
+}
+
 int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out) {
   return guard([&] {
     need(ctx && out, "null argument");
@@ -454,8 +552,14 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
     auto &c = *ctx->ctx;
     const kh::Prec p = prec_of(precision);
     kh::DBuf d((std::size_t)k * k * 2 * 8);
-    kh::gemm_tn(c, p, Z, len, k, p, Z, len, k, len, d.as<double>());
-    kh::gemm_tn(c, p, Z, len, k, p, AZ, len, k, len, d.as<double>() + (std::size_t)k * k);
+    if (p == kh::Prec::F32) {  // FP32 MFMA tiles
+      kh::gram_mfma(c, (const float *)Z, len, k, (const float *)Z, len, k, len, d.as<double>());
+      kh::gram_mfma(c, (const float *)Z, len, k, (const float *)AZ, len, k, len,
+                    d.as<double>() + (std::size_t)k * k);
+    } else {
+      kh::gemm_tn(c, p, Z, len, k, p, Z, len, k, len, d.as<double>());
+      kh::gemm_tn(c, p, Z, len, k, p, AZ, len, k, len, d.as<double>() + (std::size_t)k * k);
+    }
     KH_CHECK(hipMemcpyAsync(S, d.p, (std::size_t)k * k * 8, hipMemcpyDeviceToHost, c.stream));
     KH_CHECK(hipMemcpyAsync(G, d.as<double>() + (std::size_t)k * k, (std::size_t)k * k * 8,
                             hipMemcpyDeviceToHost, c.stream));

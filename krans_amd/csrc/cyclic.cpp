@@ -837,6 +837,22 @@ DeviceBasis CyclicDriver::run() {
 
 }  // namespace
 
+double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P) {
+  const int64_t k = std::max<int64_t>(1, P.nullity_known >= 0 ? P.nullity_known : P.k);
+  const int64_t ncv = std::max<int64_t>(std::max<int64_t>(2, P.ncv), k + 2);
+  const double ke = P.krylov == Prec::F64 ? 8.0 : 4.0;
+  const double cap = (double)(ncv / 2 + 2);
+  const double ldy = (double)pad_ld(n, (std::size_t)ke), ldx = (double)pad_ld(m, (std::size_t)ke);
+  double b = 8.0 * k * n;                              // locked basis
+  b += (cap + k) * ldy * ke + cap * ldx * ke;          // [Y | basis], X
+  b += ke * (m + n) + 8.0 * (2 * n + m + n);           // w_x, w_y, y0, best_v, tmp
+  const double w = (double)std::min<int64_t>(64, std::max<int64_t>(8, std::min(ncv, std::max<int64_t>(k, 8))));
+  b += w * 8.0 * (5.0 * n + 5.0 * m);                  // candidate windows
+  b += w * 8.0 * (4.0 * m + n);                        // CG state
+  b += (P.polish == Prec::F32 ? w * 8.0 * m : 0.0);
+  return b + (double)(64 << 20);                       // scalars, partials, slack
+}
+
 DeviceBasis cyclic_right(Context &ctx, const DeviceOp &op, RefStream &rng,
                          const CyclicParams &P, CyclicStats &S) {
   CyclicDriver d(ctx, op, rng, P, S);

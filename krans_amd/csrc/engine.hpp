@@ -141,6 +141,10 @@ void gemm_nn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
 // H(:,c) = V^T W(:,c) for c < q; H is s x q (column-major) FP64 on device.
 void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
              const void *W, int64_t ldw, int q, int64_t len, double *H);
+// H = V^T W for FP32 V, W on the matrix cores (FP32 MFMA accumulation within
+// a row chunk, FP64 sums across chunks): the LOBPCG Gram blocks.
+void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
+               int q, int64_t len, double *H);
 // W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.
 void gemm_nn_sub(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
                  const double *H, int q, Prec pw, void *W, int64_t ldw,

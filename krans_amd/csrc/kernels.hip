@@ -635,6 +635,98 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ----------------------------------------------------- FP32 MFMA Gram tiles --
+// H(s x q) = V^T W for FP32 V, W on the matrix cores: v_mfma_f32_16x16x4f32.
+// A block owns a contiguous chunk of rows and one 64 x 64 tile of H
+// (blockIdx.y, blockIdx.z); 64-row slabs of V and W are staged in LDS with
+// coalesced column reads; wave w accumulates H rows [16w, 16w+16) against the
+// four 16-column tiles. MFMA lane layout (16x16x4): A[i][k] from lane
+// i + 16k, B[k][j] from lane j + 16k, D[4*(l/16)+v][l%16] in element v of
+// lane l. Block partials (FP32) are summed in FP64 in block order.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMT = 64, kMSlab = 64;
+
+__global__ void __launch_bounds__(kBlock)
+    k_gram_mfma_partial(int64_t len, int64_t rows_per_block, const float *__restrict__ V,
+                        int64_t ldv, int s, const float *__restrict__ W, int64_t ldw, int q,
+                        float *__restrict__ part) {
+  __shared__ float Vs[kMSlab][kMT + 4];
+  __shared__ float Ws[kMSlab][kMT + 4];
+  const int j0 = blockIdx.y * kMT, c0 = blockIdx.z * kMT;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
+  floatx4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  constexpr int PER = kMSlab * kMT / kBlock;  // 16 elements per thread per slab
+  for (int64_t rb = r0; rb < r1; rb += kMSlab) {
+    float va[PER], wa[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * kBlock;
+      const int r = e % kMSlab, j = e / kMSlab;
+      const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
+      const int jv = (j0 + j) < s ? (j0 + j) : (s - 1);
+      const int jw = (c0 + j) < q ? (c0 + j) : (q - 1);
+      va[u] = V[row + (int64_t)jv * ldv];
+      wa[u] = W[row + (int64_t)jw * ldw];
+    }
+    __syncthreads();  // previous slab fully consumed
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * kBlock;
+      const int r = e % kMSlab, j = e / kMSlab;
+      const bool inr = (rb + r) < r1;
+      Vs[r][j] = (inr && j0 + j < s) ? va[u] : 0.f;
+      Ws[r][j] = (inr && c0 + j < q) ? wa[u] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k4 = 0; k4 < kMSlab / 4; ++k4) {
+      const int kr = k4 * 4 + lane / 16;
+      const float a = Vs[kr][16 * wid + (lane & 15)];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float b = Ws[kr][16 * t + (lane & 15)];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int64_t tile = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  float *out = part + tile * (kMT * kMT);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = 16 * wid + 4 * (lane / 16) + v;  // H row (V column)
+      const int col = 16 * t + (lane & 15);            // H column (W column)
+      out[col * kMT + row] = acc[t][v];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_gram_finish(const float *__restrict__ part, int nblk, int ty_tiles, int s, int q,
+                  double *__restrict__ H) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t per_tile = kMT * kMT;
+  const int64_t tiles = (int64_t)ty_tiles * ((q + kMT - 1) / kMT);
+  if (e >= tiles * per_tile)
+    return;
+  const int64_t t = e / per_tile, w = e % per_tile;
+  const int tyi = (int)(t % ty_tiles), tzi = (int)(t / ty_tiles);
+  const int jj = (int)(w % kMT), cc = (int)(w / kMT);
+  const int j = tyi * kMT + jj, c = tzi * kMT + cc;
+  if (j >= s || c >= q)
+    return;
+  const float *p = part + ((int64_t)tzi * ty_tiles + tyi) * nblk * per_tile + w;
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b)
+    acc += (double)p[(int64_t)b * per_tile];
+  H[j + (int64_t)c * s] = acc;
+}
+
 // ---------------------------------------------------------------- level 1 --
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -1171,6 +1263,26 @@ static void launch_gemm_tn(Context &ctx, const TV *V, int64_t ldv, int s, const 
   const int64_t outs = (int64_t)ty * tz * kTS * kTS;
   hipLaunchKernelGGL(k_ts_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, ctx.stream, part, nblk, ty, s, q, H);
+  KH_CHECK(hipGetLastError());
+}
+
+void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
+               int q, int64_t len, double *H) {
+  if (s <= 0 || q <= 0 || len <= 0)
+    return;
+  const int ty = (s + kMT - 1) / kMT, tz = (q + kMT - 1) / kMT;
+  int nblk = (int)std::min<int64_t>(512, std::max<int64_t>(1, (len + 4095) / 4096));
+  int64_t rpb = (len + nblk - 1) / nblk;
+  rpb = (rpb + kMSlab - 1) / kMSlab * kMSlab;
+  nblk = (int)std::max<int64_t>(1, (len + rpb - 1) / rpb);
+  const std::size_t need = (std::size_t)nblk * ty * tz * kMT * kMT;  // floats
+  float *part = reinterpret_cast<float *>(ctx.partials((need + 1) / 2));
+  TraceScope tr(ctx, kTraceVtW, 4.0 * (double)len * (s + q));
+  hipLaunchKernelGGL(k_gram_mfma_partial, dim3(nblk, ty, tz), dim3(kBlock), 0, ctx.stream, len,
+                     rpb, V, ldv, s, W, ldw, q, part);
+  const int64_t outs = (int64_t)ty * tz * kMT * kMT;
+  hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, ctx.stream, part, nblk, ty, s, q, H);
   KH_CHECK(hipGetLastError());
 }
 

@@ -6,11 +6,11 @@
 namespace kh {
 
 namespace {
-constexpr std::size_t kRing = std::size_t(1) << 22;  // 4 Mi values ahead
 constexpr std::size_t kChunk = std::size_t(1) << 15;
 }  // namespace
 
-RefStream::RefStream(unsigned seed) : gen_(seed), dist_(0.0, 1.0), ring_(kRing) {
+RefStream::RefStream(unsigned seed, std::size_t prefetch)
+    : gen_(seed), dist_(0.0, 1.0), ring_(std::max(prefetch, kChunk)), cap_(ring_.size()) {
   th_ = std::thread([this] { produce(); });
 }
 
@@ -29,20 +29,20 @@ void RefStream::produce() {
     std::size_t room;
     {
       std::unique_lock<std::mutex> lk(mu_);
-      cv_.wait(lk, [&] { return stop_ || count_ < kRing; });
+      cv_.wait(lk, [&] { return stop_ || count_ < cap_; });
       if (stop_)
         return;
-      room = std::min(kChunk, kRing - count_);
+      room = std::min(kChunk, cap_ - count_);
     }
     // generate outside the lock: gen_/dist_ belong to this thread only
     for (std::size_t i = 0; i < room; ++i)
       tmp[i] = dist_(gen_);
     {
       std::lock_guard<std::mutex> lk(mu_);
-      std::size_t tail = (head_ + count_) % kRing;
+      std::size_t tail = (head_ + count_) % cap_;
       for (std::size_t i = 0; i < room; ++i) {
         ring_[tail] = tmp[i];
-        tail = (tail + 1 == kRing) ? 0 : tail + 1;
+        tail = (tail + 1 == cap_) ? 0 : tail + 1;
       }
       count_ += room;
     }
@@ -56,11 +56,11 @@ void RefStream::fill(double *out, std::size_t n) {
     std::unique_lock<std::mutex> lk(mu_);
     cv_.wait(lk, [&] { return count_ > 0; });
     const std::size_t take = std::min(count_, n - got);
-    const std::size_t first = std::min(take, kRing - head_);
+    const std::size_t first = std::min(take, cap_ - head_);
     std::copy(ring_.begin() + (long)head_, ring_.begin() + (long)(head_ + first), out + got);
     if (take > first)
       std::copy(ring_.begin(), ring_.begin() + (long)(take - first), out + got + first);
-    head_ = (head_ + take) % kRing;
+    head_ = (head_ + take) % cap_;
     count_ -= take;
     got += take;
     lk.unlock();

@@ -71,7 +71,7 @@ struct CsrOp final : DeviceOp {
 // overlaps the device work.
 class RefStream {
 public:
-  explicit RefStream(unsigned seed);
+  explicit RefStream(unsigned seed, std::size_t prefetch = std::size_t(1) << 22);
   ~RefStream();
   RefStream(const RefStream &) = delete;
   RefStream &operator=(const RefStream &) = delete;
@@ -82,7 +82,7 @@ private:
   std::mt19937 gen_;
   std::normal_distribution<double> dist_;
   std::vector<double> ring_;
-  std::size_t head_ = 0, count_ = 0;
+  std::size_t cap_ = 0, head_ = 0, count_ = 0;
   bool stop_ = false;
   std::mutex mu_;
   std::condition_variable cv_;
@@ -137,6 +137,10 @@ struct DeviceBasis {
 // transposed operator.
 DeviceBasis cyclic_right(Context &ctx, const DeviceOp &op, RefStream &rng,
                          const CyclicParams &P, CyclicStats &S);
+
+// Device bytes the cyclic driver allocates for an m x n operator (the
+// admission footprint of the serving loop, excluding the matrix).
+double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P);
 
 // LOBPCGOptions (LOBPCGSolver.hpp:71-115).
 struct LobpcgParams {

@@ -1,0 +1,83 @@
+"""Continuous-batching server: each result is the same alone and in the mix
+(bitwise), admission by HBM footprint, error isolation."""
+import numpy as np
+import pytest
+
+import krans_amd as K
+from krans_amd import workloads as W
+from conftest import principal_sine
+
+pytestmark = pytest.mark.gpu
+
+
+def mix():
+    """Uneven requests from the three generator families."""
+    reqs = []
+    reqs.append((W.regular_incidence(8, 2048, seed=1), 8, "mixed"))
+    reqs.append((W.shifted_difference(20000, 5, 0.7), 5, "fp64"))
+    reqs.append((W.regular_incidence(3, 4096, seed=2), 3, "fp64"))
+    reqs.append((W.rg_block(5000, p=4, offdiag=8, gnnz=16, seed=3)[0], 4, "mixed"))
+    reqs.append((W.grid_incidence(2, 48, seed=4, weighted=False), 2, "mixed"))
+    reqs.append((W.shifted_difference(50000, 2, 0.5), 2, "mixed"))
+    reqs.append((W.regular_incidence(16, 1024, seed=5), 16, "fp64"))
+    reqs.append((W.regular_incidence(4, 8192, seed=6), 4, "mixed"))
+    return reqs
+
+
+def opts(k):
+    return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
+                                  max_restarts=200, cg_maxit=400)
+
+
+def test_same_alone_and_in_the_mix():
+    reqs = mix()
+    print("[case] server alone", flush=True)
+    alone = []
+    for i, (A, k, prec) in enumerate(reqs):
+        r = K.find_nullspace_cyclic_lanczos_ex(K.HipCsrMatrixAdapter.from_csr(A),
+                                               K.HipVectorFactory(A.n, 100 + i), opts(k), prec)
+        alone.append(r.right_null)
+        assert r.right_null.shape[1] == k, (i, r.right_null.shape)
+    print("[case] server mix", flush=True)
+    srv = K.NullspaceServer(slots=4)
+    ids = [srv.submit(A, opts(k), prec, seed=100 + i) for i, (A, k, prec) in enumerate(reqs)]
+    for i, rid in enumerate(ids):
+        U, info, lat, q = srv.wait(rid)
+        assert U.shape == alone[i].shape
+        assert np.array_equal(U.view(np.uint64), alone[i].view(np.uint64)), i
+        assert lat >= q >= 0
+    st = srv.stats()
+    assert st["done"] == len(reqs) and st["p99_s"] >= st["p50_s"] > 0
+    srv.close()
+
+
+def test_admission_under_a_small_budget_and_resident_matrices():
+    A = W.regular_incidence(4, 4096, seed=7)
+    H = K.HipCsrMatrixAdapter.from_csr(A)
+    print("[case] server budget", flush=True)
+    # room for about one request at a time: the rest wait in the queue
+    srv = K.NullspaceServer(slots=4, hbm_budget_bytes=1.6e8)
+    ids = [srv.submit(H, opts(4), "mixed", seed=s) for s in (1, 2, 3)]
+    outs = [srv.wait(r) for r in ids]
+    Kx = W.component_indicators(4, 4096)
+    for U, info, lat, q in outs:
+        assert U.shape[1] == 4 and principal_sine(U, Kx) <= 5e-9
+    assert max(o[3] for o in outs) > 0.0  # somebody queued
+    with pytest.raises(ValueError):
+        srv.submit(W.regular_incidence(64, 16384, seed=1), opts(64), "fp64")
+    srv.close()
+
+
+def test_bad_request_fails_alone():
+    srv = K.NullspaceServer(slots=2)
+    bad = W.CsrMatrix(2, 2, np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32),
+                      np.ones(2))
+    good = W.regular_incidence(2, 512, seed=8)
+    print("[case] server bad request", flush=True)
+    rb = srv.submit(bad, opts(1), "fp64", seed=1)
+    rg = srv.submit(good, opts(2), "fp64", seed=1)
+    with pytest.raises(ValueError):
+        srv.wait(rb)
+    U, _, _, _ = srv.wait(rg)
+    assert U.shape[1] == 2
+    srv.close()
