@@ -38,6 +38,11 @@ def parse():
                     help="approximate budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="serving mode: number of requests of the mixed stream")
+    ap.add_argument("--serve-nmax", type=float, default=1e6)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per s")
+    ap.add_argument("--serve-slots", type=int, default=4)
     return ap.parse_args()
 
 
@@ -56,6 +61,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+    if args.serve > 0:
+        serve(args)
+        return
     import krans_amd as K
     from krans_amd import workloads as W
 
@@ -198,6 +206,111 @@ def main():
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line))
+
+
+def serving_requests(count, nmax, seed=7):
+    """The serving stream: each request draws uniformly from the three
+    generator families (shifted difference, incidence of random 4-regular
+    graphs, [R | RG]); n log-uniform in [1e4, nmax], kernel dimension uniform
+    in 1..32, tolerance 1e-12."""
+    from krans_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        fam = int(rng.integers(0, 3))
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(nmax))))
+        k = int(rng.integers(1, 33))
+        if fam == 0:
+            A = W.shifted_difference(n, k, 0.5)
+        elif fam == 1:
+            verts = max(64, n // k)
+            A = W.regular_incidence(k, verts, seed=1000 + i)
+        else:
+            A = W.rg_block(max(n - k, 64), p=k, offdiag=8, gnnz=16, seed=1000 + i)[0]
+        out.append((["shiftdiff", "incidence", "rg"][fam], A, k))
+    return out
+
+
+def serve(args):
+    """Serving stream on one GPU: the continuous-batching server (mixed
+    precision, --serve-slots concurrent solves, HBM-footprint admission)
+    against plain FP64 one-at-a-time solves of the same requests."""
+    import threading
+    import krans_amd as K
+    reqs = serving_requests(args.serve, args.serve_nmax)
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(reqs)))
+
+    def opt(k):
+        return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
+                                      max_restarts=MAX_RESTARTS, cg_maxit=1000)
+
+    # batched mixed-precision server
+    srv = K.NullspaceServer(slots=args.serve_slots)
+    srv.reset_clock()
+    ids = []
+    t0 = time.perf_counter()
+    for (fam, A, k), ta in zip(reqs, arrivals):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(A, opt(k), "mixed", seed=42))
+    res = []
+    for i, r in enumerate(ids):
+        res.append(srv.wait(r))
+        print(f"[serve] done {i + 1}/{len(ids)} latency {res[-1][2]:.3f}s", file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    found = sum(r[0].shape[1] for r in res)
+    want = sum(k for _, _, k in reqs)
+    worst = 0.0
+    for (fam, A, k), (U, info, lat, q) in zip(reqs, res):
+        S = A.to_scipy()
+        for j in range(U.shape[1]):
+            worst = max(worst, np.linalg.norm(S @ U[:, j]) / A.fro())
+    lat = sorted(r[2] for r in res)
+
+    # plain FP64, one request at a time on the default context
+    t1 = time.perf_counter()
+    serial_found, serial_t = 0, []
+    for fam, A, k in reqs:
+        H = K.HipCsrMatrixAdapter.from_csr(A)
+        ts = time.perf_counter()
+        r = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 42), opt(k), "fp64")
+        serial_t.append(time.perf_counter() - ts)
+        serial_found += r.right_null.shape[1]
+        print(f"[serial fp64] {len(serial_t)}/{len(reqs)} {fam} n={A.n} k={k} "
+              f"{serial_t[-1]:.3f}s found={r.right_null.shape[1]}", file=sys.stderr, flush=True)
+    serial_wall = time.perf_counter() - t1
+
+    def pct(xs, q):
+        xs = sorted(xs)
+        pos = q * (len(xs) - 1)
+        lo = int(np.floor(pos))
+        hi = min(lo + 1, len(xs) - 1)
+        return xs[lo] + (xs[hi] - xs[lo]) * (pos - lo)
+
+    value = found / wall
+    print(json.dumps({
+        "metric": "null vectors/sec + p99 latency at ||Au||/||A||_F<=1e-12, serving stream, "
+                  "batched mixed vs one-at-a-time FP64, 1 MI355X",
+        "value": round(value, 3), "unit": "null vectors/s", "n_gpus": 1,
+        "requests": len(reqs), "rate_per_s": args.serve_rate, "slots": args.serve_slots,
+        "n_range": [1e4, args.serve_nmax],
+        "families": {f: sum(1 for r in reqs if r[0] == f) for f in ("shiftdiff", "incidence", "rg")},
+        "vectors": {"found": found, "requested": want},
+        "latency_s": {"p50": round(pct(lat, 0.5), 4), "p99": round(pct(lat, 0.99), 4)},
+        "server_stats": st,
+        "wall_s": round(wall, 3),
+        "max_res_rel": worst,
+        "fp64_serial": {"value": round(serial_found / serial_wall, 3), "found": serial_found,
+                        "wall_s": round(serial_wall, 3),
+                        "service_s": {"p50": round(pct(serial_t, 0.5), 4),
+                                      "p99": round(pct(serial_t, 0.99), 4)}},
+        "batched_over_fp64_serial": round(value / (serial_found / serial_wall), 3),
+    }))
 
 
 def copy_probe(K, ctx, mib=256):
