@@ -296,35 +296,9 @@ void CyclicDriver::upload_mask(const std::vector<int> &mask) {
   KH_CHECK(hipStreamSynchronize(c_.stream));
 }
 
-// (Preconditioned) CG on (A A^T) W = RHS for q columns, every column the
-// recurrence of cg_normal_eq_prec (CyclicLanczosSolver.cpp:48-113). The host
-// polls the device-resident states once per chunk of iterations.
 void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
                             int64_t maxit, double rel_tol, std::vector<CgState> &st) {
-  cg_.ensure(q, m_, n_, pc);
-  cg_init(c_, cg_, rhs, w, diag, rel_tol, maxit);
-  st.resize((std::size_t)q);
-  int64_t iters_done = 0;
-  int chunk = 4;
-  while (true) {
-    KH_CHECK(hipMemcpyAsync(st.data(), cg_.st.p, sizeof(CgState) * (std::size_t)q,
-                            hipMemcpyDeviceToHost, c_.stream));
-    KH_CHECK(hipStreamSynchronize(c_.stream));
-    int running = 0;
-    for (int c = 0; c < q; ++c)
-      running += st[(std::size_t)c].done == 0;
-    if (running == 0 || iters_done >= maxit)
-      break;
-    const int it = (int)std::min<int64_t>(chunk, maxit - iters_done);
-    const int *act = cg_.act.as<int>();
-    for (int k = 0; k < it; ++k) {
-      A_.mtm(c_, pc, q, cg_.pv.p, m_, cg_.tmp_n.p, n_, act);
-      A_.mm(c_, pc, q, cg_.tmp_n.p, n_, cg_.Ap.p, m_, act);
-      cg_after_products(c_, cg_, w, diag, maxit, P_.cg_breakdown_tol);
-    }
-    iters_done += it;
-    chunk = std::min(chunk * 2, 16);
-  }
+  cg_solve_ops(c_, A_, cg_, q, rhs, w, pc, diag, maxit, rel_tol, P_.cg_breakdown_tol, st);
   for (int c = 0; c < q; ++c) {
     S_.cg_iters += st[(std::size_t)c].k;
     S_.spmv += st[(std::size_t)c].k;
@@ -836,6 +810,36 @@ DeviceBasis CyclicDriver::run() {
 }
 
 }  // namespace
+
+void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
+                  Prec p, const double *diag, int64_t maxit, double rel_tol,
+                  double breakdown_tol, std::vector<CgState> &st) {
+  const int64_t m = A.rows(), n = A.cols();
+  B.ensure(q, m, n, p);
+  cg_init(c, B, rhs, w, diag, rel_tol, maxit);
+  st.resize((std::size_t)q);
+  int64_t iters_done = 0;
+  int chunk = 4;
+  while (true) {
+    KH_CHECK(hipMemcpyAsync(st.data(), B.st.p, sizeof(CgState) * (std::size_t)q,
+                            hipMemcpyDeviceToHost, c.stream));
+    KH_CHECK(hipStreamSynchronize(c.stream));
+    int running = 0;
+    for (int k = 0; k < q; ++k)
+      running += st[(std::size_t)k].done == 0;
+    if (running == 0 || iters_done >= maxit)
+      break;
+    const int it = (int)std::min<int64_t>(chunk, maxit - iters_done);
+    const int *act = B.act.as<int>();
+    for (int k = 0; k < it; ++k) {
+      A.mtm(c, p, q, B.pv.p, m, B.tmp_n.p, n, act);
+      A.mm(c, p, q, B.tmp_n.p, n, B.Ap.p, m, act);
+      cg_after_products(c, B, w, diag, maxit, breakdown_tol);
+    }
+    iters_done += it;
+    chunk = std::min(chunk * 2, 16);
+  }
+}
 
 double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P) {
   const int64_t k = std::max<int64_t>(1, P.nullity_known >= 0 ? P.nullity_known : P.k);

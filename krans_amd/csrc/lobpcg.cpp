@@ -146,7 +146,8 @@ private:
   int64_t m_, n_;
   Prec bp_;
   DeviceBasis locked_;  // FP64 accepted vectors (Y_locked and R.right_null)
-  DBuf small_, tmpm_, hbuf_dev_, v64_, v64b_, tm64_, tn64_, vc_;
+  DBuf small_, tmpm_, hbuf_dev_, v64_, v64b_, tm64_, tn64_, vc_, w64_;
+  CgBatch cg_;
   double *sc_ = nullptr;
   std::vector<double> hbuf_;
   Blk tmpq_;  // m x q scratch for apply_normal
@@ -251,8 +252,10 @@ bool LobpcgDriver::orthonormalize(Blk &X, int i0, int cnt, double tol, bool reor
 // :54-64  AZ = A^T (A Z), q columns
 void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
   auto t0 = Clock::now();
-  if (tmpq_.cap < q)
-    tmpq_.init(m_, std::max(q, 8), bp_);
+  if (tmpq_.cap < q || tmpq_.p != Z.p || AZ.p != Z.p)
+    tmpq_.init(m_, std::max(q, 8), Z.p);
+  if (AZ.p != Z.p || AZ.cap < q || Z.cap < q)
+    throw std::logic_error("apply_normal: block precision or width mismatch");
   A_.mm(c_, Z.p, q, Z.col(0), n_, tmpq_.col(0), m_, nullptr);
   A_.mtm(c_, Z.p, q, tmpq_.col(0), m_, AZ.col(0), n_, nullptr);
   S_.spmv += q;
@@ -264,8 +267,15 @@ void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
 void LobpcgDriver::gram(const Blk &Z, const Blk &AZ, int q, dense::Mat &S, dense::Mat &G) {
   auto t0 = Clock::now();
   const int qq = q * q;
-  gemm_tn(c_, Z.p, Z.col(0), n_, q, Z.p, Z.col(0), n_, q, n_, sc_ + 1024);
-  gemm_tn(c_, Z.p, Z.col(0), n_, q, AZ.p, AZ.col(0), n_, q, n_, sc_ + 1024 + qq);
+  if (Z.p == Prec::F32 && AZ.p == Prec::F32) {  // FP32 MFMA tiles
+    gram_mfma(c_, (const float *)Z.col(0), n_, q, (const float *)Z.col(0), n_, q, n_,
+              sc_ + 1024);
+    gram_mfma(c_, (const float *)Z.col(0), n_, q, (const float *)AZ.col(0), n_, q, n_,
+              sc_ + 1024 + qq);
+  } else {
+    gemm_tn(c_, Z.p, Z.col(0), n_, q, Z.p, Z.col(0), n_, q, n_, sc_ + 1024);
+    gemm_tn(c_, Z.p, Z.col(0), n_, q, AZ.p, AZ.col(0), n_, q, n_, sc_ + 1024 + qq);
+  }
   std::vector<double> h((std::size_t)2 * qq);
   rb(c_, sc_ + 1024, 2 * qq, h.data());
   S = dense::Mat(q, q);
@@ -619,6 +629,11 @@ DeviceBasis LobpcgDriver::run() {
   S_.frob_norm_est = estimate_fro();
   const double accept_thr = std::max(P_.tol_abs, P_.tol_rel * std::max(1.0, S_.frob_norm_est));
   S_.accept_threshold = accept_thr;
+  // FP32 blocks cannot resolve ||A x|| below ~1e-6 ||A||: such iterates become
+  // candidates and are refined in FP64 before the acceptance test
+  const double cand_thr =
+      bp_ == Prec::F32 ? std::max(accept_thr, 1e-5 * std::max(1.0, S_.frob_norm_est)) : accept_thr;
+  w64_ = DBuf((std::size_t)m_ * 8);
 
   Blk X, AX, Rb, W, Pb, Z, AZ, scratch, Xn;
   X.init(n_, B, bp_);
@@ -689,7 +704,7 @@ DeviceBasis LobpcgDriver::run() {
     double best_unacc = std::numeric_limits<double>::infinity();
     for (int j = 0; j < B; ++j) {
       const double Av_orig = avo[(std::size_t)j];
-      if (Av_orig <= accept_thr) {
+      if (Av_orig <= cand_thr) {
         double *v = vc_.as<double>();
         to64(X.col(j), bp_, v);
         project_out(v, Prec::F64);
@@ -710,6 +725,26 @@ DeviceBasis LobpcgDriver::run() {
                 break;
               }
           }
+        }
+        if (!reject && bp_ == Prec::F32) {
+          // FP32 iterate: one FP64 projector pass v <- v - A^T (A A^T)^{-1} A v
+          // (iterative refinement), then the reference's FP64 acceptance test
+          A_.mv(c_, Prec::F64, v, tm64_.p);
+          ++S_.spmv;
+          std::vector<CgState> st;
+          cg_solve_ops(c_, A_, cg_, 1, tm64_.p, w64_.p, Prec::F64, nullptr, 1000, 1e-12, 1e-30,
+                       st);
+          S_.spmv += st[0].k;
+          S_.spmtv += st[0].k;
+          A_.mtv(c_, Prec::F64, w64_.p, tn64_.p);
+          ++S_.spmtv;
+          axpby(c_, Prec::F64, n_, -1.0, tn64_.p, 1.0, v);
+          project_out(v, Prec::F64);
+          const double nr = nrm(Prec::F64, v, n_);
+          if (!(nr > 1e-14))
+            reject = true;
+          else
+            axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nr, v);
         }
         double Av_acc = std::numeric_limits<double>::infinity();
         if (!reject) {
@@ -842,8 +877,6 @@ DeviceBasis LobpcgDriver::run() {
 
 DeviceBasis lobpcg(Context &ctx, const DeviceOp &op, RefStream &rng, const LobpcgParams &P,
                    LobpcgStats &S) {
-  if (P.block != Prec::F64)
-    throw std::invalid_argument("LOBPCG: mixed precision blocks are not available yet");
   LobpcgDriver d(ctx, op, rng, P, S);
   return d.run();
 }

@@ -126,6 +126,14 @@ struct CyclicStats {
   PhaseTimes t;
 };
 
+// (Preconditioned) CG on (A A^T) W = RHS for q columns (column-major, length
+// rows), each column the recurrence of cg_normal_eq_prec
+// (CyclicLanczosSolver.cpp:48-113); the host polls the device-resident
+// states once per chunk of iterations. Returns the per-column states.
+void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
+                  Prec p, const double *diag, int64_t maxit, double rel_tol,
+                  double breakdown_tol, std::vector<CgState> &st);
+
 // A device block of `count` FP64 vectors of length `len` (column-major).
 struct DeviceBasis {
   int64_t len = 0, count = 0, cap = 0;

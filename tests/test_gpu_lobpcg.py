@@ -93,3 +93,26 @@ def test_cxx_quickstart_drop_in():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "cyclic   3" in r.stdout and "lobpcg   3" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["rg", "shiftdiff"])
+def test_lobpcg_mixed_precision(case):
+    """FP32 blocks with FP32-MFMA Gram products; candidates are refined by an
+    FP64 projector pass before the reference's FP64 acceptance test."""
+    if case == "rg":
+        A, G = W.rg_block(3000, p=8, offdiag=8, gnnz=16, seed=11)
+        Kx, nev = W.rg_kernel(G), 8
+    else:
+        A = W.shifted_difference(2000, 3, 0.5)
+        Kx, nev = W.shifted_difference_kernel(2000, 3, 0.5), 3
+    print(f"[case] lobpcg mixed {case}", flush=True)
+    r = K.find_nullspace_lobpcg_ex(hip(A), K.HipVectorFactory(A.n, 42),
+                                   K.LOBPCGOptions(nev=nev, block_size=nev, tol_rel=1e-12,
+                                                   tol_abs=0.0, max_iters=3000),
+                                   precision="mixed")
+    U = r.right_null
+    assert U.shape[1] == nev and r.converged_to_target
+    assert residuals(A, U).max() <= r.accept_threshold
+    assert principal_sine(U, Kx) <= 1e-8
+    assert np.abs(U.T @ U - np.eye(nev)).max() <= 1e-10
+    print("mixed iters", r.iters)
