@@ -69,8 +69,22 @@ struct TinyGEP {
   dense::Mat Y;
 };
 
-// :88-119
-TinyGEP solve_tiny_gep(dense::Mat &G, dense::Mat &S) {
+// :88-119. FP64 blocks use the reference's jitter ladder. FP32 blocks carry
+// Gram errors near the FP32 unit roundoff (6e-8), so their ladder starts
+// there: a smaller ridge leaves a nearly dependent [X W P] with a negative
+// pivot, or with a Cholesky factor whose inverse amplifies the FP32 noise in
+// G. The ridge moves each Ritz value theta by about theta * ridge, which is
+// negligible for the near-zero values this solver targets.
+bool try_tiny_gep(dense::Mat &G, dense::Mat &S, bool f32, TinyGEP &out);
+
+TinyGEP solve_tiny_gep(dense::Mat &G, dense::Mat &S, bool f32 = false) {
+  TinyGEP out;
+  if (!try_tiny_gep(G, S, f32, out))
+    throw std::runtime_error("LOBPCG tiny GEP: S not SPD (after jitter)");
+  return out;
+}
+
+bool try_tiny_gep(dense::Mat &G, dense::Mat &S, bool f32, TinyGEP &out) {
   const int k = G.r;
   for (int j = 0; j < k; ++j)
     for (int i = 0; i < j; ++i) {
@@ -83,8 +97,9 @@ TinyGEP solve_tiny_gep(dense::Mat &G, dense::Mat &S) {
   for (double v : S.a)
     fro += v * v;
   const double s_norm = std::max(1.0, std::sqrt(fro));
-  const double jitters[] = {0.0, 1e-14, 1e-12, 1e-10};
-  for (double jv : jitters) {
+  static const double j64[] = {0.0, 1e-14, 1e-12, 1e-10};
+  static const double j32[] = {2e-7, 1e-6, 1e-5, 1e-4};
+  for (double jv : f32 ? j32 : j64) {
     dense::Mat Sj = S;
     if (jv > 0.0)
       for (int i = 0; i < k; ++i)
@@ -113,7 +128,6 @@ TinyGEP solve_tiny_gep(dense::Mat &G, dense::Mat &S) {
         const double c = 0.5 * (C(i, j) + C(j, i));
         C(i, j) = C(j, i) = c;
       }
-    TinyGEP out;
     dense::Mat Q;
     if (!dense::sym_eig(C, out.theta, Q))
       continue;
@@ -125,9 +139,9 @@ TinyGEP solve_tiny_gep(dense::Mat &G, dense::Mat &S) {
           s += Li(t, i) * Q(t, j);
         out.Y(i, j) = s;
       }
-    return out;
+    return true;
   }
-  throw std::runtime_error("LOBPCG tiny GEP: S not SPD (after jitter)");
+  return false;
 }
 
 class LobpcgDriver {
@@ -672,7 +686,7 @@ DeviceBasis LobpcgDriver::run() {
     dense::Mat S0, G0;
     gram(X, AX, B, S0, G0);
     auto th = Clock::now();
-    TinyGEP t = solve_tiny_gep(G0, S0);
+    TinyGEP t = solve_tiny_gep(G0, S0, bp_ == Prec::F32);
     S_.t_host += since(th);
     combine(X, 0, B, t.Y, 0, B, Xn, 0);
     dcopy(X.col(0), Xn.col(0), (std::size_t)B * cb);
@@ -799,7 +813,7 @@ DeviceBasis LobpcgDriver::run() {
       apply_normal(X, B, AX);
       dense::Mat S1, G1;
       gram(X, AX, B, S1, G1);
-      TinyGEP t1 = solve_tiny_gep(G1, S1);
+      TinyGEP t1 = solve_tiny_gep(G1, S1, bp_ == Prec::F32);
       std::fill(theta.begin(), theta.end(), 0.0);
       const int lim = std::min<int>((int)t1.theta.size(), B);
       for (int j = 0; j < lim; ++j)
@@ -827,12 +841,12 @@ DeviceBasis LobpcgDriver::run() {
       dcopy(Pb.col(ini), Z.col(0), (std::size_t)na * cb);
     }
     // Z = [X_act | W_act | P_act] (:693-704)
-    const int np_ = (its > 1) ? na : 0;
+    const int np0 = (its > 1) ? na : 0;
     dcopy(Z.col(0), X.col(ini), (std::size_t)na * cb);
     dcopy(Z.col(na), W.col(ini), (std::size_t)na * cb);
-    if (np_)
+    if (np0)
       dcopy(Z.col(2 * na), Pb.col(ini), (std::size_t)na * cb);
-    const int nv = 2 * na + np_;
+    const int nv = 2 * na + np0;
     if (nv == 0)
       continue;
     if (P_.max_tiny_dim > 0 && nv > P_.max_tiny_dim)
@@ -841,7 +855,23 @@ DeviceBasis LobpcgDriver::run() {
     dense::Mat S, G;
     gram(Z, AZ, nv, S, G);
     auto th = Clock::now();
-    TinyGEP sol = solve_tiny_gep(G, S);
+    TinyGEP sol;
+    int np_ = np0;
+    if (!try_tiny_gep(G, S, bp_ == Prec::F32, sol)) {
+      if (bp_ != Prec::F32 || np_ == 0)
+        throw std::runtime_error("LOBPCG tiny GEP: S not SPD (after jitter)");
+      // FP32 blocks: P has become dependent on [X W] to working precision;
+      // restart this step without it (leading 2na x 2na Gram blocks)
+      np_ = 0;
+      dense::Mat S2(2 * na, 2 * na), G2(2 * na, 2 * na);
+      for (int j = 0; j < 2 * na; ++j)
+        for (int i = 0; i < 2 * na; ++i) {
+          S2(i, j) = S(i, j);
+          G2(i, j) = G(i, j);
+        }
+      sol = solve_tiny_gep(G2, S2, true);
+      ++S_.p_drops;
+    }
     S_.t_host += since(th);
     const int take = std::min<int>(na, (int)sol.theta.size());
     // P_new = [W P] Y[na:, :take]; X_new = X Y[:na, :take] + P_new (:718-759)

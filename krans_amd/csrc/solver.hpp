@@ -173,6 +173,7 @@ struct LobpcgStats {
   double frob_norm_est = 0, accept_threshold = 0, best_unaccepted_residual = 0;
   bool converged_to_target = false;
   int64_t spmv = 0, spmtv = 0;
+  int64_t p_drops = 0;  // FP32 steps restarted without P (dependent [X W P])
   double t_gram = 0, t_sparse = 0, t_host = 0;
 };
 
