@@ -43,6 +43,13 @@ def parse():
     ap.add_argument("--serve-nmax", type=float, default=1e6)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per s")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3a", "c3b", "c4"],
+                    help="c2: the headline incidence solve; c3a/c3b: precision variants "
+                         "(shifted difference n=2^24, weighted grid incidence); c4: LOBPCG "
+                         "on [R | RG] with FP32-MFMA Gram blocks")
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="c3a/c3b/c4 only: shrink the problem size by this factor")
+    ap.add_argument("--max-iters", type=int, default=3000, help="c4: LOBPCG max_iters")
     return ap.parse_args()
 
 
@@ -63,6 +70,9 @@ def main():
         sys.exit(2)
     if args.serve > 0:
         serve(args)
+        return
+    if args.config != "c2":
+        variant(args)
         return
     import krans_amd as K
     from krans_amd import workloads as W
@@ -208,6 +218,150 @@ def main():
     print(json.dumps(line))
 
 
+def principal_sine(U, Q):
+    """Largest principal-angle sine between span U and span Q (both
+    orthonormal n x k): ||U - Q Q^T U||_2."""
+    R = U - Q @ (Q.T @ U)
+    return float(np.sqrt(max(0.0, np.linalg.eigvalsh(R.T @ R).max())))
+
+
+def variant(args):
+    """The precision variants (shifted difference n=2^24, N=16, c=0.9;
+    weighted incidence of 16 grids 512 x 512) with cyclic Lanczos k=16, and
+    block LOBPCG (nev=32, block 32, [X W P] = 96 columns) on [R | RG] with
+    m=2,097,152: the same solve in mixed precision and in FP64, tol_rel
+    1e-12, tol_abs 0. One step = one full solve per precision. Prints one JSON
+    line with both precisions, accuracy against the exact kernel and the
+    roofline of the class that bounds the mixed solve (SpMV for the cyclic
+    variants, the FP32-MFMA Gram tiles for LOBPCG)."""
+    import krans_amd as K
+    from krans_amd import workloads as W
+
+    t0 = time.time()
+    cfg, sc = args.config, args.scale
+    if cfg == "c3a":
+        n = int((1 << 24) / sc)
+        A = W.shifted_difference(n, 16, 0.9)
+        Q = W.shifted_difference_kernel(n, 16, 0.9)
+        k, solver = 16, "cyclic"
+        # ||A||_F ~ sqrt(1.81 m); sigma_min = 0.1: prescreen below it
+        desc = f"cyclic Lanczos k=16 on the shifted difference n={n}, N=16, c=0.9"
+    elif cfg == "c3b":
+        side = int(512 / np.sqrt(sc))
+        A = W.grid_incidence(16, side, seed=42, weighted=True)
+        Q = W.component_indicators(16, side * side)
+        k, solver = 16, "cyclic"
+        desc = (f"cyclic Lanczos k=16 on the weighted incidence of 16 grids {side}x{side}, "
+                "weights 10^(-3U)")
+    else:
+        m = int(2097152 / sc)
+        A, G = W.rg_block(m, p=32, offdiag=8, gnnz=16, seed=42)
+        Q = W.rg_kernel(G)
+        del G
+        k, solver = 32, "lobpcg"
+        desc = f"LOBPCG nev=32 block 32 ([X W P] 96 columns) on [R | RG], m={m}, n={m + 32}"
+    H = K.HipCsrMatrixAdapter.from_csr(A)
+    S = A.to_scipy()
+    fro = A.fro()
+    setup_s = time.time() - t0
+
+    if solver == "cyclic":
+        # prescreen threshold 0.05 (below sigma_min = 0.1) for the shifted
+        # difference; the grids have sigma_min ~ 1e-5 * sigma_max, so there
+        # the prescreen only removes candidates that are far from null
+        pre = 0.05 / fro if cfg == "c3a" else 1e-6
+        opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=pre,
+                                     max_restarts=MAX_RESTARTS, cg_maxit=2000)
+        cls = "spmv"
+
+        def solve(vf, prec):
+            return K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision=prec)
+    else:
+        opt = K.LOBPCGOptions(nev=k, block_size=k, tol_rel=1e-12, tol_abs=0.0,
+                              max_iters=args.max_iters)
+        cls = "gram"
+
+        def solve(vf, prec):
+            return K.find_nullspace_lobpcg_ex(H, vf, opt, precision=prec)
+
+    def check(U):
+        if U.shape[1] == 0:
+            return None, None
+        res = np.linalg.norm(S @ U, axis=0)
+        sine = principal_sine(U, Q) if U.shape[1] == Q.shape[1] else None
+        return float(res.max() / fro), sine
+
+    out = {}
+    trace = None
+    for prec in ("mixed", "fp64"):
+        if prec == "fp64" and args.fp64_steps == 0:
+            continue
+        lat, found, r0 = [], [], None
+        for i in range(args.warmup + args.steps):
+            vf = K.HipVectorFactory(A.n, 42)
+            H.ctx.sync()
+            timed = i >= args.warmup
+            traced = timed and prec == "mixed" and trace is None
+            if traced:
+                H.ctx.trace_begin(cls)
+            t = time.perf_counter()
+            r = solve(vf, prec)
+            H.ctx.sync()
+            dt = time.perf_counter() - t
+            if traced:
+                trace = H.ctx.trace_end()
+            print(f"[{cfg} {prec}] solve {i}: {dt:.3f}s found={r.right_null.shape[1]}",
+                  file=sys.stderr, flush=True)
+            if timed:
+                lat.append(dt)
+                found.append(r.right_null.shape[1])
+                r0 = r
+        res, sine = check(r0.right_null)
+        ent = {"value": round(sum(found) / sum(lat), 4), "s_per_solve": round(sum(lat) / len(lat), 4),
+               "vectors": found[0], "max_res_rel": res, "sine_vs_exact": sine}
+        if solver == "cyclic":
+            ent.update(restarts=r0.restarts, cg_solves=r0.cg_solves, phase_s=r0.times)
+        else:
+            ent.update(iters=r0.iters, converged=r0.converged_to_target, phase_s=r0.times)
+        out[prec] = ent
+        del r0
+
+    n_l, amount, ms = trace
+    if cls == "gram":
+        achieved = amount / (ms * 1e-3) / 1e12 if ms > 0 else None
+        peak, unit, bound = 157.3, "TFLOP/s", "mfma"
+        kern = "k_gram_mfma_partial (Z^T [Z AZ] FP32 MFMA 16x16x4)"
+        traffic = load_traffic("k_gram_mfma_partial")
+    else:
+        achieved = amount / (ms * 1e-3) / 1e9 if ms > 0 else None
+        peak, unit, bound = 8000.0, "GB/s", "hbm"
+        kern = "k_spmv_* / k_spmm_* (CSR and CSC-twin gathers, mixed solve)"
+        traffic = None
+    line = {
+        "metric": "null vectors/sec at ||Au||/||A||_F<=1e-12, mixed vs FP64, 1 MI355X",
+        "value": out["mixed"]["value"], "unit": "null vectors/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * out["mixed"]["s_per_solve"], 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 bulk + fp64 inner products/polish",
+        "data": "synthetic (seeded generator, seed 42)",
+        "config": {"workload": desc, "m": A.m, "n": A.n, "nnz": A.nnz, "k": k,
+                   "tol_rel": 1e-12, "tol_abs": 0.0, "fro": fro},
+        "mixed": out["mixed"], "fp64": out.get("fp64"),
+        "mixed_over_fp64": (round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+                            if "fp64" in out else None),
+        "roofline": {"kernel": kern, "bound": bound,
+                     "achieved": None if achieved is None else round(achieved, 2),
+                     "peak": peak, "unit": unit,
+                     "frac": None if achieved is None else round(achieved / peak, 4),
+                     "traffic": traffic, "launches": n_l,
+                     "avg_us_per_launch": round(1e3 * ms / max(1, n_l), 3)},
+        "cpu_baseline": None,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(line))
+
+
 def serving_requests(count, nmax, seed=7):
     """The serving stream: each request draws uniformly from the three
     generator families (shifted difference, incidence of random 4-regular
@@ -331,12 +485,13 @@ def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/traffic_*.json, made by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     runs of this script, scaled by the same counters on a plain copy)."""
-    path = os.path.join(ROOT, "profiles", "traffic_vtw_fp32.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    return d.get(kernel, {}).get("hbm_bytes_per_launch")
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
+        with open(path) as f:
+            d = json.load(f)
+        if kernel in d:
+            return d[kernel].get("hbm_bytes_per_launch")
+    return None
 
 
 def cpu_baseline(budget_s):

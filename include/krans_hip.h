@@ -198,9 +198,10 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ,
                 int k, int64_t len, double *S, double *G);
 /* Kernel timing with HIP events on the context's stream: between begin and
  * end every launch of the class (1 = reorthogonalisation V^T w products,
- * 2 = w -= V h updates, 3 = sparse products) is bracketed by an event pair;
- * end returns the launch count, their summed algorithmic bytes (each array
- * touched once) and summed device time in ms. */
+ * 2 = w -= V h updates, 3 = sparse products, 4 = FP32-MFMA Gram tiles) is
+ * bracketed by an event pair; end returns the launch count, their summed
+ * algorithmic bytes (each array touched once; for class 4 the summed flops,
+ * 2 len s q per product) and summed device time in ms. */
 int kh_trace_begin(kh_context *ctx, int kernel_class);
 int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms);
 /* Average device time (ms) of `reps` launches of y = op(A) x, measured with

@@ -569,7 +569,7 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
 
 int kh_trace_begin(kh_context *ctx, int kernel_class) {
   return guard([&] {
-    need(ctx && kernel_class >= 0 && kernel_class <= 3, "bad argument");
+    need(ctx && kernel_class >= 0 && kernel_class <= 4, "bad argument");
     auto &c = *ctx->ctx;
     c.sync();
     c.trace_reset();

@@ -34,7 +34,13 @@ enum class Prec : int { F64 = 0, F32 = 1 };
 // Optional per-launch timing of one kernel class with HIP events on the
 // context's stream (for roofline reporting): algorithmic bytes of each traced
 // launch and its event pair.
-enum TraceClass : int { kTraceNone = 0, kTraceVtW = 1, kTraceVhSub = 2, kTraceSpmv = 3 };
+enum TraceClass : int {
+  kTraceNone = 0,
+  kTraceVtW = 1,
+  kTraceVhSub = 2,
+  kTraceSpmv = 3,
+  kTraceGram = 4,  // FP32-MFMA Gram tiles; the recorded amount is flops, not bytes
+};
 struct TraceRec {
   double bytes;
   hipEvent_t a, b;

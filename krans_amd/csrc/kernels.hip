@@ -1277,7 +1277,7 @@ void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
   nblk = (int)std::max<int64_t>(1, (len + rpb - 1) / rpb);
   const std::size_t need = (std::size_t)nblk * ty * tz * kMT * kMT;  // floats
   float *part = reinterpret_cast<float *>(ctx.partials((need + 1) / 2));
-  TraceScope tr(ctx, kTraceVtW, 4.0 * (double)len * (s + q));
+  TraceScope tr(ctx, kTraceGram, 2.0 * (double)len * s * q);
   hipLaunchKernelGGL(k_gram_mfma_partial, dim3(nblk, ty, tz), dim3(kBlock), 0, ctx.stream, len,
                      rpb, V, ldv, s, W, ldw, q, part);
   const int64_t outs = (int64_t)ty * tz * kMT * kMT;
