@@ -105,6 +105,33 @@ int kh_matrix_create_csr(kh_context *ctx, int64_t m, int64_t n,
                          const int32_t *rowptr, const int32_t *col,
                          const double *val, kh_matrix **out);
 int kh_matrix_destroy(kh_matrix *A);
+/* --- matrix ingestion (host only; no device or context needed) ---------- *
+ * Canonical CSR from an IJA triplet file, a Matrix Market file or raw 0-based
+ * triplets: rows in order, columns strictly ascending per row, repeated
+ * (i, j) summed in input order, explicit zeros kept — what Eigen's
+ * setFromTriplets produces (src/IO/MatrixLoader.cpp:53-54), laid out as CSR.
+ *   kh_load_ija:            load_ija_triplet_matrix(path, rows, cols)
+ *                           (include/krans/IO/MatrixLoader.hpp:12-13,
+ *                           src/IO/MatrixLoader.cpp:21-56); rows/cols < 0
+ *                           take the header's.
+ *   kh_load_matrix_market:  load_matrix_market(path) (MatrixLoader.hpp:9,
+ *                           MatrixLoader.cpp:12-18, Eigen 3.4 loadMarket).
+ *   kh_csr_from_triplets:   setFromTriplets on caller arrays; indices outside
+ *                           m x n are KH_EINVAL.
+ * A file that cannot be opened, or has no valid header, is KH_ERUNTIME (the
+ * reference's std::runtime_error). declared_nnz is the header's entry count
+ * (-1 for raw triplets), entries_read the triplets taken. */
+typedef struct kh_host_csr_s kh_host_csr;
+int kh_load_ija(const char *path, int64_t rows, int64_t cols, kh_host_csr **out);
+int kh_load_matrix_market(const char *path, kh_host_csr **out);
+int kh_csr_from_triplets(int64_t m, int64_t n, int64_t count, const int32_t *I,
+                         const int32_t *J, const double *V, kh_host_csr **out);
+int kh_host_csr_info(const kh_host_csr *h, int64_t *m, int64_t *n, int64_t *nnz,
+                     int64_t *declared_nnz, int64_t *entries_read);
+/* Pointers stay valid until kh_host_csr_free; rowptr has m+1 entries. */
+int kh_host_csr_data(const kh_host_csr *h, const int32_t **rowptr, const int32_t **col,
+                     const double **val);
+int kh_host_csr_free(kh_host_csr *h);
 /* rows(), cols() (MatrixInterface.hpp:20-23), nnz and exact ||A||_F */
 int kh_matrix_info(const kh_matrix *A, int64_t *m, int64_t *n, int64_t *nnz,
                    double *fro);

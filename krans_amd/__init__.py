@@ -91,6 +91,13 @@ SIGNATURES = {
     "kh_context_destroy": (I32, [P]),
     "kh_context_sync": (I32, [P]),
     "kh_matrix_create_csr": (I32, [P, I64, I64, P, P, P, C.POINTER(P)]),
+    "kh_load_ija": (I32, [C.c_char_p, I64, I64, C.POINTER(P)]),
+    "kh_load_matrix_market": (I32, [C.c_char_p, C.POINTER(P)]),
+    "kh_csr_from_triplets": (I32, [I64, I64, I64, P, P, P, C.POINTER(P)]),
+    "kh_host_csr_info": (I32, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
+                               C.POINTER(I64), C.POINTER(I64)]),
+    "kh_host_csr_data": (I32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+    "kh_host_csr_free": (I32, [P]),
     "kh_matrix_destroy": (I32, [P]),
     "kh_matrix_info": (I32, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), C.POINTER(D)]),
     "kh_matrix_download_csr": (I32, [P, P, P, P]),
@@ -501,6 +508,60 @@ def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
 def find_nullspace_lobpcg(A, vf, opt, precision="fp64"):
     """Backwards-compatible wrapper (LOBPCGSolver.hpp:147-149)."""
     return find_nullspace_lobpcg_ex(A, vf, opt, precision).right_null
+
+
+def _take_host_csr(h):
+    """Copy a kh_host_csr into a CsrMatrix and free it; returns (A, declared
+    entry count, entries read)."""
+    from .workloads import CsrMatrix
+    try:
+        m, n, nnz, dn, er = I64(), I64(), I64(), I64(), I64()
+        _check(lib().kh_host_csr_info(h, C.byref(m), C.byref(n), C.byref(nnz), C.byref(dn),
+                                      C.byref(er)))
+        rp, ci, va = P(), P(), P()
+        _check(lib().kh_host_csr_data(h, C.byref(rp), C.byref(ci), C.byref(va)))
+
+        def arr(ptr, ctype, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).copy()
+        A = CsrMatrix(m.value, n.value, arr(rp, C.c_int32, m.value + 1, np.int32),
+                      arr(ci, C.c_int32, nnz.value, np.int32),
+                      arr(va, C.c_double, nnz.value, np.float64))
+        return A, dn.value, er.value
+    finally:
+        lib().kh_host_csr_free(h)
+
+
+def load_ija(path, rows=-1, cols=-1, counts=False):
+    """load_ija_triplet_matrix (src/IO/MatrixLoader.cpp:21-56) as canonical
+    CSR: header "rows cols nnz", 0-based "i j value" triplets, repeats summed
+    in input order, explicit zeros kept; rows/cols >= 0 override the header.
+    With counts=True returns (A, header entry count, triplets read)."""
+    h = P()
+    _check(lib().kh_load_ija(os.fsencode(path), int(rows), int(cols), C.byref(h)))
+    out = _take_host_csr(h)
+    return out if counts else out[0]
+
+
+def load_matrix_market(path):
+    """load_matrix_market (src/IO/MatrixLoader.cpp:12-18) as canonical CSR."""
+    h = P()
+    _check(lib().kh_load_matrix_market(os.fsencode(path), C.byref(h)))
+    return _take_host_csr(h)[0]
+
+
+def csr_from_triplets(m, n, I, J, V):
+    """Canonical CSR of 0-based triplets with setFromTriplets semantics."""
+    I = np.ascontiguousarray(I, dtype=np.int32)
+    J = np.ascontiguousarray(J, dtype=np.int32)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    if not (I.shape == J.shape == V.shape and I.ndim == 1):
+        raise ValueError("csr_from_triplets: I, J, V must be 1-D arrays of one length")
+    h = P()
+    _check(lib().kh_csr_from_triplets(int(m), int(n), I.size, _ptr(I), _ptr(J), _ptr(V),
+                                      C.byref(h)))
+    return _take_host_csr(h)[0]
 
 
 class NullspaceServer:

@@ -7,11 +7,15 @@
 #include "../../include/krans_hip.h"
 #include "capi_internal.hpp"
 #include "dense.hpp"
+#include "ingest.hpp"
 #include "server.hpp"
 #include "solver.hpp"
 
 struct kh_server_s {
   std::unique_ptr<kh::Server> srv;
+};
+struct kh_host_csr_s {
+  kh::HostCsr H;
 };
 
 namespace {
@@ -138,6 +142,65 @@ int kh_matrix_create_csr(kh_context *ctx, int64_t m, int64_t n, const int32_t *r
     M->A.reset(kh::csr_create(*ctx->ctx, m, n, rowptr, col, val));
     *out = M.release();
   });
+}
+
+int kh_load_ija(const char *path, int64_t rows, int64_t cols, kh_host_csr **out) {
+  return guard([&] {
+    need(path && out, "null argument");
+    auto h = std::make_unique<kh_host_csr_s>();
+    h->H = kh::load_ija(path, rows, cols);
+    *out = h.release();
+  });
+}
+
+int kh_load_matrix_market(const char *path, kh_host_csr **out) {
+  return guard([&] {
+    need(path && out, "null argument");
+    auto h = std::make_unique<kh_host_csr_s>();
+    h->H = kh::load_matrix_market(path);
+    *out = h.release();
+  });
+}
+
+int kh_csr_from_triplets(int64_t m, int64_t n, int64_t count, const int32_t *I,
+                         const int32_t *J, const double *V, kh_host_csr **out) {
+  return guard([&] {
+    need(out, "null argument");
+    auto h = std::make_unique<kh_host_csr_s>();
+    h->H = kh::csr_from_triplets(m, n, I, J, V, count);
+    *out = h.release();
+  });
+}
+
+int kh_host_csr_info(const kh_host_csr *h, int64_t *m, int64_t *n, int64_t *nnz,
+                     int64_t *declared_nnz, int64_t *entries_read) {
+  return guard([&] {
+    need(h, "null handle");
+    if (m)
+      *m = h->H.m;
+    if (n)
+      *n = h->H.n;
+    if (nnz)
+      *nnz = (int64_t)h->H.col.size();
+    if (declared_nnz)
+      *declared_nnz = h->H.declared_nnz;
+    if (entries_read)
+      *entries_read = h->H.entries_read;
+  });
+}
+
+int kh_host_csr_data(const kh_host_csr *h, const int32_t **rowptr, const int32_t **col,
+                     const double **val) {
+  return guard([&] {
+    need(h && rowptr && col && val, "null argument");
+    *rowptr = h->H.rowptr.data();
+    *col = h->H.col.data();
+    *val = h->H.val.data();
+  });
+}
+
+int kh_host_csr_free(kh_host_csr *h) {
+  return guard([&] { delete h; });
 }
 
 int kh_matrix_destroy(kh_matrix *A) {
@@ -552,7 +615,9 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
     auto &c = *ctx->ctx;
     const kh::Prec p = prec_of(precision);
     kh::DBuf d((std::size_t)k * k * 2 * 8);
-    if (p == kh::Prec::F32) {  // FP32 MFMA tiles
+    if (p == kh::Prec::F32 && k <= 128) {  // fused FP32 MFMA pass
+      kh::gram2_mfma(c, (const float *)Z, len, (const float *)AZ, len, k, len, d.as<double>());
+    } else if (p == kh::Prec::F32) {  // FP32 MFMA tiles
       kh::gram_mfma(c, (const float *)Z, len, k, (const float *)Z, len, k, len, d.as<double>());
       kh::gram_mfma(c, (const float *)Z, len, k, (const float *)AZ, len, k, len,
                     d.as<double>() + (std::size_t)k * k);

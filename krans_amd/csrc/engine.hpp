@@ -149,6 +149,10 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
              const void *W, int64_t ldw, int q, int64_t len, double *H);
 // H = V^T W for FP32 V, W on the matrix cores (FP32 MFMA accumulation within
 // a row chunk, FP64 sums across chunks): the LOBPCG Gram blocks.
+// S = Z^T Z and G = Z^T AZ (q <= 128) in one fused pass: SG[0:q*q] = S,
+// SG[q*q:2*q*q] = G, column-major, FP64 (FP32 MFMA, FP64 across row chunks).
+void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
+                int64_t len, double *SG);
 void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
                int q, int64_t len, double *H);
 // W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.

@@ -727,6 +727,155 @@ __global__ void __launch_bounds__(kBlock)
   H[j + (int64_t)c * s] = acc;
 }
 
+// ------------------------------------------- fused FP32 MFMA Gram pair --
+// S = Z^T Z (upper-triangle tiles only) and G = Z^T AZ in one pass over the
+// rows, q <= kG2MaxQ. A block of 8 waves owns a contiguous chunk of rows and
+// every output tile of both products, so Z and AZ are read from HBM exactly
+// once. 32-row slabs of Z and AZ are staged column-major in LDS with a row
+// stride of 34 floats: the 32 lanes of a column write 32 consecutive banks,
+// and the MFMA operand reads (16 columns x 2 rows per half-wave) hit 32
+// distinct banks. The next slab is loaded into registers while the current
+// one is consumed. Tile list: S tiles (ti <= tj) then G tiles, dealt to the
+// waves round-robin. Block partials (FP32, at most 4096 rows each) are summed
+// in FP64 in block order by k_gram2_finish.
+constexpr int kG2Threads = 512, kG2Slab = 32, kG2Ld = 34, kG2MaxQ = 128;
+constexpr int kG2MaxT = kG2MaxQ / 16;                                   // 8 column tiles
+constexpr int kG2MaxTiles = kG2MaxT * (kG2MaxT + 1) / 2 + kG2MaxT * kG2MaxT;  // 100
+constexpr int kG2Waves = kG2Threads / 64;
+constexpr int kG2PerWave = (kG2MaxTiles + kG2Waves - 1) / kG2Waves;     // 13
+
+__device__ inline void g2_tile(int t, int T, int &ti, int &tj, bool &is_s) {
+  const int ns = T * (T + 1) / 2;
+  if (t < ns) {
+    int i = 0, rem = t;
+    while (rem >= T - i) {
+      rem -= T - i;
+      ++i;
+    }
+    ti = i;
+    tj = i + rem;
+    is_s = true;
+  } else {
+    ti = (t - ns) / T;
+    tj = (t - ns) % T;
+    is_s = false;
+  }
+}
+
+__global__ void __launch_bounds__(kG2Threads)
+    k_gram2_partial(int64_t len, int64_t rows_per_block, const float *__restrict__ Z,
+                    int64_t ldz, const float *__restrict__ AZ, int64_t ldaz, int q,
+                    float *__restrict__ part) {
+  __shared__ float Zs[kG2MaxQ * kG2Ld];
+  __shared__ float As[kG2MaxQ * kG2Ld];
+  const int T = (q + 15) / 16, QP = T * 16;
+  const int ntiles = T * (T + 1) / 2 + T * T;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
+  // this wave's tiles: operand offsets in LDS (floats)
+  int aoff[kG2PerWave], boff[kG2PerWave];
+  bool bs[kG2PerWave];
+#pragma unroll
+  for (int u = 0; u < kG2PerWave; ++u) {
+    const int t = wid + u * kG2Waves;
+    int ti = 0, tj = 0;
+    bool is_s = true;
+    if (t < ntiles)
+      g2_tile(t, T, ti, tj, is_s);
+    aoff[u] = (16 * ti + (lane & 15)) * kG2Ld + (lane >> 4);
+    boff[u] = (16 * tj + (lane & 15)) * kG2Ld + (lane >> 4);
+    bs[u] = is_s;
+  }
+  floatx4 acc[kG2PerWave];
+#pragma unroll
+  for (int u = 0; u < kG2PerWave; ++u)
+    acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // staging: element e = tid + u*512 of the (32 rows x QP cols) slab of Z and
+  // of AZ: row e % 32, column e / 32
+  constexpr int kLd = kG2Slab * kG2MaxQ / kG2Threads;  // 8 per operand
+  float za[kLd], aa[kLd];
+  auto load = [&](int64_t rb) {
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int e = threadIdx.x + u * kG2Threads;
+      const int r = e % kG2Slab, c = e / kG2Slab;
+      const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
+      za[u] = c < q ? Z[row + (int64_t)c * ldz] : 0.f;
+      aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : 0.f;
+    }
+  };
+  if (r0 < r1)
+    load(r0);
+  for (int64_t rb = r0; rb < r1; rb += kG2Slab) {
+    __syncthreads();  // previous slab consumed
+#pragma unroll
+    for (int u = 0; u < kLd; ++u) {
+      const int e = threadIdx.x + u * kG2Threads;
+      const int r = e % kG2Slab, c = e / kG2Slab;
+      if (c < QP) {
+        const bool ok = (rb + r) < r1 && c < q;
+        Zs[c * kG2Ld + r] = ok ? za[u] : 0.f;
+        As[c * kG2Ld + r] = ok ? aa[u] : 0.f;
+      }
+    }
+    __syncthreads();
+    if (rb + kG2Slab < r1)
+      load(rb + kG2Slab);
+#pragma unroll
+    for (int k = 0; k < kG2Slab; k += 4) {
+#pragma unroll
+      for (int u = 0; u < kG2PerWave; ++u) {
+        if (wid + u * kG2Waves < ntiles) {
+          const float a = Zs[aoff[u] + k];
+          const float b = bs[u] ? Zs[boff[u] + k] : As[boff[u] + k];
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[u], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float *out = part + (int64_t)blockIdx.x * ntiles * 256;
+#pragma unroll
+  for (int u = 0; u < kG2PerWave; ++u) {
+    const int t = wid + u * kG2Waves;
+    if (t < ntiles)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        out[t * 256 + (4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[u][v];
+  }
+}
+
+// SG[0 : q*q] = S, SG[q*q : 2*q*q] = G, both column-major q x q; S is
+// mirrored from its upper triangle.
+__global__ void __launch_bounds__(kBlock)
+    k_gram2_finish(const float *__restrict__ part, int nblk, int q, double *__restrict__ SG) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t qq = (int64_t)q * q;
+  if (e >= 2 * qq)
+    return;
+  const bool is_g = e >= qq;
+  const int64_t w = is_g ? e - qq : e;
+  int i = (int)(w % q), j = (int)(w / q);  // H(i, j)
+  if (!is_g && i > j) {
+    const int t = i;
+    i = j;
+    j = t;
+  }
+  const int T = (q + 15) / 16;
+  const int ntiles = T * (T + 1) / 2 + T * T;
+  const int ti = i / 16, tj = j / 16;
+  int t;
+  if (!is_g)
+    t = ti * T - ti * (ti - 1) / 2 + (tj - ti);
+  else
+    t = T * (T + 1) / 2 + ti * T + tj;
+  const float *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b)
+    acc += (double)p[(int64_t)b * ntiles * 256];
+  SG[e] = acc;
+}
+
 // ---------------------------------------------------------------- level 1 --
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -1283,6 +1432,36 @@ void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
   const int64_t outs = (int64_t)ty * tz * kMT * kMT;
   hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)), dim3(kBlock),
                      0, ctx.stream, part, nblk, ty, s, q, H);
+  KH_CHECK(hipGetLastError());
+}
+
+void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
+                int64_t len, double *SG) {
+  if (q <= 0 || len <= 0)
+    return;
+  if (q > kG2MaxQ)
+    throw std::invalid_argument("gram2_mfma: q above the fused-kernel limit");
+  const int T = (q + 15) / 16;
+  const int ntiles = T * (T + 1) / 2 + T * T;
+  // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
+  // unless the partials would not fit the context's reduction buffer
+  const int64_t maxblk =
+      std::max<int64_t>(1, (int64_t)(2 * ctx.part_cap) / ((int64_t)ntiles * 256));
+  int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);
+  rpb = std::min<int64_t>(rpb, 4096);
+  rpb = std::max<int64_t>(rpb, (len + maxblk - 1) / maxblk);
+  rpb = (rpb + kG2Slab - 1) / kG2Slab * kG2Slab;
+  const int64_t nblk64 = (len + rpb - 1) / rpb;
+  const std::size_t need = (std::size_t)nblk64 * ntiles * 256;  // floats
+  float *part = reinterpret_cast<float *>(ctx.partials((need + 1) / 2));
+  {
+    TraceScope tr(ctx, kTraceGram, 2.0 * (double)len * q * q * 2.0);
+    hipLaunchKernelGGL(k_gram2_partial, dim3((unsigned)nblk64), dim3(kG2Threads), 0, ctx.stream,
+                       len, rpb, Z, ldz, AZ, ldaz, q, part);
+  }
+  const int64_t outs = 2 * (int64_t)q * q;
+  hipLaunchKernelGGL(k_gram2_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, ctx.stream, part, (int)nblk64, q, SG);
   KH_CHECK(hipGetLastError());
 }
 

@@ -165,12 +165,14 @@ private:
   double *sc_ = nullptr;
   std::vector<double> hbuf_;
   Blk tmpq_;  // m x q scratch for apply_normal
+  DBuf hproj_;  // locked^T X coefficients of a block projection
 
   void random_col(void *dst, Prec p);
   double nrm(Prec p, const void *x, int64_t len);
   void project_out(void *x, Prec p);  // one pass vs locked (LOBPCGSolver.cpp:188-195)
   bool mgs(Blk &X, int i0, int cnt, double tol, bool reorth);
   bool orthonormalize(Blk &X, int i0, int cnt, double tol, bool reorth, Blk &scratch);
+  void mgs_dev(Blk &X, int cnt, bool reorth, double *dn);
   void apply_normal(const Blk &Z, int q, Blk &AZ);
   void gram(const Blk &Z, const Blk &AZ, int q, dense::Mat &S, dense::Mat &G);
   void combine(const Blk &Z, int z0, int q, const dense::Mat &Y, int row0, int p,
@@ -241,24 +243,84 @@ bool LobpcgDriver::mgs(Blk &X, int i0, int cnt, double tol, bool reorth) {
   return true;
 }
 
-// :137-160 (the MGS part runs on a scratch copy, as the reference's clones)
+// modified_gram_schmidt (GramSchmidt.cpp:55-82) on columns [0, cnt) of X
+// with every norm left on the device: dn[2i] = ||x_i|| after the first pass,
+// dn[2i+1] after the re-orthogonalisation pass. Columns are scaled by the
+// device norm without a host round trip; the caller replays the reference's
+// `norm < tol` tests on the downloaded norms afterwards (a failed column only
+// leaves non-finite values in X, which the caller then discards).
+void LobpcgDriver::mgs_dev(Blk &X, int cnt, bool reorth, double *dn) {
+  for (int i = 0; i < cnt; ++i) {
+    void *xi = X.col(i);
+    const int passes = reorth ? 2 : 1;
+    for (int ps = 0; ps < passes; ++ps) {
+      if (i > 0) {
+        gemv_t(c_, X.p, X.col(0), n_, i, X.p, xi, n_, sc_ + 16);
+        gemv_n_sub(c_, X.p, X.col(0), n_, i, sc_ + 16, X.p, xi, n_, dn + 2 * i + ps);
+      } else {
+        dot(c_, X.p, xi, X.p, xi, n_, dn + 2 * i + ps, true);
+      }
+    }
+    scale_dev(c_, X.p, n_, dn + 2 * i + passes - 1, true, xi);
+  }
+}
+
+// :137-160. Phase 1 (each column against the locked basis, then normalised)
+// runs on all columns at once in the scratch block; phase 2 (MGS with
+// re-orthogonalisation on the normalised copy, as the reference's clones)
+// keeps its norms on the device. The host reads the norms twice per call and
+// replays the reference's sequential tests, so X ends exactly as the
+// reference leaves it: on a phase-1 failure at column i, columns < i are
+// projected and normalised, column i projected, later columns untouched; on
+// a phase-2 failure X holds the phase-1 result.
 bool LobpcgDriver::orthonormalize(Blk &X, int i0, int cnt, double tol, bool reorth,
                                   Blk &scratch) {
-  for (int i = 0; i < cnt; ++i) {
-    void *x = X.col(i0 + i);
-    project_out(x, X.p);
-    const double nn = nrm(X.p, x, n_);
-    if (nn < tol)
-      return false;
-    axpby(c_, X.p, n_, 0.0, nullptr, 1.0 / nn, x);
-  }
   if (cnt == 0)
     return true;
-  KH_CHECK(hipMemcpyAsync(scratch.col(0), X.col(i0), (std::size_t)cnt * n_ * esz(X.p),
+  const std::size_t cbytes = (std::size_t)n_ * esz(X.p);
+  if (scratch.p != X.p || scratch.cap < cnt)
+    throw std::logic_error("orthonormalize: scratch block precision or width mismatch");
+  KH_CHECK(hipMemcpyAsync(scratch.col(0), X.col(i0), (std::size_t)cnt * cbytes,
                           hipMemcpyDeviceToDevice, c_.stream));
-  if (!mgs(scratch, 0, cnt, tol, reorth))
+  double *dn = sc_ + 60000;
+  const int nb = (int)locked_.count;
+  if (nb > 0) {
+    const std::size_t need = (std::size_t)nb * cnt * 8;
+    if (hproj_.bytes < need)
+      hproj_ = DBuf(need);
+    gemm_tn(c_, Prec::F64, locked_.col(0), n_, nb, X.p, scratch.col(0), n_, cnt, n_,
+            hproj_.as<double>());
+    gemm_nn_sub(c_, Prec::F64, locked_.col(0), n_, nb, hproj_.as<double>(), cnt, X.p,
+                scratch.col(0), n_, n_, dn);
+  } else {
+    col_norms(c_, X.p, scratch.col(0), n_, cnt, n_, dn);
+  }
+  std::vector<double> nn((std::size_t)cnt), inv((std::size_t)cnt);
+  rb(c_, dn, cnt, nn.data());
+  int fail = -1;
+  for (int i = 0; i < cnt && fail < 0; ++i)
+    if (nn[(std::size_t)i] < tol)
+      fail = i;
+  const int good = fail < 0 ? cnt : fail;
+  for (int i = 0; i < cnt; ++i)
+    inv[(std::size_t)i] = i < good ? 1.0 / nn[(std::size_t)i] : 1.0;
+  up(c_, inv.data(), cnt, dn + 512);
+  scale_cols(c_, X.p, n_, good, scratch.col(0), n_, dn + 512, false);
+  const int back = fail < 0 ? cnt : fail + 1;
+  KH_CHECK(hipMemcpyAsync(X.col(i0), scratch.col(0), (std::size_t)back * cbytes,
+                          hipMemcpyDeviceToDevice, c_.stream));
+  if (fail >= 0)
     return false;
-  KH_CHECK(hipMemcpyAsync(X.col(i0), scratch.col(0), (std::size_t)cnt * n_ * esz(X.p),
+  mgs_dev(scratch, cnt, reorth, dn);
+  std::vector<double> n12((std::size_t)2 * cnt);
+  rb(c_, dn, 2 * cnt, n12.data());
+  for (int i = 0; i < cnt; ++i) {
+    if (n12[(std::size_t)2 * i] < tol)
+      return false;
+    if (reorth && n12[(std::size_t)2 * i + 1] < tol)
+      return false;
+  }
+  KH_CHECK(hipMemcpyAsync(X.col(i0), scratch.col(0), (std::size_t)cnt * cbytes,
                           hipMemcpyDeviceToDevice, c_.stream));
   return true;
 }
@@ -281,7 +343,9 @@ void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
 void LobpcgDriver::gram(const Blk &Z, const Blk &AZ, int q, dense::Mat &S, dense::Mat &G) {
   auto t0 = Clock::now();
   const int qq = q * q;
-  if (Z.p == Prec::F32 && AZ.p == Prec::F32) {  // FP32 MFMA tiles
+  if (Z.p == Prec::F32 && AZ.p == Prec::F32 && q <= 128) {  // fused FP32 MFMA pass
+    gram2_mfma(c_, (const float *)Z.col(0), n_, (const float *)AZ.col(0), n_, q, n_, sc_ + 1024);
+  } else if (Z.p == Prec::F32 && AZ.p == Prec::F32) {  // FP32 MFMA tiles
     gram_mfma(c_, (const float *)Z.col(0), n_, q, (const float *)Z.col(0), n_, q, n_,
               sc_ + 1024);
     gram_mfma(c_, (const float *)Z.col(0), n_, q, (const float *)AZ.col(0), n_, q, n_,

@@ -128,6 +128,16 @@ def lib():
         L.orc_orth_against.argtypes = [I64, I64, P, P, D, I32]
         L.orc_subspace_diff.restype = D
         L.orc_subspace_diff.argtypes = [I64, I64, P, I64, P, D]
+        L.orc_load_ija.restype = P
+        L.orc_load_ija.argtypes = [C.c_char_p, I32, I32]
+        L.orc_load_mtx.restype = P
+        L.orc_load_mtx.argtypes = [C.c_char_p]
+        L.orc_triplets.restype = P
+        L.orc_triplets.argtypes = [I64, I64, I64, P, P, P]
+        L.orc_csc_info.argtypes = [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
+                                   C.POINTER(I64)]
+        L.orc_csc_copy.argtypes = [P, P, P, P]
+        L.orc_csc_free.argtypes = [P]
         _lib = L
     return _lib
 
@@ -280,3 +290,38 @@ def subspace_diff(B1, B2, tol=1e-12):
     B2 = np.asfortranarray(B2, dtype=np.float64)
     return lib().orc_subspace_diff(B1.shape[0], B1.shape[1], _p(B1),
                                    B2.shape[1], _p(B2), tol)
+
+
+def _take_csc(h):
+    if not h:
+        raise RuntimeError(lib().orc_last_error().decode())
+    L = lib()
+    try:
+        r, c, nnz, er = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        L.orc_csc_info(h, C.byref(r), C.byref(c), C.byref(nnz), C.byref(er))
+        colptr = np.zeros(c.value + 1, np.int64)
+        rowidx = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        L.orc_csc_copy(h, _p(colptr), _p(rowidx), _p(val))
+        return (r.value, c.value, colptr, rowidx, val, er.value)
+    finally:
+        L.orc_csc_free(h)
+
+
+def load_ija(path, rows=-1, cols=-1):
+    """load_ija_triplet_matrix restated (src/IO/MatrixLoader.cpp:21-56):
+    returns (rows, cols, colptr, rowidx, val, entries_read), Eigen's CSC."""
+    return _take_csc(lib().orc_load_ija(os.fsencode(path), rows, cols))
+
+
+def load_mtx(path):
+    """load_matrix_market restated (MatrixLoader.cpp:12-18, Eigen loadMarket)."""
+    return _take_csc(lib().orc_load_mtx(os.fsencode(path)))
+
+
+def from_triplets(rows, cols, I, J, V):
+    """setFromTriplets restated: Eigen's CSC of 0-based triplets."""
+    I = np.ascontiguousarray(I, np.int32)
+    J = np.ascontiguousarray(J, np.int32)
+    V = np.ascontiguousarray(V, np.float64)
+    return _take_csc(lib().orc_triplets(rows, cols, I.size, _p(I), _p(J), _p(V)))

@@ -258,6 +258,49 @@ extern "C" {
 
 const char *orc_last_error() { return g_err.c_str(); }
 
+// Loaders: a handle to an Eigen-style CSC, or null with orc_last_error set.
+void *orc_load_ija(const char *path, int rows, int cols) {
+  Csc *out = nullptr;
+  guard([&] { out = new Csc(load_ija(path, rows, cols)); });
+  return out;
+}
+
+void *orc_load_mtx(const char *path) {
+  Csc *out = nullptr;
+  guard([&] { out = new Csc(load_mtx(path)); });
+  return out;
+}
+
+void *orc_triplets(int64_t rows, int64_t cols, int64_t count, const int32_t *I,
+                   const int32_t *J, const double *V) {
+  Csc *out = nullptr;
+  guard([&] {
+    std::vector<int> i(I, I + count), j(J, J + count);
+    std::vector<double> v(V, V + count);
+    out = new Csc(set_from_triplets(rows, cols, i, j, v));
+    out->entries_read = count;
+  });
+  return out;
+}
+
+void orc_csc_info(const void *h, int64_t *rows, int64_t *cols, int64_t *nnz,
+                  int64_t *entries_read) {
+  const Csc &C = *static_cast<const Csc *>(h);
+  *rows = C.rows;
+  *cols = C.cols;
+  *nnz = (int64_t)C.rowidx.size();
+  *entries_read = C.entries_read;
+}
+
+void orc_csc_copy(const void *h, int64_t *colptr, int32_t *rowidx, double *val) {
+  const Csc &C = *static_cast<const Csc *>(h);
+  std::copy(C.colptr.begin(), C.colptr.end(), colptr);
+  std::copy(C.rowidx.begin(), C.rowidx.end(), rowidx);
+  std::copy(C.val.begin(), C.val.end(), val);
+}
+
+void orc_csc_free(void *h) { delete static_cast<Csc *>(h); }
+
 void *orc_op_csr(int64_t m, int64_t n, const int32_t *rowptr,
                  const int32_t *col, const double *val) {
   auto *h = new OpHolder;

@@ -19,6 +19,7 @@
 #include <memory>
 #include <random>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace orc {
@@ -218,5 +219,17 @@ double max_pairwise_abs_dot(const std::vector<Vec> &basis);
 void compute_rref(std::vector<Vec> &rows, double tol);
 double subspace_diff(const std::vector<Vec> &b1, const std::vector<Vec> &b2,
                      double tol);
+
+// Matrix loaders (src/IO/MatrixLoader.cpp) producing Eigen's compressed CSC.
+struct Csc {
+  int64_t rows = 0, cols = 0, entries_read = 0;
+  std::vector<int64_t> colptr;
+  std::vector<int> rowidx;
+  std::vector<double> val;
+};
+Csc set_from_triplets(int64_t rows, int64_t cols, const std::vector<int> &I,
+                      const std::vector<int> &J, const std::vector<double> &V);
+Csc load_ija(const std::string &path, int rows, int cols);
+Csc load_mtx(const std::string &path);
 
 } // namespace orc

@@ -80,7 +80,8 @@ def test_cgs2_orthogonalises(ctx, s, n):
 
 
 @pytest.mark.parametrize("k,n,prec", [(3, 1000, 0), (40, 50_000, 0), (96, 200_000, 0),
-                                      (3, 1000, 1), (17, 4099, 1), (96, 200_000, 1)])
+                                      (3, 1000, 1), (17, 4099, 1), (96, 200_000, 1),
+                                      (128, 70_001, 1), (130, 5000, 1), (96, 3_000_017, 1)])
 def test_gram_blocks(ctx, k, n, prec):
     print(f"[case] gram k={k} n={n} prec={prec}", flush=True)
     rng = np.random.default_rng(k)
@@ -96,8 +97,9 @@ def test_gram_blocks(ctx, k, n, prec):
     Z64, A64 = Z.astype(np.float64), AZ.astype(np.float64)
     Sr, Gr = Z64.T @ Z64, Z64.T @ A64
     scale = np.sqrt(np.outer(np.sum(Z64 * Z64, 0), np.sum(A64 * A64, 0)))
-    # FP64: fixed-order FP64 sums; FP32: FP32 MFMA accumulation over 4096-row
-    # chunks, FP64 across chunks
+    # FP64: fixed-order FP64 sums; FP32: FP32 MFMA accumulation over row
+    # chunks of at most 4096 rows, FP64 across chunks (k <= 128: the fused
+    # S/G pass; k = 130: the 64 x 64 tile kernel)
     tol = 1e-13 if prec == 0 else 1e-5
     assert np.abs(S - Sr).max() <= tol * scale.max()
     assert np.abs(G - Gr).max() <= tol * scale.max()
