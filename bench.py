@@ -280,14 +280,15 @@ def variant(args):
         else:
             opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
                                          max_restarts=MAX_RESTARTS, cg_maxit=4000,
-                                         precond_AAt="jacobi")
+                                         precond_AAt="jacobi", verbose=True)
         cls = "spmv"
 
         def solve(vf, prec):
             return K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision=prec)
     else:
+        # verbose: the solver's own progress line every 20 iterations (stderr)
         opt = K.LOBPCGOptions(nev=k, block_size=k, tol_rel=1e-12, tol_abs=0.0,
-                              max_iters=args.max_iters)
+                              max_iters=args.max_iters, verbose=True, progress_every=20)
         cls = "gram"
 
         def solve(vf, prec):

@@ -209,6 +209,10 @@ int kh_matrix_jacobi_AAt(kh_matrix *A, const double *r, double *y);
 /* y = op(A) x on device vectors, s columns (ld = vector length) */
 int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s,
                 const void *x, void *y);
+/* The same product on row-major blocks (element (r, c) at r * s + c), the
+ * layout the LOBPCG normal products use for blocks of 4 or more columns. */
+int kh_spmm_rows_dev(kh_matrix *A, int transpose, int precision, int s,
+                     const void *x, void *y);
 /* out = a.b (FP64 accumulate); precision of a, b */
 int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b,
                int64_t len, double *out);

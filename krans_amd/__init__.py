@@ -128,6 +128,7 @@ SIGNATURES = {
     "kh_vec_axpby": (I32, [P, I64, D, P, D, P]),
     "kh_matrix_jacobi_AAt": (I32, [P, P, P]),
     "kh_spmm_dev": (I32, [P, I32, I32, I32, P, P]),
+    "kh_spmm_rows_dev": (I32, [P, I32, I32, I32, P, P]),
     "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),

@@ -571,6 +571,16 @@ int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s, const void *x
   });
 }
 
+int kh_spmm_rows_dev(kh_matrix *A, int transpose, int precision, int s, const void *x,
+                     void *y) {
+  return guard([&] {
+    need(A && x && y && s >= 1, "bad argument");
+    auto &c = *A->owner->ctx;
+    kh::spmm_rows(c, *A->A, transpose != 0, prec_of(precision), s, x, y);
+    c.sync();
+  });
+}
+
 int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b, int64_t len,
                double *out) {
   return guard([&] {
@@ -617,6 +627,8 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
     kh::DBuf d((std::size_t)k * k * 2 * 8);
     if (p == kh::Prec::F32 && k <= 128) {  // fused FP32 MFMA pass
       kh::gram2_mfma(c, (const float *)Z, len, (const float *)AZ, len, k, len, d.as<double>());
+    } else if (k <= 128) {  // fused FP64 MFMA pass
+      kh::gram2_mfma(c, (const double *)Z, len, (const double *)AZ, len, k, len, d.as<double>());
     } else if (p == kh::Prec::F32) {  // FP32 MFMA tiles
       kh::gram_mfma(c, (const float *)Z, len, k, (const float *)Z, len, k, len, d.as<double>());
       kh::gram_mfma(c, (const float *)Z, len, k, (const float *)AZ, len, k, len,

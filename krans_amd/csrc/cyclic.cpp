@@ -409,7 +409,7 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
     }
     if (P_.verbose)
       for (int a = 0; a < qa; ++a)
-        std::printf("[Cyclic-HIP] polish pass=%d cg_its=%d ||A v'||=%.3e\n", pass,
+        std::fprintf(stderr, "[Cyclic-HIP] polish pass=%d cg_its=%d ||A v'||=%.3e\n", pass,
                     st[(std::size_t)a].k, rp[(std::size_t)a]);
   }
   for (int i = 0; i < q; ++i)
@@ -636,7 +636,7 @@ DeviceBasis CyclicDriver::run() {
     S_.t.tiny += since(t0);
     if (!finite) {
       if (P_.verbose)
-        std::printf("[Cyclic-HIP] non-finite projected problem, random restart\n");
+        std::fprintf(stderr, "[Cyclic-HIP] non-finite projected problem, random restart\n");
       random_right();
       continue;
     }
@@ -741,7 +741,7 @@ DeviceBasis CyclicDriver::run() {
         if (o.hit)
           S_.cg_maxit_hits += 1;  // one counted hit per column and pass set
         if (P_.verbose)
-          std::printf("[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
+          std::fprintf(stderr, "[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
                       (long long)restart, o.res_raw, o.res, accept_thr_);
         if (o.res <= accept_thr_) {
           push_basis(vfin);
@@ -778,7 +778,7 @@ DeviceBasis CyclicDriver::run() {
       ++restarts_since_accept;
       if (P_.stagnation_restarts > 0 && restarts_since_accept >= P_.stagnation_restarts) {
         if (P_.verbose)
-          std::printf("[Cyclic-HIP] stopping due to stagnation\n");
+          std::fprintf(stderr, "[Cyclic-HIP] stopping due to stagnation\n");
         break;
       }
     } else {

@@ -125,6 +125,13 @@ void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
           const void *x, int64_t ldx, void *y, int64_t ldy,
           const int *active);
 
+// Row-major block product: Y(r, c) = sum_p op(A)(r, p) X(p, c) with X and Y
+// stored row-major (stride s); same per-entry rounding as spmm.
+void spmm_rows(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s, const void *X,
+               void *Y);
+// Column-major (leading dimension ld) <-> row-major (stride s) block copies.
+void cols_to_rows(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *R);
+void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void *X, int64_t ldx);
 // out[0] = sum_i a_i b_i (FP64 accumulate, fixed reduction tree); with
 // do_sqrt: out[0] = sqrt(max(0,sum)).
 void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
@@ -153,6 +160,9 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
 // SG[q*q:2*q*q] = G, column-major, FP64 (FP32 MFMA, FP64 across row chunks).
 void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
                 int64_t len, double *SG);
+// FP64 blocks: the same pass on FP64 MFMA (v_mfma_f64_16x16x4f64).
+void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
+                int q, int64_t len, double *SG);
 void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
                int q, int64_t len, double *H);
 // W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.

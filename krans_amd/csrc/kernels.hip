@@ -635,6 +635,78 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// --------------------------------------------- row-major block products --
+// Y = op(A) X for a block of s columns stored row-major (element (r, c) at
+// r * s + c). Thread t handles (row t / s, column t % s): the threads of one
+// row read the row's indices and values (cache-served broadcasts) and, per
+// nonzero, one contiguous s-element row of X, so every gather is coalesced
+// however large the block. Sequential ascending sum, no contraction: the
+// same rounding as the one-lane-per-row kernels.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_spmm_rows(int64_t rows, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+                const T *__restrict__ val, int s, const T *__restrict__ X, T *__restrict__ Y) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  const int64_t total = rows * (int64_t)s;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = t / s;
+    const int c = (int)(t - i * s);
+    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+    A acc = (A)0;
+    for (int32_t p = p0; p < p1; ++p)
+      acc += (A)val[p] * (A)X[(int64_t)idx[p] * s + c];
+    Y[t] = (T)acc;
+  }
+}
+
+// Column-major (ld) <-> row-major (stride s) block transposes through LDS,
+// 64 x 64 tiles: both global sides are read and written in 256-byte runs.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_cols_to_rows(int64_t len, int s, const T *__restrict__ X, int64_t ldx, T *__restrict__ R) {
+  __shared__ T tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int c = c0 + cc;
+    const int64_t r = r0 + tx;
+    if (c < s && r < len)
+      tile[cc][tx] = X[r + (int64_t)c * ldx];
+  }
+  __syncthreads();
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr;
+    const int c = c0 + tx;
+    if (c < s && r < len)
+      R[r * s + c] = tile[tx][rr];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_rows_to_cols(int64_t len, int s, const T *__restrict__ R, T *__restrict__ X, int64_t ldx) {
+  __shared__ T tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr;
+    const int c = c0 + tx;
+    if (c < s && r < len)
+      tile[rr][tx] = R[r * s + c];
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int c = c0 + cc;
+    const int64_t r = r0 + tx;
+    if (c < s && r < len)
+      X[r + (int64_t)c * ldx] = tile[tx][cc];
+  }
+}
+
 // ----------------------------------------------------- FP32 MFMA Gram tiles --
 // H(s x q) = V^T W for FP32 V, W on the matrix cores: v_mfma_f32_16x16x4f32.
 // A block owns a contiguous chunk of rows and one 64 x 64 tile of H
@@ -762,14 +834,34 @@ __device__ inline void g2_tile(int t, int T, int &ti, int &tj, bool &is_s) {
   }
 }
 
+// 16x16x4 MFMA in the element type: FP32 (v_mfma_f32_16x16x4f32) or FP64
+// (v_mfma_f64_16x16x4f64); both take A[i][k] from lane i + 16k and B[k][j]
+// from lane j + 16k and return D[4*(l/16)+v][l%16] in element v of lane l.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+template <class T> struct Mfma16;
+template <> struct Mfma16<float> {
+  using V = floatx4;
+  static __device__ inline V step(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma16<double> {
+  using V = doublex4;
+  static __device__ inline V step(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <class T>
 __global__ void __launch_bounds__(kG2Threads)
-    k_gram2_partial(int64_t len, int64_t rows_per_block, const float *__restrict__ Z,
-                    int64_t ldz, const float *__restrict__ AZ, int64_t ldaz, int q,
-                    float *__restrict__ part) {
-  __shared__ float Zs[kG2MaxQ * kG2Ld];
-  __shared__ float As[kG2MaxQ * kG2Ld];
-  const int T = (q + 15) / 16, QP = T * 16;
-  const int ntiles = T * (T + 1) / 2 + T * T;
+    k_gram2_partial(int64_t len, int64_t rows_per_block, const T *__restrict__ Z,
+                    int64_t ldz, const T *__restrict__ AZ, int64_t ldaz, int q,
+                    T *__restrict__ part) {
+  using V4 = typename Mfma16<T>::V;
+  __shared__ T Zs[kG2MaxQ * kG2Ld];
+  __shared__ T As[kG2MaxQ * kG2Ld];
+  const int nt = (q + 15) / 16, QP = nt * 16;
+  const int ntiles = nt * (nt + 1) / 2 + nt * nt;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
@@ -782,27 +874,27 @@ __global__ void __launch_bounds__(kG2Threads)
     int ti = 0, tj = 0;
     bool is_s = true;
     if (t < ntiles)
-      g2_tile(t, T, ti, tj, is_s);
+      g2_tile(t, nt, ti, tj, is_s);
     aoff[u] = (16 * ti + (lane & 15)) * kG2Ld + (lane >> 4);
     boff[u] = (16 * tj + (lane & 15)) * kG2Ld + (lane >> 4);
     bs[u] = is_s;
   }
-  floatx4 acc[kG2PerWave];
+  V4 acc[kG2PerWave];
 #pragma unroll
   for (int u = 0; u < kG2PerWave; ++u)
-    acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    acc[u] = V4{(T)0, (T)0, (T)0, (T)0};
   // staging: element e = tid + u*512 of the (32 rows x QP cols) slab of Z and
   // of AZ: row e % 32, column e / 32
   constexpr int kLd = kG2Slab * kG2MaxQ / kG2Threads;  // 8 per operand
-  float za[kLd], aa[kLd];
+  T za[kLd], aa[kLd];
   auto load = [&](int64_t rb) {
 #pragma unroll
     for (int u = 0; u < kLd; ++u) {
       const int e = threadIdx.x + u * kG2Threads;
       const int r = e % kG2Slab, c = e / kG2Slab;
       const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
-      za[u] = c < q ? Z[row + (int64_t)c * ldz] : 0.f;
-      aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : 0.f;
+      za[u] = c < q ? Z[row + (int64_t)c * ldz] : (T)0;
+      aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : (T)0;
     }
   };
   if (r0 < r1)
@@ -815,8 +907,8 @@ __global__ void __launch_bounds__(kG2Threads)
       const int r = e % kG2Slab, c = e / kG2Slab;
       if (c < QP) {
         const bool ok = (rb + r) < r1 && c < q;
-        Zs[c * kG2Ld + r] = ok ? za[u] : 0.f;
-        As[c * kG2Ld + r] = ok ? aa[u] : 0.f;
+        Zs[c * kG2Ld + r] = ok ? za[u] : (T)0;
+        As[c * kG2Ld + r] = ok ? aa[u] : (T)0;
       }
     }
     __syncthreads();
@@ -827,14 +919,14 @@ __global__ void __launch_bounds__(kG2Threads)
 #pragma unroll
       for (int u = 0; u < kG2PerWave; ++u) {
         if (wid + u * kG2Waves < ntiles) {
-          const float a = Zs[aoff[u] + k];
-          const float b = bs[u] ? Zs[boff[u] + k] : As[boff[u] + k];
-          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[u], 0, 0, 0);
+          const T a = Zs[aoff[u] + k];
+          const T b = bs[u] ? Zs[boff[u] + k] : As[boff[u] + k];
+          acc[u] = Mfma16<T>::step(a, b, acc[u]);
         }
       }
     }
   }
-  float *out = part + (int64_t)blockIdx.x * ntiles * 256;
+  T *out = part + (int64_t)blockIdx.x * ntiles * 256;
 #pragma unroll
   for (int u = 0; u < kG2PerWave; ++u) {
     const int t = wid + u * kG2Waves;
@@ -847,8 +939,9 @@ __global__ void __launch_bounds__(kG2Threads)
 
 // SG[0 : q*q] = S, SG[q*q : 2*q*q] = G, both column-major q x q; S is
 // mirrored from its upper triangle.
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-    k_gram2_finish(const float *__restrict__ part, int nblk, int q, double *__restrict__ SG) {
+    k_gram2_finish(const T *__restrict__ part, int nblk, int q, double *__restrict__ SG) {
   const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t qq = (int64_t)q * q;
   if (e >= 2 * qq)
@@ -861,15 +954,15 @@ __global__ void __launch_bounds__(kBlock)
     i = j;
     j = t;
   }
-  const int T = (q + 15) / 16;
-  const int ntiles = T * (T + 1) / 2 + T * T;
+  const int nt = (q + 15) / 16;
+  const int ntiles = nt * (nt + 1) / 2 + nt * nt;
   const int ti = i / 16, tj = j / 16;
   int t;
   if (!is_g)
-    t = ti * T - ti * (ti - 1) / 2 + (tj - ti);
+    t = ti * nt - ti * (ti - 1) / 2 + (tj - ti);
   else
-    t = T * (T + 1) / 2 + ti * T + tj;
-  const float *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
+    t = nt * (nt + 1) / 2 + ti * nt + tj;
+  const T *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
   double acc = 0.0;
   for (int b = 0; b < nblk; ++b)
     acc += (double)p[(int64_t)b * ntiles * 256];
@@ -1435,34 +1528,96 @@ void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
   KH_CHECK(hipGetLastError());
 }
 
-void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
-                int64_t len, double *SG) {
+void spmm_rows(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s, const void *X,
+               void *Y) {
+  const int64_t rows = trans ? A.n : A.m;
+  if (rows <= 0 || s <= 0)
+    return;
+  const double e = p == Prec::F64 ? 8.0 : 4.0;
+  TraceScope tr(ctx, kTraceSpmv,
+                (e + 4.0) * A.nnz + 4.0 * (rows + 1) + e * s * ((trans ? A.m : A.n) + rows));
+  const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
+  const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
+  const int64_t total = rows * (int64_t)s;
+  int64_t g = (total + kBlock - 1) / kBlock;
+  g = std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_spmm_rows<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, rows,
+                       ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(), s,
+                       (const double *)X, (double *)Y);
+  else
+    hipLaunchKernelGGL(k_spmm_rows<float>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, rows,
+                       ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(), s,
+                       (const float *)X, (float *)Y);
+  KH_CHECK(hipGetLastError());
+}
+
+void cols_to_rows(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *R) {
+  if (len <= 0 || s <= 0)
+    return;
+  const dim3 grid((unsigned)((len + 63) / 64), (unsigned)((s + 63) / 64));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_cols_to_rows<double>, grid, dim3(kBlock), 0, ctx.stream, len, s,
+                       (const double *)X, ldx, (double *)R);
+  else
+    hipLaunchKernelGGL(k_cols_to_rows<float>, grid, dim3(kBlock), 0, ctx.stream, len, s,
+                       (const float *)X, ldx, (float *)R);
+  KH_CHECK(hipGetLastError());
+}
+
+void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void *X, int64_t ldx) {
+  if (len <= 0 || s <= 0)
+    return;
+  const dim3 grid((unsigned)((len + 63) / 64), (unsigned)((s + 63) / 64));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_rows_to_cols<double>, grid, dim3(kBlock), 0, ctx.stream, len, s,
+                       (const double *)R, (double *)X, ldx);
+  else
+    hipLaunchKernelGGL(k_rows_to_cols<float>, grid, dim3(kBlock), 0, ctx.stream, len, s,
+                       (const float *)R, (float *)X, ldx);
+  KH_CHECK(hipGetLastError());
+}
+
+template <class T>
+static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, const T *AZ, int64_t ldaz, int q,
+                         int64_t len, double *SG) {
   if (q <= 0 || len <= 0)
     return;
   if (q > kG2MaxQ)
     throw std::invalid_argument("gram2_mfma: q above the fused-kernel limit");
-  const int T = (q + 15) / 16;
-  const int ntiles = T * (T + 1) / 2 + T * T;
+  const int nt = (q + 15) / 16;
+  const int ntiles = nt * (nt + 1) / 2 + nt * nt;
   // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
   // unless the partials would not fit the context's reduction buffer
-  const int64_t maxblk =
-      std::max<int64_t>(1, (int64_t)(2 * ctx.part_cap) / ((int64_t)ntiles * 256));
+  const int64_t maxblk = std::max<int64_t>(
+      1, (int64_t)(ctx.part_cap * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
   int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);
   rpb = std::min<int64_t>(rpb, 4096);
   rpb = std::max<int64_t>(rpb, (len + maxblk - 1) / maxblk);
   rpb = (rpb + kG2Slab - 1) / kG2Slab * kG2Slab;
   const int64_t nblk64 = (len + rpb - 1) / rpb;
-  const std::size_t need = (std::size_t)nblk64 * ntiles * 256;  // floats
-  float *part = reinterpret_cast<float *>(ctx.partials((need + 1) / 2));
+  const std::size_t need = (std::size_t)nblk64 * ntiles * 256;  // elements
+  T *part = reinterpret_cast<T *>(ctx.partials((need * sizeof(T) + 7) / 8));
   {
-    TraceScope tr(ctx, kTraceGram, 2.0 * (double)len * q * q * 2.0);
-    hipLaunchKernelGGL(k_gram2_partial, dim3((unsigned)nblk64), dim3(kG2Threads), 0, ctx.stream,
-                       len, rpb, Z, ldz, AZ, ldaz, q, part);
+    TraceScope tr(ctx, sizeof(T) == 4 ? kTraceGram : kTraceNone,
+                  2.0 * (double)len * q * q * 2.0);
+    hipLaunchKernelGGL(k_gram2_partial<T>, dim3((unsigned)nblk64), dim3(kG2Threads), 0,
+                       ctx.stream, len, rpb, Z, ldz, AZ, ldaz, q, part);
   }
   const int64_t outs = 2 * (int64_t)q * q;
-  hipLaunchKernelGGL(k_gram2_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)), dim3(kBlock),
-                     0, ctx.stream, part, (int)nblk64, q, SG);
+  hipLaunchKernelGGL(k_gram2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, ctx.stream, part, (int)nblk64, q, SG);
   KH_CHECK(hipGetLastError());
+}
+
+void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
+                int64_t len, double *SG) {
+  launch_gram2(ctx, Z, ldz, AZ, ldaz, q, len, SG);
+}
+
+void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
+                int q, int64_t len, double *SG) {
+  launch_gram2(ctx, Z, ldz, AZ, ldaz, q, len, SG);
 }
 
 void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,

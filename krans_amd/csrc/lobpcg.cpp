@@ -165,6 +165,7 @@ private:
   double *sc_ = nullptr;
   std::vector<double> hbuf_;
   Blk tmpq_;  // m x q scratch for apply_normal
+  DBuf zr_, tr_;  // row-major n x q and m x q blocks of the row-major path
   DBuf hproj_;  // locked^T X coefficients of a block projection
 
   void random_col(void *dst, Prec p);
@@ -325,13 +326,31 @@ bool LobpcgDriver::orthonormalize(Blk &X, int i0, int cnt, double tol, bool reor
   return true;
 }
 
-// :54-64  AZ = A^T (A Z), q columns
+// :54-64  AZ = A^T (A Z), q columns. Blocks of 4 or more columns go through
+// row-major copies, where one gather reads a whole q-wide row of the block
+// (the column-major gathers would fetch a cache line per column and nonzero
+// once the block outgrows the caches); per-entry arithmetic is unchanged.
 void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
   auto t0 = Clock::now();
-  if (tmpq_.cap < q || tmpq_.p != Z.p || AZ.p != Z.p)
-    tmpq_.init(m_, std::max(q, 8), Z.p);
   if (AZ.p != Z.p || AZ.cap < q || Z.cap < q)
     throw std::logic_error("apply_normal: block precision or width mismatch");
+  if (q >= 4 && A_.has_rows()) {
+    const std::size_t e = esz(Z.p);
+    if (zr_.bytes < (std::size_t)n_ * q * e)
+      zr_ = DBuf((std::size_t)n_ * q * e);
+    if (tr_.bytes < (std::size_t)m_ * q * e)
+      tr_ = DBuf((std::size_t)m_ * q * e);
+    cols_to_rows(c_, Z.p, n_, q, Z.col(0), n_, zr_.p);
+    A_.mm_rows(c_, Z.p, q, zr_.p, tr_.p, false);
+    A_.mm_rows(c_, Z.p, q, tr_.p, zr_.p, true);
+    rows_to_cols(c_, Z.p, n_, q, zr_.p, AZ.col(0), n_);
+    S_.spmv += q;
+    S_.spmtv += q;
+    S_.t_sparse += since(t0);
+    return;
+  }
+  if (tmpq_.cap < q || tmpq_.p != Z.p)
+    tmpq_.init(m_, std::max(q, 8), Z.p);
   A_.mm(c_, Z.p, q, Z.col(0), n_, tmpq_.col(0), m_, nullptr);
   A_.mtm(c_, Z.p, q, tmpq_.col(0), m_, AZ.col(0), n_, nullptr);
   S_.spmv += q;
@@ -345,6 +364,9 @@ void LobpcgDriver::gram(const Blk &Z, const Blk &AZ, int q, dense::Mat &S, dense
   const int qq = q * q;
   if (Z.p == Prec::F32 && AZ.p == Prec::F32 && q <= 128) {  // fused FP32 MFMA pass
     gram2_mfma(c_, (const float *)Z.col(0), n_, (const float *)AZ.col(0), n_, q, n_, sc_ + 1024);
+  } else if (Z.p == Prec::F64 && AZ.p == Prec::F64 && q <= 128) {  // fused FP64 MFMA pass
+    gram2_mfma(c_, (const double *)Z.col(0), n_, (const double *)AZ.col(0), n_, q, n_,
+               sc_ + 1024);
   } else if (Z.p == Prec::F32 && AZ.p == Prec::F32) {  // FP32 MFMA tiles
     gram_mfma(c_, (const float *)Z.col(0), n_, q, (const float *)Z.col(0), n_, q, n_,
               sc_ + 1024);
@@ -773,7 +795,18 @@ DeviceBasis LobpcgDriver::run() {
       axpby(c_, bp_, n_, -theta[(std::size_t)j], X.col(j), 1.0, Rb.col(j));
     }
     // ||A x_j|| for all j at once (X does not change within the loop)
-    A_.mm(c_, bp_, B, X.col(0), n_, AXres.col(0), m_, nullptr);
+    if (B >= 4 && A_.has_rows()) {
+      const std::size_t e = esz(bp_);
+      if (zr_.bytes < (std::size_t)n_ * B * e)
+        zr_ = DBuf((std::size_t)n_ * B * e);
+      if (tr_.bytes < (std::size_t)m_ * B * e)
+        tr_ = DBuf((std::size_t)m_ * B * e);
+      cols_to_rows(c_, bp_, n_, B, X.col(0), n_, zr_.p);
+      A_.mm_rows(c_, bp_, B, zr_.p, tr_.p, false);
+      rows_to_cols(c_, bp_, m_, B, tr_.p, AXres.col(0), m_);
+    } else {
+      A_.mm(c_, bp_, B, X.col(0), n_, AXres.col(0), m_, nullptr);
+    }
     S_.spmv += B;
     col_norms(c_, bp_, AXres.col(0), m_, B, m_, sc_ + 200);
     std::vector<double> avo((std::size_t)B);
@@ -840,7 +873,7 @@ DeviceBasis LobpcgDriver::run() {
           ++locked_total;
           ++nconv;
           if (P_.verbose)
-            std::printf("[LOBPCG-HIP] accept: ||A v||=%.3e locked=%lld/%lld\n", Av_acc,
+            std::fprintf(stderr, "[LOBPCG-HIP] accept: ||A v||=%.3e locked=%lld/%lld\n", Av_acc,
                         (long long)locked_total, (long long)k_target);
           if (locked_total >= k_target)
             break;
@@ -855,7 +888,7 @@ DeviceBasis LobpcgDriver::run() {
     S_.iters = its;
     S_.best_unaccepted_residual = std::isfinite(best_unacc) ? best_unacc : 0.0;
     if (P_.verbose && P_.progress_every > 0 && (its == 1 || its % P_.progress_every == 0))
-      std::printf("[LOBPCG-HIP] it=%lld locked=%lld/%lld best=%.3e thr=%.3e\n",
+      std::fprintf(stderr, "[LOBPCG-HIP] it=%lld locked=%lld/%lld best=%.3e thr=%.3e\n",
                   (long long)its, (long long)locked_total, (long long)k_target,
                   S_.best_unaccepted_residual, accept_thr);
     if (locked_total >= k_target)

@@ -2,6 +2,7 @@
 // abstraction they run on, the reference-compatible random stream, options,
 // results and the two drivers (cyclic Lanczos / Golub-Kahan and LOBPCG).
 #pragma once
+#include <stdexcept>
 #include <condition_variable>
 #include <cstdint>
 #include <memory>
@@ -29,6 +30,13 @@ struct DeviceOp {
                   int64_t ldy, const int *act) const = 0;
   virtual void mtm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y,
                    int64_t ldy, const int *act) const = 0;
+  // Block products on row-major blocks (stride s): Y = op(A) X, op = A or
+  // A^T. Operators without them report false and callers use mm/mtm.
+  virtual bool has_rows() const { return false; }
+  virtual void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const {
+    (void)c, (void)p, (void)s, (void)X, (void)Y, (void)trans;
+    throw std::logic_error("row-major block products not provided by this operator");
+  }
   // diag(A A^T) on the device (Jacobi preconditioner), or nullptr.
   virtual const double *diag_AAt() const { return nullptr; }
   virtual double fro() const { return 0.0; }
@@ -56,6 +64,10 @@ struct CsrOp final : DeviceOp {
   void mtm(Context &c, Prec p, int s, const void *x, int64_t ldx, void *y, int64_t ldy,
            const int *act) const override {
     spmm(c, A, !t, p, s, x, ldx, y, ldy, act);
+  }
+  bool has_rows() const override { return true; }
+  void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const override {
+    spmm_rows(c, A, t != trans, p, s, X, Y);
   }
   const double *diag_AAt() const override { return t ? nullptr : A.diag_AAt.as<double>(); }
   double fro() const override { return A.fro; }
