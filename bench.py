@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--scale", type=float, default=1.0,
                     help="c3a/c3b/c4 only: shrink the problem size by this factor")
     ap.add_argument("--max-iters", type=int, default=3000, help="c4: LOBPCG max_iters")
+    ap.add_argument("--matrix-free", action="store_true",
+                    help="c3a: apply the shifted difference as a stencil (no CSR in HBM)")
     return ap.parse_args()
 
 
@@ -260,7 +262,11 @@ def variant(args):
         del G
         k, solver = 32, "lobpcg"
         desc = f"LOBPCG nev=32 block 32 ([X W P] 96 columns) on [R | RG], m={m}, n={m + 32}"
-    H = K.HipCsrMatrixAdapter.from_csr(A)
+    if cfg == "c3a" and args.matrix_free:
+        H = K.HipShiftedDifferenceOperator(n, 16, 0.9)
+        desc += " (matrix-free stencil)"
+    else:
+        H = K.HipCsrMatrixAdapter.from_csr(A)
     S = A.to_scipy()
     fro = A.fro()
     setup_s = time.time() - t0

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <functional>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -175,22 +176,13 @@ private:
   kh_rng *rng_ = nullptr;
 };
 
-class HipCsrMatrixAdapter final : public MatrixInterface {
+/** Common base of the HIP matrix operands: a kh_matrix handle (CSR or
+ * matrix-free) on one context. The solvers accept any HipMatrix. */
+class HipMatrix : public MatrixInterface {
 public:
-  /** Canonical CSR on the host (see kh_matrix_create_csr). */
-  HipCsrMatrixAdapter(kh_context *ctx, std::size_t m, std::size_t n, const int32_t *rowptr,
-                      const int32_t *col, const double *val)
-      : ctx_(ctx) {
-    check(kh_matrix_create_csr(ctx, (int64_t)m, (int64_t)n, rowptr, col, val, &A_));
-    int64_t mm = 0, nn = 0, nz = 0;
-    check(kh_matrix_info(A_, &mm, &nn, &nz, &fro_));
-    m_ = (std::size_t)mm;
-    n_ = (std::size_t)nn;
-    nnz_ = (std::size_t)nz;
-  }
-  ~HipCsrMatrixAdapter() override { kh_matrix_destroy(A_); }
-  HipCsrMatrixAdapter(const HipCsrMatrixAdapter &) = delete;
-  HipCsrMatrixAdapter &operator=(const HipCsrMatrixAdapter &) = delete;
+  ~HipMatrix() override { kh_matrix_destroy(A_); }
+  HipMatrix(const HipMatrix &) = delete;
+  HipMatrix &operator=(const HipMatrix &) = delete;
 
   void multiply(const VectorInterface &x, VectorInterface &y) const override {
     apply(x, y, 0, "multiply");
@@ -200,22 +192,33 @@ public:
   }
   std::size_t rows() const override { return m_; }
   std::size_t cols() const override { return n_; }
+  /** Stored entries (CSR), the entries a CSR would hold (stencil), or
+   * SIZE_MAX when unknown (callback operator). */
   std::size_t nnz() const { return nnz_; }
   double frobenius_norm() const { return fro_; }
   kh_matrix *handle() const { return A_; }
   kh_context *context() const { return ctx_; }
+
+protected:
+  explicit HipMatrix(kh_context *ctx) : ctx_(ctx) {}
+  void adopt(kh_matrix *A) {
+    A_ = A;
+    int64_t mm = 0, nn = 0, nz = 0;
+    check(kh_matrix_info(A_, &mm, &nn, &nz, &fro_));
+    m_ = (std::size_t)mm;
+    n_ = (std::size_t)nn;
+    nnz_ = nz < 0 ? (std::size_t)-1 : (std::size_t)nz;
+  }
 
 private:
   void apply(const VectorInterface &x, VectorInterface &y, int t, const char *what) const {
     const auto *hx = dynamic_cast<const HipVector *>(&x);
     auto *hy = dynamic_cast<HipVector *>(&y);
     if (!hx || !hy)
-      throw std::invalid_argument(std::string("HipCsrMatrixAdapter::") + what +
-                                  " backend mismatch");
+      throw std::invalid_argument(std::string("HipMatrix::") + what + " backend mismatch");
     const std::size_t nin = t ? m_ : n_, nout = t ? n_ : m_;
     if (hx->size() != nin || hy->size() != nout)
-      throw std::invalid_argument(std::string("HipCsrMatrixAdapter::") + what +
-                                  " size mismatch");
+      throw std::invalid_argument(std::string("HipMatrix::") + what + " size mismatch");
     check(kh_spmm_dev(A_, t, KH_PREC_FP64, 1, hx->device_ptr(), hy->device_ptr()));
   }
   kh_context *ctx_;
@@ -224,11 +227,71 @@ private:
   double fro_ = 0.0;
 };
 
+/** Replaces EigenSparseMatrixAdapter: canonical CSR on the host (see
+ * kh_matrix_create_csr), held in HBM with its CSC twin. */
+class HipCsrMatrixAdapter final : public HipMatrix {
+public:
+  HipCsrMatrixAdapter(kh_context *ctx, std::size_t m, std::size_t n, const int32_t *rowptr,
+                      const int32_t *col, const double *val)
+      : HipMatrix(ctx) {
+    kh_matrix *A = nullptr;
+    check(kh_matrix_create_csr(ctx, (int64_t)m, (int64_t)n, rowptr, col, val, &A));
+    adopt(A);
+  }
+};
+
+/** The quick start's matrix-free operator (examples/quickstart.cpp:25-37,
+ * a StdMatrixAdapter of two lambdas): (A x)_i = x_{i+N} - c x_i, A is
+ * (n-N) x n, applied by HIP kernels with no matrix in memory. */
+class HipShiftedDifferenceOperator final : public HipMatrix {
+public:
+  HipShiftedDifferenceOperator(kh_context *ctx, std::size_t n, std::size_t N, double c)
+      : HipMatrix(ctx) {
+    kh_matrix *A = nullptr;
+    check(kh_matrix_create_shifted_difference(ctx, (int64_t)n, (int64_t)N, c, &A));
+    adopt(A);
+  }
+};
+
+/** The device analogue of StdMatrixAdapter (include/krans/adapters/
+ * universal/StdMatrixAdapter.hpp:24-87): a user operator that launches its
+ * own device work. apply(stream, precision, transpose, s, x, ldx, y, ldy)
+ * computes y(:, j) = op(A) x(:, j), j < s, on device columns of double
+ * (Precision::FP64) or float (Precision::Mixed), enqueued on `stream` (a
+ * hipStream_t) or finished before returning; throwing reports failure. It
+ * is called with all earlier work on the stream complete. */
+class HipOperatorAdapter final : public HipMatrix {
+public:
+  using Apply = std::function<void(void *stream, Precision p, bool transpose, int s,
+                                   const void *x, int64_t ldx, void *y, int64_t ldy)>;
+  HipOperatorAdapter(kh_context *ctx, std::size_t m, std::size_t n, Apply apply)
+      : HipMatrix(ctx), apply_(std::move(apply)) {
+    kh_matrix *A = nullptr;
+    check(kh_matrix_create_operator(ctx, (int64_t)m, (int64_t)n, &HipOperatorAdapter::call,
+                                    this, &A));
+    adopt(A);
+  }
+
+private:
+  static int call(void *user, void *stream, int precision, int transpose, int s, const void *x,
+                  int64_t ldx, void *y, int64_t ldy) {
+    try {
+      static_cast<HipOperatorAdapter *>(user)->apply_(
+          stream, precision == KH_PREC_FP64 ? Precision::FP64 : Precision::Mixed,
+          transpose != 0, s, x, ldx, y, ldy);
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+  Apply apply_;
+};
+
 /** M^{-1} = diag(A A^T)^{-1} for the polish CG (CyclicLanczosOptions::
  * precond_AAt). The HIP solver recognises it and applies it inside its CG
  * kernels; called directly it runs the same device kernel. */
 struct JacobiAAtPreconditioner {
-  const HipCsrMatrixAdapter *A;
+  const HipMatrix *A;
   void operator()(const VectorInterface &r, VectorInterface &y) const {
     const auto *hr = dynamic_cast<const HipVector *>(&r);
     auto *hy = dynamic_cast<HipVector *>(&y);

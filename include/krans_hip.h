@@ -104,6 +104,28 @@ int kh_context_sync(kh_context *ctx);
 int kh_matrix_create_csr(kh_context *ctx, int64_t m, int64_t n,
                          const int32_t *rowptr, const int32_t *col,
                          const double *val, kh_matrix **out);
+/* Matrix-free operators (device analogues of the reference's
+ * StdMatrixAdapter with y = A x / y = A^T x callbacks,
+ * include/krans/adapters/universal/StdMatrixAdapter.hpp:24-87):
+ *   kh_matrix_create_shifted_difference: (A x)_i = x_{i+N} - c x_i, A is
+ *     (n-N) x n (examples/quickstart.cpp:25-37), as HIP kernels with no
+ *     matrix in memory; products are bit-identical to the materialised CSR.
+ *   kh_matrix_create_operator: `apply(user, stream, precision, transpose, s,
+ *     x, ldx, y, ldy)` must compute y(:, j) = op(A) x(:, j) for j < s (op = A,
+ *     or A^T when transpose != 0) on device columns of type double
+ *     (precision KH_PREC_FP64) or float (KH_PREC_MIXED), enqueued on `stream`
+ *     (a hipStream_t) or finished before it returns, and return 0 on success
+ *     (anything else surfaces as KH_ERUNTIME). It is called with all earlier
+ *     work on the stream complete; `user` must outlive the matrix.
+ * kh_matrix_download_* and kh_server_submit_resident need a CSR matrix
+ * (KH_EINVAL for these operators); kh_matrix_info reports nnz = 2(n-N) for
+ * the stencil and -1 for a callback operator, whose ||A||_F reads 0. */
+typedef int (*kh_apply_fn)(void *user, void *stream, int precision, int transpose, int s,
+                           const void *x, int64_t ldx, void *y, int64_t ldy);
+int kh_matrix_create_shifted_difference(kh_context *ctx, int64_t n, int64_t N, double c,
+                                        kh_matrix **out);
+int kh_matrix_create_operator(kh_context *ctx, int64_t m, int64_t n, kh_apply_fn apply,
+                              void *user, kh_matrix **out);
 int kh_matrix_destroy(kh_matrix *A);
 /* --- matrix ingestion (host only; no device or context needed) ---------- *
  * Canonical CSR from an IJA triplet file, a Matrix Market file or raw 0-based

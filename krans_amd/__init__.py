@@ -91,6 +91,8 @@ SIGNATURES = {
     "kh_context_destroy": (I32, [P]),
     "kh_context_sync": (I32, [P]),
     "kh_matrix_create_csr": (I32, [P, I64, I64, P, P, P, C.POINTER(P)]),
+    "kh_matrix_create_shifted_difference": (I32, [P, I64, I64, D, C.POINTER(P)]),
+    "kh_matrix_create_operator": (I32, [P, I64, I64, P, P, C.POINTER(P)]),
     "kh_load_ija": (I32, [C.c_char_p, I64, I64, C.POINTER(P)]),
     "kh_load_matrix_market": (I32, [C.c_char_p, C.POINTER(P)]),
     "kh_csr_from_triplets": (I32, [I64, I64, I64, P, P, P, C.POINTER(P)]),
@@ -220,28 +222,16 @@ def default_context():
     return _default_ctx
 
 
-class HipCsrMatrixAdapter:
-    """MatrixInterface over a device CSR matrix with its CSC twin."""
+class HipMatrix:
+    """MatrixInterface over a kh_matrix handle (a device CSR matrix or a
+    matrix-free device operator); the solvers accept any HipMatrix."""
 
-    def __init__(self, m, n, rowptr, col, val, ctx=None):
-        self.ctx = ctx or default_context()
-        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
-        col = np.ascontiguousarray(col, dtype=np.int32)
-        val = np.ascontiguousarray(val, dtype=np.float64)
-        if rowptr.shape[0] != m + 1:
-            raise ValueError("rowptr must have m+1 entries")
-        h = P()
-        _check(lib().kh_matrix_create_csr(self.ctx.h, m, n, _ptr(rowptr), _ptr(col),
-                                          _ptr(val), C.byref(h)))
+    def _adopt(self, h, ctx):
+        self.ctx = ctx
         self.h = h
         mm, nn, nz, fr = I64(), I64(), I64(), D()
         _check(lib().kh_matrix_info(h, C.byref(mm), C.byref(nn), C.byref(nz), C.byref(fr)))
         self.m, self.n, self.nnz, self.fro = mm.value, nn.value, nz.value, fr.value
-
-    @classmethod
-    def from_csr(cls, A, ctx=None):
-        """From any object with m, n, rowptr, col, val (e.g. workloads.CsrMatrix)."""
-        return cls(A.m, A.n, A.rowptr, A.col, A.val, ctx=ctx)
 
     def rows(self):
         return self.m
@@ -260,10 +250,30 @@ class HipCsrMatrixAdapter:
         except Exception:
             pass
 
+class HipCsrMatrixAdapter(HipMatrix):
+    """MatrixInterface over a device CSR matrix with its CSC twin."""
+
+    def __init__(self, m, n, rowptr, col, val, ctx=None):
+        ctx = ctx or default_context()
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        if rowptr.shape[0] != m + 1:
+            raise ValueError("rowptr must have m+1 entries")
+        h = P()
+        _check(lib().kh_matrix_create_csr(ctx.h, m, n, _ptr(rowptr), _ptr(col),
+                                          _ptr(val), C.byref(h)))
+        self._adopt(h, ctx)
+
+    @classmethod
+    def from_csr(cls, A, ctx=None):
+        """From any object with m, n, rowptr, col, val (e.g. workloads.CsrMatrix)."""
+        return cls(A.m, A.n, A.rowptr, A.col, A.val, ctx=ctx)
+
     def multiply(self, x, precision="fp64"):
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape != (self.n,):
-            raise ValueError("HipCsrMatrixAdapter::multiply size mismatch")
+            raise ValueError("HipMatrix::multiply size mismatch")
         y = np.empty(self.m)
         _check(lib().kh_matrix_multiply(self.h, 0, PREC[precision], _ptr(x), _ptr(y)))
         return y
@@ -271,7 +281,7 @@ class HipCsrMatrixAdapter:
     def multiply_transpose(self, x, precision="fp64"):
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape != (self.m,):
-            raise ValueError("HipCsrMatrixAdapter::multiply_transpose size mismatch")
+            raise ValueError("HipMatrix::multiply_transpose size mismatch")
         y = np.empty(self.n)
         _check(lib().kh_matrix_multiply(self.h, 1, PREC[precision], _ptr(x), _ptr(y)))
         return y
@@ -295,6 +305,53 @@ class HipCsrMatrixAdapter:
         out = D()
         _check(lib().kh_time_spmv(self.h, int(transpose), PREC[precision], reps, C.byref(out)))
         return out.value
+
+
+for _name in ("multiply", "multiply_transpose", "time_spmv"):
+    setattr(HipMatrix, _name, getattr(HipCsrMatrixAdapter, _name))
+
+
+class HipShiftedDifferenceOperator(HipMatrix):
+    """The quick start's matrix-free operator (examples/quickstart.cpp:25-37):
+    (A x)_i = x_{i+N} - c x_i, A is (n-N) x n, applied by HIP kernels with no
+    matrix in memory (kh_matrix_create_shifted_difference)."""
+
+    def __init__(self, n, N, c, ctx=None):
+        ctx = ctx or default_context()
+        h = P()
+        _check(lib().kh_matrix_create_shifted_difference(ctx.h, int(n), int(N), float(c),
+                                                         C.byref(h)))
+        self._adopt(h, ctx)
+
+
+APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                       C.c_int64, C.c_void_p, C.c_int64)
+
+
+class HipOperatorAdapter(HipMatrix):
+    """A user operator (kh_matrix_create_operator), the device analogue of
+    StdMatrixAdapter: apply(stream, precision, transpose, s, x, ldx, y, ldy)
+    with device pointers (ints) computes y(:, j) = op(A) x(:, j), j < s, for
+    precision "fp64" (double columns) or "mixed" (float columns), finished
+    or enqueued on `stream` before it returns. An exception reports failure."""
+
+    def __init__(self, m, n, apply, ctx=None):
+        ctx = ctx or default_context()
+        self._apply = apply
+
+        def tramp(user, stream, precision, transpose, s, x, ldx, y, ldy):
+            try:
+                apply(stream, "fp64" if precision == 0 else "mixed", bool(transpose), s, x, ldx,
+                      y, ldy)
+                return 0
+            except Exception as e:  # reported as KH_ERUNTIME by the solver
+                self.last_error = e
+                return 1
+        self._tramp = APPLY_FN(tramp)  # kept alive with the matrix
+        h = P()
+        _check(lib().kh_matrix_create_operator(ctx.h, int(m), int(n), self._tramp, None,
+                                               C.byref(h)))
+        self._adopt(h, ctx)
 
 
 class HipVectorFactory:
@@ -394,7 +451,7 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
 
     Returns right_null as an (n x k) array of orthonormal columns.
     """
-    if not isinstance(A, HipCsrMatrixAdapter):
+    if not isinstance(A, HipMatrix):
         raise ValueError("find_nullspace_cyclic_lanczos: backend mismatch (HIP matrix required)")
     o = opt._c()
     kcap = max(int(opt.k), int(opt.nullity_known), 1)
@@ -477,7 +534,7 @@ class LOBPCGResult:
 
 def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
     """find_nullspace_lobpcg_ex (LOBPCGSolver.hpp:142-144)."""
-    if not isinstance(A, HipCsrMatrixAdapter):
+    if not isinstance(A, HipMatrix):
         raise ValueError("find_nullspace_lobpcg: backend mismatch (HIP matrix required)")
     o = opt._c()
     kcap = max(int(opt.nev), int(opt.nullity_known), 1)
@@ -592,7 +649,7 @@ class NullspaceServer:
         HipCsrMatrixAdapter (resident). Returns a request id."""
         o = opt._c()
         rid = I64()
-        if isinstance(A, HipCsrMatrixAdapter):
+        if isinstance(A, HipMatrix):
             _check(lib().kh_server_submit_resident(self.h, A.h, C.byref(o), PREC[precision],
                                                    seed, C.byref(rid)))
             n = A.n

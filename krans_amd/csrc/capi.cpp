@@ -203,6 +203,31 @@ int kh_host_csr_free(kh_host_csr *h) {
   return guard([&] { delete h; });
 }
 
+int kh_matrix_create_shifted_difference(kh_context *ctx, int64_t n, int64_t N, double c,
+                                        kh_matrix **out) {
+  return guard([&] {
+    need(ctx && out, "null argument");
+    auto M = std::make_unique<kh_matrix_s>();
+    M->owner = ctx;
+    M->op = std::make_unique<kh::StencilOp>(*ctx->ctx, n, N, c);
+    M->nnz_equiv = 2 * (n - N);
+    *out = M.release();
+  });
+}
+
+int kh_matrix_create_operator(kh_context *ctx, int64_t m, int64_t n, kh_apply_fn apply,
+                              void *user, kh_matrix **out) {
+  return guard([&] {
+    need(ctx && out && apply, "null argument");
+    need(m >= 0 && n >= 0, "negative dimensions");
+    auto M = std::make_unique<kh_matrix_s>();
+    M->owner = ctx;
+    M->op = std::make_unique<kh::FunctorOp>(m, n, apply, user);
+    M->nnz_equiv = -1;
+    *out = M.release();
+  });
+}
+
 int kh_matrix_destroy(kh_matrix *A) {
   return guard([&] { delete A; });
 }
@@ -211,13 +236,13 @@ int kh_matrix_info(const kh_matrix *A, int64_t *m, int64_t *n, int64_t *nnz, dou
   return guard([&] {
     need(A, "null matrix");
     if (m)
-      *m = A->A->m;
+      *m = A->rows();
     if (n)
-      *n = A->A->n;
+      *n = A->cols();
     if (nnz)
-      *nnz = A->A->nnz;
+      *nnz = A->A ? A->A->nnz : A->nnz_equiv;
     if (fro)
-      *fro = A->A->fro;
+      *fro = A->fro();
   });
 }
 
@@ -225,7 +250,7 @@ int kh_matrix_download_csr(const kh_matrix *A, int32_t *rowptr, int32_t *col, do
   return guard([&] {
     need(A, "null matrix");
     auto &c = *A->owner->ctx;
-    const auto &D = *A->A;
+    const auto &D = A->csr("kh_matrix_download_csr");
     if (rowptr)
       KH_CHECK(hipMemcpyAsync(rowptr, D.rp.p, (std::size_t)(D.m + 1) * 4,
                               hipMemcpyDeviceToHost, c.stream));
@@ -244,7 +269,7 @@ int kh_matrix_download_csc(const kh_matrix *A, int32_t *colptr, int32_t *rowidx,
   return guard([&] {
     need(A, "null matrix");
     auto &c = *A->owner->ctx;
-    const auto &D = *A->A;
+    const auto &D = A->csr("kh_matrix_download_csc");
     if (colptr)
       KH_CHECK(hipMemcpyAsync(colptr, D.cp.p, (std::size_t)(D.n + 1) * 4,
                               hipMemcpyDeviceToHost, c.stream));
@@ -266,19 +291,19 @@ int kh_matrix_multiply(kh_matrix *A, int transpose, int precision, const double 
   return guard([&] {
     need(A && x && y, "null argument");
     auto &c = *A->owner->ctx;
-    const auto &D = *A->A;
+    const auto V = A->view(transpose != 0);
     const kh::Prec p = prec_of(precision);
-    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    const int64_t nin = (*V).cols(), nout = (*V).rows();
     kh::DBuf dx((std::size_t)std::max<int64_t>(1, nin) * 8),
         dy((std::size_t)std::max<int64_t>(1, nout) * 8);
     KH_CHECK(hipMemcpyAsync(dx.p, x, (std::size_t)nin * 8, hipMemcpyHostToDevice, c.stream));
     if (p == kh::Prec::F64) {
-      kh::spmv(c, D, transpose != 0, p, dx.p, dy.p);
+      (*V).mv(c, p, dx.p, dy.p);
     } else {
       kh::DBuf x32((std::size_t)std::max<int64_t>(1, nin) * 4),
           y32((std::size_t)std::max<int64_t>(1, nout) * 4);
       kh::convert(c, kh::Prec::F64, dx.p, kh::Prec::F32, x32.p, nin);
-      kh::spmv(c, D, transpose != 0, p, x32.p, y32.p);
+      (*V).mv(c, p, x32.p, y32.p);
       kh::convert(c, kh::Prec::F32, y32.p, kh::Prec::F64, dy.p, nout);
       c.sync();
     }
@@ -313,14 +338,14 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     auto &c = *A->owner->ctx;
     kh::CyclicParams P = to_params(*opt, precision);
     P.y0_host = y0;
-    kh::CsrOp op(*A->A, false);
+    const auto op = A->view(false);
     kh::CyclicStats S;
-    kh::DeviceBasis R = kh::cyclic_right(c, op, rng->s, P, S);
+    kh::DeviceBasis R = kh::cyclic_right(c, *op, rng->s, P, S);
     std::memset(res, 0, sizeof(*res));
     res->n_right = R.count;
     put_basis(c, R, right, right_cap);
     kh::CyclicStats SL;
-    if (P.return_left_space && A->A->m > 0 && A->A->n > 0) {
+    if (P.return_left_space && A->rows() > 0 && A->cols() > 0) {
       // CyclicLanczosSolver.cpp:626-659: a second run on A^T with the listed
       // options copied and no preconditioner.
       kh::CyclicParams L = P;
@@ -328,11 +353,11 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
       L.jacobi_AAt = false;
       L.nullity_known = -1;
       if (P.rank_known >= 0) {
-        const int left_nullity = (int)A->A->m - P.rank_known;
+        const int left_nullity = (int)A->rows() - P.rank_known;
         L.nullity_known = std::max(0, std::min(left_nullity, (int)P.k));
       }
-      kh::CsrOp opt_t(*A->A, true);
-      kh::DeviceBasis Lb = kh::cyclic_right(c, opt_t, rng->s, L, SL);
+      const auto opt_t = A->view(true);
+      kh::DeviceBasis Lb = kh::cyclic_right(c, *opt_t, rng->s, L, SL);
       res->n_left = Lb.count;
       put_basis(c, Lb, left, left_cap);
     }
@@ -379,9 +404,9 @@ int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt, int preci
     P.block = prec_of(precision);
     P.initial_block_host = initial_block;
     P.n_initial = initial_block ? n_initial : 0;
-    kh::CsrOp op(*A->A, false);
+    const auto op = A->view(false);
     kh::LobpcgStats S;
-    kh::DeviceBasis R = kh::lobpcg(c, op, rng->s, P, S);
+    kh::DeviceBasis R = kh::lobpcg(c, *op, rng->s, P, S);
     std::memset(res, 0, sizeof(*res));
     res->n_right = R.count;
     put_basis(c, R, right, right_cap);
@@ -432,7 +457,8 @@ int kh_server_submit_resident(kh_server *s, kh_matrix *A, const kh_cyclic_option
                               int precision, uint32_t seed, int64_t *id) {
   return guard([&] {
     need(s && A && opt && id, "null argument");
-    *id = s->srv->submit_resident(A->A.get(), to_params(*opt, precision), seed);
+    *id = s->srv->submit_resident(&A->csr("kh_server_submit_resident"),
+                                  to_params(*opt, precision), seed);
   });
 }
 
@@ -552,7 +578,9 @@ int kh_matrix_jacobi_AAt(kh_matrix *A, const double *r, double *y) {
   return guard([&] {
     need(A && r && y, "null argument");
     auto &c = *A->owner->ctx;
-    kh::diag_solve(c, kh::Prec::F64, A->A->m, A->A->diag_AAt.as<double>(), r, y);
+    const double *d = A->A ? A->A->diag_AAt.as<double>() : A->op->diag_AAt();
+    need(d != nullptr, "kh_matrix_jacobi_AAt: this operator has no diag(A A^T)");
+    kh::diag_solve(c, kh::Prec::F64, A->rows(), d, r, y);
     c.sync();
   });
 }
@@ -561,12 +589,12 @@ int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s, const void *x
   return guard([&] {
     need(A && x && y && s >= 1, "bad argument");
     auto &c = *A->owner->ctx;
-    const auto &D = *A->A;
-    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    const auto V = A->view(transpose != 0);
+    const int64_t nin = (*V).cols(), nout = (*V).rows();
     if (s == 1)
-      kh::spmv(c, D, transpose != 0, prec_of(precision), x, y);
+      (*V).mv(c, prec_of(precision), x, y);
     else
-      kh::spmm(c, D, transpose != 0, prec_of(precision), s, x, nin, y, nout, nullptr);
+      (*V).mm(c, prec_of(precision), s, x, nin, y, nout, nullptr);
     c.sync();
   });
 }
@@ -576,7 +604,9 @@ int kh_spmm_rows_dev(kh_matrix *A, int transpose, int precision, int s, const vo
   return guard([&] {
     need(A && x && y && s >= 1, "bad argument");
     auto &c = *A->owner->ctx;
-    kh::spmm_rows(c, *A->A, transpose != 0, prec_of(precision), s, x, y);
+    const auto V = A->view(transpose != 0);
+    need((*V).has_rows(), "kh_spmm_rows_dev: this operator has no row-major block product");
+    (*V).mm_rows(c, prec_of(precision), s, x, y, false);
     c.sync();
   });
 }
@@ -678,20 +708,20 @@ int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps, double *a
   return guard([&] {
     need(A && avg_ms && reps >= 1, "bad argument");
     auto &c = *A->owner->ctx;
-    const auto &D = *A->A;
+    const auto V = A->view(transpose != 0);
     const kh::Prec p = prec_of(precision);
     const std::size_t e = p == kh::Prec::F64 ? 8 : 4;
-    const int64_t nin = transpose ? D.m : D.n, nout = transpose ? D.n : D.m;
+    const int64_t nin = (*V).cols(), nout = (*V).rows();
     kh::DBuf x((std::size_t)std::max<int64_t>(1, nin) * e),
         y((std::size_t)std::max<int64_t>(1, nout) * e);
     kh::fill_zero(c, p, nin, x.p);
     hipEvent_t e0, e1;
     KH_CHECK(hipEventCreate(&e0));
     KH_CHECK(hipEventCreate(&e1));
-    kh::spmv(c, D, transpose != 0, p, x.p, y.p);
+    (*V).mv(c, p, x.p, y.p);
     KH_CHECK(hipEventRecord(e0, c.stream));
     for (int r = 0; r < reps; ++r)
-      kh::spmv(c, D, transpose != 0, p, x.p, y.p);
+      (*V).mv(c, p, x.p, y.p);
     KH_CHECK(hipEventRecord(e1, c.stream));
     KH_CHECK(hipEventSynchronize(e1));
     float ms = 0;

@@ -1,7 +1,7 @@
 // C++ entry points of the KraNS plugin API on the HIP backend:
 // krans::solvers::find_nullspace_cyclic_lanczos[_ex] and
 // find_nullspace_lobpcg[_ex] (reference: CyclicLanczosSolver.hpp:170-178,
-// LOBPCGSolver.hpp:142-149). They accept HipCsrMatrixAdapter +
+// LOBPCGSolver.hpp:142-149). They accept HipMatrix (CSR or matrix-free) +
 // HipVectorFactory operands, run the device-resident drivers and return
 // HipVector results owning HBM; any other backend is a backend mismatch
 // (std::invalid_argument), as in the reference's adapters.
@@ -16,16 +16,16 @@ namespace krans::solvers {
 namespace {
 
 struct HipOperands {
-  const hip::HipCsrMatrixAdapter *A;
+  const hip::HipMatrix *A;
   const hip::HipVectorFactory *F;
 };
 
 HipOperands require_hip(const MatrixInterface &A, const VectorFactory &vf, const char *who) {
-  const auto *H = dynamic_cast<const hip::HipCsrMatrixAdapter *>(&A);
+  const auto *H = dynamic_cast<const hip::HipMatrix *>(&A);
   const auto *F = dynamic_cast<const hip::HipVectorFactory *>(&vf);
   if (!H || !F)
     throw std::invalid_argument(std::string(who) +
-                                ": backend mismatch (HipCsrMatrixAdapter and "
+                                ": backend mismatch (a HIP matrix and a "
                                 "HipVectorFactory required)");
   if (H->context() != F->context())
     throw std::invalid_argument(std::string(who) + ": matrix and factory on different contexts");
@@ -100,9 +100,9 @@ CyclicLanczosResult find_nullspace_cyclic_lanczos_ex(const MatrixInterface &A,
     P.y0_host = y0.data();
   }
   auto &c = *ctx->ctx;
-  kh::CsrOp op(*M->A, false);
+  const auto op = M->view(false);
   kh::CyclicStats S;
-  kh::DeviceBasis R = kh::cyclic_right(c, op, ops.F->rng()->s, P, S);
+  kh::DeviceBasis R = kh::cyclic_right(c, *op, ops.F->rng()->s, P, S);
   CyclicLanczosResult out;
   out.right_null = wrap(ctx, R);
   out.restarts = (std::size_t)S.restarts;
@@ -111,18 +111,18 @@ CyclicLanczosResult find_nullspace_cyclic_lanczos_ex(const MatrixInterface &A,
   out.converged_to_target = S.converged_to_target;
   out.cg_solves = (std::size_t)S.cg_solves;
   out.cg_maxit_hits = (std::size_t)S.cg_maxit_hits;
-  if (opt.return_left_space && M->A->m > 0 && M->A->n > 0) {
+  if (opt.return_left_space && M->rows() > 0 && M->cols() > 0) {
     kh::CyclicParams L = P;
     L.y0_host = nullptr;
     L.jacobi_AAt = false;
     L.nullity_known = -1;
     if (P.rank_known >= 0) {
-      const int left_nullity = (int)M->A->m - P.rank_known;
+      const int left_nullity = (int)M->rows() - P.rank_known;
       L.nullity_known = std::max(0, std::min(left_nullity, (int)P.k));
     }
-    kh::CsrOp opt_t(*M->A, true);
+    const auto opt_t = M->view(true);
     kh::CyclicStats SL;
-    kh::DeviceBasis Lb = kh::cyclic_right(c, opt_t, ops.F->rng()->s, L, SL);
+    kh::DeviceBasis Lb = kh::cyclic_right(c, *opt_t, ops.F->rng()->s, L, SL);
     out.left_null = wrap(ctx, Lb);
     out.cg_solves += (std::size_t)SL.cg_solves;
     out.cg_maxit_hits += (std::size_t)SL.cg_maxit_hits;
@@ -172,9 +172,9 @@ LOBPCGResult find_nullspace_lobpcg_ex(const MatrixInterface &A, const VectorFact
   P.initial_block_host = ib.empty() ? nullptr : ib.data();
   P.n_initial = (int64_t)opt.initial_block.size();
   auto &c = *ctx->ctx;
-  kh::CsrOp op(*M->A, false);
+  const auto op = M->view(false);
   kh::LobpcgStats S;
-  kh::DeviceBasis R = kh::lobpcg(c, op, ops.F->rng()->s, P, S);
+  kh::DeviceBasis R = kh::lobpcg(c, *op, ops.F->rng()->s, P, S);
   LOBPCGResult out;
   out.right_null = wrap(ctx, R);
   out.theta = S.theta;

@@ -129,6 +129,18 @@ void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
 // stored row-major (stride s); same per-entry rounding as spmm.
 void spmm_rows(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s, const void *X,
                void *Y);
+// Shifted-difference stencil (A x)_i = x_{i+N} - c x_i, A (n-N) x n, applied
+// without a matrix: y = op(A) x (+ optional y -= beta z), and on blocks
+// (column-major with active mask, or row-major with stride s).
+void sd_apply(Context &ctx, int64_t n, int64_t N, double c, bool trans, Prec p, const void *x,
+              void *y, const void *z, const double *beta);
+void sd_block(Context &ctx, int64_t n, int64_t N, double c, bool trans, Prec p, int s,
+              const void *x, int64_t ldx, void *y, int64_t ldy, const int *act, bool row_major);
+// y = y + (-beta) z, beta on the device.
+void sub_beta(Context &ctx, Prec p, int64_t len, const double *beta, const void *z, void *y);
+// Y(:, c) = X(:, c) where act[c] (device mask).
+void copy_active(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *Y,
+                 int64_t ldy, const int *act);
 // Column-major (leading dimension ld) <-> row-major (stride s) block copies.
 void cols_to_rows(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *R);
 void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void *X, int64_t ldx);

@@ -635,6 +635,109 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ------------------------------------------ shifted-difference stencil --
+// A is (n-N) x n with (A x)_i = x_{i+N} - c x_i (examples/quickstart.cpp's
+// StdMatrixAdapter lambda), never materialised. Expressions and operand
+// order follow the reference's lambdas term by term, no contraction; the
+// optional epilogue y = y + (-beta) z is the one of the CSR kernels.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sd_mv(int64_t m, int64_t N, double c, const T *__restrict__ x, T *__restrict__ y,
+            const T *__restrict__ z, const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  const T cc = (T)c;
+  const T b = z ? (T)(*beta) : (T)0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * kBlock) {
+    T out = x[i + N] - cc * x[i];
+    if (z)
+      out = out + (-b) * z[i];
+    y[i] = out;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sd_mtv(int64_t n, int64_t N, double c, const T *__restrict__ x, T *__restrict__ y,
+             const T *__restrict__ z, const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  const T cc = (T)c;
+  const T b = z ? (T)(*beta) : (T)0;
+  const int64_t m = n - N;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * kBlock) {
+    T out = (j >= N ? x[j - N] : (T)0) - (j < m ? cc * x[j] : (T)0);
+    if (z)
+      out = out + (-b) * z[j];
+    y[j] = out;
+  }
+}
+
+// Blocks: column-major (ldx, ldy, optional active mask) or row-major (stride
+// s, rm != 0); thread t covers (row t % rows, column t / rows) column-major,
+// (row t / s, column t % s) row-major.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sd_block(int64_t n, int64_t N, double c, int trans, int s, const T *__restrict__ x,
+               int64_t ldx, T *__restrict__ y, int64_t ldy, const int *__restrict__ act, int rm) {
+#pragma clang fp contract(off)
+  const T cc = (T)c;
+  const int64_t m = n - N;
+  const int64_t rows = trans ? n : m;
+  const int64_t total = rows * (int64_t)s;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kBlock) {
+    int64_t i;
+    int col;
+    if (rm) {
+      i = t / s;
+      col = (int)(t - i * s);
+    } else {
+      col = (int)(t / rows);
+      i = t - (int64_t)col * rows;
+    }
+    if (act && !act[col])
+      continue;
+    // element (r, col) of x / y
+    const int64_t xs = rm ? s : 1, xc = rm ? 1 : ldx, ys = rm ? s : 1, yc = rm ? 1 : ldy;
+    const T *xx = x + (int64_t)col * xc;
+    T out;
+    if (!trans)
+      out = xx[(i + N) * xs] - cc * xx[i * xs];
+    else
+      out = (i >= N ? xx[(i - N) * xs] : (T)0) - (i < m ? cc * xx[i * xs] : (T)0);
+    y[i * ys + (int64_t)col * yc] = out;
+  }
+}
+
+// y = y + (-beta) z with beta on the device (the fused epilogue of the CSR
+// kernels, for operators that cannot fuse it).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sub_beta(int64_t len, const double *__restrict__ beta, const T *__restrict__ z,
+               T *__restrict__ y) {
+#pragma clang fp contract(off)
+  const T b = (T)(*beta);
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock)
+    y[i] = y[i] + (-b) * z[i];
+}
+
+// Y(:, c) = X(:, c) for the columns with act[c] != 0.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_copy_active(int64_t len, int s, const T *__restrict__ X, int64_t ldx, T *__restrict__ Y,
+                  int64_t ldy, const int *__restrict__ act) {
+  const int64_t total = len * (int64_t)s;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int c = (int)(t / len);
+    const int64_t i = t - (int64_t)c * len;
+    if (act[c])
+      Y[i + (int64_t)c * ldy] = X[i + (int64_t)c * ldx];
+  }
+}
+
 // --------------------------------------------- row-major block products --
 // Y = op(A) X for a block of s columns stored row-major (element (r, c) at
 // r * s + c). Thread t handles (row t / s, column t % s): the threads of one
@@ -835,8 +938,9 @@ __device__ inline void g2_tile(int t, int T, int &ti, int &tj, bool &is_s) {
 }
 
 // 16x16x4 MFMA in the element type: FP32 (v_mfma_f32_16x16x4f32) or FP64
-// (v_mfma_f64_16x16x4f64); both take A[i][k] from lane i + 16k and B[k][j]
-// from lane j + 16k and return D[4*(l/16)+v][l%16] in element v of lane l.
+// (v_mfma_f64_16x16x4f64). Both take A[i][k] from lane i + 16k and B[k][j]
+// from lane j + 16k. Element v of lane l holds D[row(l, v)][l % 16] with
+// row = 4*(l/16) + v for FP32 and row = l/16 + 4*v for FP64.
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 template <class T> struct Mfma16;
 template <> struct Mfma16<float> {
@@ -844,12 +948,14 @@ template <> struct Mfma16<float> {
   static __device__ inline V step(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 template <> struct Mfma16<double> {
   using V = doublex4;
   static __device__ inline V step(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 template <class T>
@@ -933,7 +1039,7 @@ __global__ void __launch_bounds__(kG2Threads)
     if (t < ntiles)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        out[t * 256 + (4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[u][v];
+        out[t * 256 + Mfma16<T>::row(lane, v) * 16 + (lane & 15)] = acc[u][v];
   }
 }
 
@@ -1525,6 +1631,77 @@ void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
   const int64_t outs = (int64_t)ty * tz * kMT * kMT;
   hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((outs + kBlock - 1) / kBlock)), dim3(kBlock),
                      0, ctx.stream, part, nblk, ty, s, q, H);
+  KH_CHECK(hipGetLastError());
+}
+
+void sd_apply(Context &ctx, int64_t n, int64_t N, double c, bool trans, Prec p, const void *x,
+              void *y, const void *z, const double *beta) {
+  const int64_t rows = trans ? n : n - N;
+  if (rows <= 0)
+    return;
+  const double e = p == Prec::F64 ? 8.0 : 4.0;
+  TraceScope tr(ctx, kTraceSpmv, e * (double)(n + (n - N)) + (z ? e * rows : 0.0));
+  const dim3 g(grid_for(rows));
+  if (p == Prec::F64) {
+    if (trans)
+      hipLaunchKernelGGL(k_sd_mtv<double>, g, dim3(kBlock), 0, ctx.stream, n, N, c,
+                         (const double *)x, (double *)y, (const double *)z, beta);
+    else
+      hipLaunchKernelGGL(k_sd_mv<double>, g, dim3(kBlock), 0, ctx.stream, n - N, N, c,
+                         (const double *)x, (double *)y, (const double *)z, beta);
+  } else {
+    if (trans)
+      hipLaunchKernelGGL(k_sd_mtv<float>, g, dim3(kBlock), 0, ctx.stream, n, N, c,
+                         (const float *)x, (float *)y, (const float *)z, beta);
+    else
+      hipLaunchKernelGGL(k_sd_mv<float>, g, dim3(kBlock), 0, ctx.stream, n - N, N, c,
+                         (const float *)x, (float *)y, (const float *)z, beta);
+  }
+  KH_CHECK(hipGetLastError());
+}
+
+void sd_block(Context &ctx, int64_t n, int64_t N, double c, bool trans, Prec p, int s,
+              const void *x, int64_t ldx, void *y, int64_t ldy, const int *act, bool row_major) {
+  const int64_t rows = trans ? n : n - N;
+  if (rows <= 0 || s <= 0)
+    return;
+  int64_t g = (rows * (int64_t)s + kBlock - 1) / kBlock;
+  g = std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_sd_block<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, n, N,
+                       c, trans ? 1 : 0, s, (const double *)x, ldx, (double *)y, ldy, act,
+                       row_major ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_sd_block<float>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, n, N,
+                       c, trans ? 1 : 0, s, (const float *)x, ldx, (float *)y, ldy, act,
+                       row_major ? 1 : 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void sub_beta(Context &ctx, Prec p, int64_t len, const double *beta, const void *z, void *y) {
+  if (len <= 0)
+    return;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_sub_beta<double>, dim3(grid_for(len)), dim3(kBlock), 0, ctx.stream, len,
+                       beta, (const double *)z, (double *)y);
+  else
+    hipLaunchKernelGGL(k_sub_beta<float>, dim3(grid_for(len)), dim3(kBlock), 0, ctx.stream, len,
+                       beta, (const float *)z, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void copy_active(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *Y,
+                 int64_t ldy, const int *act) {
+  if (len <= 0 || s <= 0)
+    return;
+  int64_t g = (len * (int64_t)s + kBlock - 1) / kBlock;
+  g = std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_copy_active<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream,
+                       len, s, (const double *)X, ldx, (double *)Y, ldy, act);
+  else
+    hipLaunchKernelGGL(k_copy_active<float>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, len,
+                       s, (const float *)X, ldx, (float *)Y, ldy, act);
   KH_CHECK(hipGetLastError());
 }
 

@@ -1,8 +1,11 @@
 // The quick-start program of the reference (examples/quickstart.cpp) written
 // against the unchanged KraNS API, with the HIP adapters in place of the
-// Std/Eigen ones: the shifted-difference matrix (n=1000, N=3, c=0.5) as a
-// device CSR, a seed-42 factory shared by both solvers, and the same checks
-// (vector count, max ||A u||, distance to the exact kernel).
+// Std/Eigen ones: the shifted-difference operator (n=1000, N=3, c=0.5) as a
+// device CSR, as the matrix-free HipShiftedDifferenceOperator (the
+// reference's StdMatrixAdapter lambdas) and as a HipOperatorAdapter whose
+// callback applies the CSR; a seed-42 factory shared by both solvers, and the
+// same checks (vector count, max ||A u||, distance to the exact kernel). The
+// three operands give the same products, so their bases must agree bitwise.
 // Exit code 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -29,7 +32,19 @@ int main() {
 
   krans::hip::HipContext ctx(0);
   krans::hip::HipCsrMatrixAdapter A(ctx.get(), m, n, rp.data(), col.data(), val.data());
-  krans::hip::HipVectorFactory vf(ctx.get(), n, /*seed=*/42);
+  krans::hip::HipShiftedDifferenceOperator S(ctx.get(), n, N, c);
+  // a user operator: each column through the CSR's own product entry point
+  krans::hip::HipOperatorAdapter F(
+      ctx.get(), m, n,
+      [&](void *, krans::hip::Precision p, bool t, int s, const void *x, int64_t ldx, void *y,
+          int64_t ldy) {
+        const int prec = p == krans::hip::Precision::FP64 ? KH_PREC_FP64 : KH_PREC_MIXED;
+        const std::size_t e = p == krans::hip::Precision::FP64 ? 8 : 4;
+        for (int j = 0; j < s; ++j)
+          krans::hip::check(kh_spmm_dev(A.handle(), t ? 1 : 0, prec, 1,
+                                        static_cast<const char *>(x) + (std::size_t)j * ldx * e,
+                                        static_cast<char *>(y) + (std::size_t)j * ldy * e));
+      });
 
   int failures = 0;
   auto report = [&](const char *name, const auto &U) {
@@ -59,18 +74,37 @@ int main() {
   };
 
   std::printf("solver   vectors   max ||A u||   max dist(u, ker A)\n");
-  {
-    krans::solvers::CyclicLanczosOptions opt;
-    opt.k = N;
-    auto U = krans::solvers::find_nullspace_cyclic_lanczos(A, vf, opt);
-    report("cyclic", U);
+  std::vector<std::vector<double>> first;  // bases of the CSR run
+  const krans::MatrixInterface *ops[3] = {&A, &S, &F};
+  const char *names[3] = {"csr", "stencil", "callback"};
+  for (int o = 0; o < 3; ++o) {
+    krans::hip::HipVectorFactory vf(ctx.get(), n, /*seed=*/42);
+    std::printf("-- operand: %s\n", names[o]);
+    std::vector<std::vector<double>> got;
+    {
+      krans::solvers::CyclicLanczosOptions opt;
+      opt.k = N;
+      auto U = krans::solvers::find_nullspace_cyclic_lanczos(*ops[o], vf, opt);
+      report("cyclic", U);
+      for (const auto &u : U)
+        got.push_back(static_cast<const krans::hip::HipVector &>(*u).to_host());
+    }
+    {
+      krans::solvers::LOBPCGOptions opt;
+      opt.nev = N;
+      auto U = krans::solvers::find_nullspace_lobpcg(*ops[o], vf, opt);
+      report("lobpcg", U);
+      for (const auto &u : U)
+        got.push_back(static_cast<const krans::hip::HipVector &>(*u).to_host());
+    }
+    if (o == 0) {
+      first = got;
+    } else if (got != first) {
+      std::printf("%s: basis differs from the CSR run\n", names[o]);
+      ++failures;
+    }
   }
-  {
-    krans::solvers::LOBPCGOptions opt;
-    opt.nev = N;
-    auto U = krans::solvers::find_nullspace_lobpcg(A, vf, opt);
-    report("lobpcg", U);
-  }
+  krans::hip::HipVectorFactory vf(ctx.get(), n, /*seed=*/42);
   // the reference's error convention: a foreign backend is rejected
   try {
     struct Other final : krans::VectorFactory {
