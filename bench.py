@@ -161,6 +161,19 @@ def main():
     avg_bytes = trace["bytes"] / max(1, trace["launches"])
     avg_ms = trace["ms"] / max(1, trace["launches"])
     peak = 8000.0
+    # per-class breakdown of one more (untimed) mixed solve: every launch of
+    # the reorthogonalisation products, updates and sparse products
+    classes = ["vtw", "vh_sub", "spmv"]
+    vf = K.HipVectorFactory(A.n, 42)
+    H.ctx.trace_begin_classes(classes)
+    K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision="mixed")
+    per_class = {}
+    for k_, (n_l, b_l, ms_l) in H.ctx.trace_classes_end(classes).items():
+        gbs = b_l / (ms_l * 1e-3) / 1e9 if ms_l > 0 else None
+        per_class[k_] = {"launches": n_l, "GBps": None if gbs is None else round(gbs, 1),
+                         "frac": None if gbs is None else round(gbs / 8000.0, 4),
+                         "avg_us": round(1e3 * ms_l / max(1, n_l), 2),
+                         "device_ms": round(ms_l, 1)}
     ms_spmv = H.time_spmv(transpose=False, precision="fp32", reps=200)
     ms_spmtv = H.time_spmv(transpose=True, precision="fp32", reps=200)
     traffic = load_traffic("k_vtw_partial<float, float>")
@@ -209,6 +222,7 @@ def main():
             "traffic": traffic,
             "launches": trace["launches"], "avg_bytes_per_launch": round(avg_bytes),
             "avg_us_per_launch": round(1e3 * avg_ms, 3),
+            "per_class_mixed_solve": per_class,
             "spmv_fp32": {"avg_us": round(1e3 * ms_spmv, 2),
                           "GBps": round(spmv_bytes(A, 4) / (ms_spmv * 1e-3) / 1e9, 1),
                           "spmtv_avg_us": round(1e3 * ms_spmtv, 2),

@@ -244,6 +244,11 @@ int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b,
  * coefficients of both passes, norm_out ||w|| afterwards. */
 int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s,
                 void *w, int64_t len, double *h_out, double *norm_out);
+/* The same with the pass-1 update and pass-2 projection fused into one read
+ * of V (fused != 0, s <= 1024; what kh_cgs2_dev and the Golub-Kahan steps
+ * use) or as four separate sweeps (fused == 0); the results are identical. */
+int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w,
+                   int64_t len, int fused, double *h_out, double *norm_out);
 /* G = Z^T [Z AZ]: the S and G Gram blocks of LOBPCGSolver.cpp:67-80 on
  * FP32 MFMA (precision 1) or FP64 (precision 0); Z, AZ len x k column-major,
  * S and G k x k FP64 out (host). */
@@ -257,6 +262,11 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ,
  * 2 len s q per product) and summed device time in ms. */
 int kh_trace_begin(kh_context *ctx, int kernel_class);
 int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms);
+/* Several classes at once: bit c of `mask` traces class c (1..4);
+ * kh_trace_class reads one class's totals before kh_trace_end clears all. */
+int kh_trace_begin_mask(kh_context *ctx, unsigned mask);
+int kh_trace_class(kh_context *ctx, int kernel_class, int64_t *launches, double *bytes,
+                   double *ms);
 /* Average device time (ms) of `reps` launches of y = op(A) x, measured with
  * HIP events on the library stream. */
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps,

@@ -133,10 +133,13 @@ SIGNATURES = {
     "kh_spmm_rows_dev": (I32, [P, I32, I32, I32, P, P]),
     "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
+    "kh_cgs2_dev_ex": (I32, [P, I32, P, I32, P, I64, I32, P, P]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
     "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),
     "kh_trace_begin": (I32, [P, I32]),
     "kh_trace_end": (I32, [P, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
+    "kh_trace_begin_mask": (I32, [P, C.c_uint]),
+    "kh_trace_class": (I32, [P, I32, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
 }
 
 _lib = None
@@ -210,6 +213,25 @@ class HipContext:
         n, b, ms = I64(), D(), D()
         _check(lib().kh_trace_end(self.h, C.byref(n), C.byref(b), C.byref(ms)))
         return n.value, b.value, ms.value
+
+    def trace_begin_classes(self, classes):
+        """Trace several kernel classes at once (kh_trace_begin_mask)."""
+        mask = 0
+        for k in classes:
+            mask |= 1 << self.TRACE_CLASSES[k]
+        _check(lib().kh_trace_begin_mask(self.h, mask))
+
+    def trace_classes_end(self, classes):
+        """{class: (launches, algorithmic bytes or flops, device ms)}, then
+        clears the trace."""
+        out = {}
+        for k in classes:
+            n, b, ms = I64(), D(), D()
+            _check(lib().kh_trace_class(self.h, self.TRACE_CLASSES[k], C.byref(n), C.byref(b),
+                                        C.byref(ms)))
+            out[k] = (n.value, b.value, ms.value)
+        self.trace_end()
+        return out
 
 
 _default_ctx = None

@@ -54,12 +54,14 @@ def build(verbose=False):
 
 
 EXAMPLES = {"quickstart_hip": os.path.join(os.path.dirname(HERE), "tests", "cxx",
-                                           "quickstart_hip.cpp")}
+                                           "quickstart_hip.cpp"),
+            "krans_nullspace_hip": os.path.join(HERE, "apps", "krans_nullspace_hip.cpp")}
 
 
 def build_examples():
     """C++ programs written against the KraNS API + HIP adapters (plain g++,
-    linked to libkrans_hip.so): the drop-in check of the C++ boundary."""
+    linked to libkrans_hip.so): the quick-start drop-in check of the C++
+    boundary and the krans_nullspace command line on the HIP backend."""
     inc = os.path.join(os.path.dirname(HERE), "include")
     for name, src in EXAMPLES.items():
         if not os.path.exists(src):

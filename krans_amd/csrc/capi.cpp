@@ -623,16 +623,21 @@ int kh_dot_dev(kh_context *ctx, int precision, const void *a, const void *b, int
   });
 }
 
-int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
-                double *h_out, double *norm_out) {
+int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
+                   int fused, double *h_out, double *norm_out) {
   return guard([&] {
     need(ctx && V && w && s >= 0 && s <= 4096, "bad argument");
+    need(!fused || s <= 1024, "fused CGS2 takes at most 1024 columns");
     auto &c = *ctx->ctx;
     const kh::Prec p = prec_of(precision);
     double *h1 = c.dscal + 16, *h2 = h1 + 4096, *nrm = c.dscal + 8;
     kh::gemv_t(c, p, V, len, s, p, w, len, h1);
-    kh::gemv_n_sub(c, p, V, len, s, h1, p, w, len, nullptr);
-    kh::gemv_t(c, p, V, len, s, p, w, len, h2);
+    if (fused && s > 0) {
+      kh::gemv_n_sub_t(c, p, V, len, s, h1, p, w, len, h2);
+    } else {
+      kh::gemv_n_sub(c, p, V, len, s, h1, p, w, len, nullptr);
+      kh::gemv_t(c, p, V, len, s, p, w, len, h2);
+    }
     kh::gemv_n_sub(c, p, V, len, s, h2, p, w, len, nrm);
     std::vector<double> a((std::size_t)s), b((std::size_t)s);
     KH_CHECK(hipMemcpyAsync(a.data(), h1, (std::size_t)s * 8, hipMemcpyDeviceToHost, c.stream));
@@ -646,6 +651,11 @@ int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, i
     if (norm_out)
       *norm_out = nn;
   });
+}
+
+int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
+                double *h_out, double *norm_out) {
+  return kh_cgs2_dev_ex(ctx, precision, V, s, w, len, s <= 1024 ? 1 : 0, h_out, norm_out);
 }
 
 int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, int k,
@@ -680,7 +690,40 @@ int kh_trace_begin(kh_context *ctx, int kernel_class) {
     auto &c = *ctx->ctx;
     c.sync();
     c.trace_reset();
-    c.trace_class = kernel_class;
+    c.trace_mask = kernel_class ? (1u << kernel_class) : 0u;
+  });
+}
+
+int kh_trace_begin_mask(kh_context *ctx, unsigned mask) {
+  return guard([&] {
+    need(ctx && (mask & ~0x1eu) == 0, "bad class mask (bits 1..4)");
+    auto &c = *ctx->ctx;
+    c.sync();
+    c.trace_reset();
+    c.trace_mask = mask;
+  });
+}
+
+int kh_trace_class(kh_context *ctx, int kernel_class, int64_t *launches, double *bytes,
+                   double *ms) {
+  return guard([&] {
+    need(ctx && launches && bytes && ms, "null argument");
+    auto &c = *ctx->ctx;
+    c.sync();
+    double tb = 0.0, tm = 0.0;
+    int64_t nl = 0;
+    for (auto &r : c.trace) {
+      if (r.cls != kernel_class)
+        continue;
+      float e = 0.0f;
+      KH_CHECK(hipEventElapsedTime(&e, r.a, r.b));
+      tb += r.bytes;
+      tm += (double)e;
+      ++nl;
+    }
+    *launches = nl;
+    *bytes = tb;
+    *ms = tm;
   });
 }
 
@@ -699,7 +742,7 @@ int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms) 
     *launches = (int64_t)c.trace.size();
     *bytes = tb;
     *ms = tm;
-    c.trace_class = kh::kTraceNone;
+    c.trace_mask = 0;
     c.trace_reset();
   });
 }

@@ -562,8 +562,17 @@ DeviceBasis CyclicDriver::run() {
         groups.push_back({Xb, ldx_, (int)nxc, kp});
       else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
         groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
-      for (int pass = 0; pass < passes; ++pass)
-        cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+      const Group &g0 = groups.front();
+      if (passes == 2 && g0.count > 0) {
+        // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
+        // fused; w -= V h2 with ||w||. Same arithmetic as two cgs_pass calls.
+        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
+        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
+        gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+      } else {
+        for (int pass = 0; pass < passes; ++pass)
+          cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+      }
       if (passes <= 0)
         dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
       if (nyc + nxc >= ncv)

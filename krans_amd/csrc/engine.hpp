@@ -31,9 +31,10 @@ enum class Prec : int { F64 = 0, F32 = 1 };
 // ---------------------------------------------------------------------------
 // Context: one device, one stream, a slab of device scalars and a pinned
 // host mirror used to read scalars back with one small copy.
-// Optional per-launch timing of one kernel class with HIP events on the
+// Optional per-launch timing of kernel classes with HIP events on the
 // context's stream (for roofline reporting): algorithmic bytes of each traced
-// launch and its event pair.
+// launch, its class and its event pair. trace_mask has bit c set for every
+// traced class c.
 enum TraceClass : int {
   kTraceNone = 0,
   kTraceVtW = 1,
@@ -44,10 +45,11 @@ enum TraceClass : int {
 struct TraceRec {
   double bytes;
   hipEvent_t a, b;
+  int cls;
 };
 
 struct Context {
-  int trace_class = kTraceNone;
+  unsigned trace_mask = 0;
   std::vector<TraceRec> trace;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
@@ -155,6 +157,10 @@ void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                 const double *h, Prec pw, void *w, int64_t len,
                 double *nrm2_out = nullptr);
+// w -= V h, then out = V^T w for the updated w (FP64): the middle of a CGS
+// pair with one read of V; same results as gemv_n_sub followed by gemv_t.
+void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
+                  Prec pw, void *w, int64_t len, double *out);
 // out = V h (overwrite)
 void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
             const double *h, Prec po, void *out, int64_t len);

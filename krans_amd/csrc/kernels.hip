@@ -346,6 +346,111 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The middle of a classical Gram-Schmidt pair in one read of V: w -= V h
+// (the update of pass 1, exactly as k_vh_sub computes it) and then the block
+// partials of V^T w for the updated w (the projection of pass 2, exactly as
+// k_vtw_partial computes it from the stored w). The second column sweep
+// re-reads the rows this thread just read, which are still in cache, so HBM
+// sees V once instead of twice. part layout [c * nb + block].
+template <class TV, class TW>
+__global__ void __launch_bounds__(kBlock)
+    k_vh_sub_vtw(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                 const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
+                 int nb) {
+  constexpr int CU = ColUnroll<TV>::value;
+  extern __shared__ double sh[];  // hs[s], then red[s][kWaves]
+  double *hs = sh;
+  double *red = sh + s;
+  for (int j = threadIdx.x; j < s; j += kBlock)
+    hs[j] = h[j];
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
+  int64_t rr[kR];
+  double acc[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    rr[k] = r < len ? r : len - 1;
+    acc[k] = 0.0;
+  }
+  const int s_main = s - s % CU;
+  // pass-1 update
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    TV x[CU][kR];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const TV *vc = V + (int64_t)(c0 + u) * ld;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        x[u][k] = vc[rr[k]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const double hc = hs[c0 + u];
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        acc[k] += (double)x[u][k] * hc;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const TV *vc = V + (int64_t)c * ld;
+    const double hc = hs[c];
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      acc[k] += (double)vc[rr[k]] * hc;
+  }
+  double wv[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    const TW nw = (TW)((double)w[rr[k]] - acc[k]);
+    if (r < len)
+      w[r] = nw;
+    wv[k] = r < len ? (double)nw : 0.0;
+  }
+  // pass-2 projection of the updated w
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    TV x[CU][kR];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const TV *vc = V + (int64_t)(c0 + u) * ld;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        x[u][k] = vc[rr[k]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      double a = 0.0;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        a += (double)x[u][k] * wv[k];
+      a = wave_sum(a);
+      if (lane == 0)
+        red[(c0 + u) * kWaves + wid] = a;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const TV *vc = V + (int64_t)c * ld;
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      a += (double)vc[rr[k]] * wv[k];
+    a = wave_sum(a);
+    if (lane == 0)
+      red[c * kWaves + wid] = a;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < s; c += kBlock) {
+    double t = 0.0;
+    for (int w2 = 0; w2 < kWaves; ++w2)
+      t += red[c * kWaves + w2];
+    part[(int64_t)c * nb + blockIdx.x] = t;
+  }
+}
+
 // OUT(:,c) = V H(:,c)  (overwrite), c < q.
 template <class TV, class TO>
 __global__ void __launch_bounds__(kBlock)
@@ -1368,7 +1473,9 @@ struct TraceScope {
   bool on;
   hipEvent_t a = nullptr;
   double bytes;
-  TraceScope(Context &ctx, int cls, double b) : c(ctx), on(ctx.trace_class == cls), bytes(b) {
+  int cls;
+  TraceScope(Context &ctx, int cls_, double b)
+      : c(ctx), on(cls_ > 0 && ((ctx.trace_mask >> cls_) & 1u)), bytes(b), cls(cls_) {
     if (on) {
       a = c.take_event();
       KH_CHECK(hipEventRecord(a, c.stream));
@@ -1378,7 +1485,7 @@ struct TraceScope {
     if (on) {
       hipEvent_t b2 = c.take_event();
       (void)hipEventRecord(b2, c.stream);
-      c.trace.push_back({bytes, a, b2});
+      c.trace.push_back({bytes, a, b2, cls});
     }
   }
 };
@@ -1561,6 +1668,39 @@ void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
     launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm2_out);
   else
     launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm2_out);
+}
+
+template <class TV, class TW>
+static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, const double *h,
+                                TW *w, int64_t len, double *out) {
+  const int nb = vtw_blocks(len);
+  double *part = ctx.partials((std::size_t)nb * s);
+  {
+    // V read once from HBM, w read and written, partials written
+    TraceScope tr(ctx, kTraceVhSub,
+                  (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * nb * s);
+    hipLaunchKernelGGL((k_vh_sub_vtw<TV, TW>), dim3(nb), dim3(kBlock),
+                       (std::size_t)s * (1 + kWaves) * sizeof(double), ctx.stream, len, V, ld, s,
+                       h, w, part, nb);
+  }
+  hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
+                  Prec pw, void *w, int64_t len, double *out) {
+  if (s <= 0)
+    return;
+  if (s > 1024)
+    throw std::invalid_argument("gemv_n_sub_t: more than 1024 columns");
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (double *)w, len, out);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (float *)w, len, out);
+  else if (pv == Prec::F64)
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (float *)w, len, out);
+  else
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out);
 }
 
 template <class TV, class TO>

@@ -839,12 +839,18 @@ DeviceBasis LobpcgDriver::run() {
         }
         if (!reject && bp_ == Prec::F32) {
           // FP32 iterate: one FP64 projector pass v <- v - A^T (A A^T)^{-1} A v
-          // (iterative refinement), then the reference's FP64 acceptance test
+          // (iterative refinement), then the reference's FP64 acceptance test.
+          // A v' = A v - A A^T w is the CG residual, so CG only has to reduce
+          // ||A v|| to a tenth of the acceptance threshold; diag(A A^T)
+          // preconditions it when the operator provides it.
           A_.mv(c_, Prec::F64, v, tm64_.p);
           ++S_.spmv;
+          const double av = nrm(Prec::F64, tm64_.p, m_);
+          const double rel =
+              av > 0.0 ? std::min(1e-2, std::max(1e-12, 0.1 * accept_thr / av)) : 1e-12;
           std::vector<CgState> st;
-          cg_solve_ops(c_, A_, cg_, 1, tm64_.p, w64_.p, Prec::F64, nullptr, 1000, 1e-12, 1e-30,
-                       st);
+          cg_solve_ops(c_, A_, cg_, 1, tm64_.p, w64_.p, Prec::F64, A_.diag_AAt(), 1000, rel,
+                       1e-30, st);
           S_.spmv += st[0].k;
           S_.spmtv += st[0].k;
           A_.mtv(c_, Prec::F64, w64_.p, tn64_.p);

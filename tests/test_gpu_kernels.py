@@ -127,3 +127,27 @@ def test_spmm_rows_bit_exact(s, transpose):
     for c in range(s):
         ref = Op.mtv(X[:, c]) if transpose else Op.mv(X[:, c])
         assert np.array_equal(Y[:, c].view(np.uint64), ref.view(np.uint64)), c
+
+
+@pytest.mark.parametrize("s,n,prec", [(1, 5000, 0), (33, 100_003, 0), (97, 300_000, 1),
+                                      (7, 2049, 1)])
+def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
+    """The fused middle kernel (w -= V h1 and V^T w in one read of V) gives
+    bitwise the results of the separate update and projection sweeps."""
+    print(f"[case] cgs2 fused s={s} n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(s + n)
+    V, _ = np.linalg.qr(rng.standard_normal((n, s)))
+    w = rng.standard_normal(n) + V @ rng.standard_normal(s)
+    dt = np.float64 if prec == 0 else np.float32
+    V, w = V.astype(dt), w.astype(dt)
+    outs = []
+    for fused in (0, 1):
+        dV, dw = Dev(ctx, V), Dev(ctx, w)
+        h = np.zeros(s)
+        nrm = np.zeros(1)
+        K._check(K.lib().kh_cgs2_dev_ex(ctx.h, prec, dV.p, s, dw.p, n, fused,
+                                        h.ctypes.data_as(C.c_void_p),
+                                        nrm.ctypes.data_as(C.c_void_p)))
+        outs.append((dw.get(), h, nrm))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
