@@ -231,6 +231,18 @@ int kh_matrix_jacobi_AAt(kh_matrix *A, const double *r, double *y);
 /* y = op(A) x on device vectors, s columns (ld = vector length) */
 int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s,
                 const void *x, void *y);
+/* H = V^T W (s x q, column-major, FP64 on the host) for device blocks with
+ * leading dimension len: the tall-skinny product of the block Gram-Schmidt
+ * steps (FP64 MFMA for s, q <= 128 in FP64; FP64-accumulated tiles
+ * otherwise). */
+int kh_gemm_tn_dev(kh_context *ctx, int precision, const void *V, int s, const void *W,
+                   int q, int64_t len, double *H);
+/* W = V H (subtract == 0) or W -= V H (subtract != 0) for device blocks
+ * with leading dimension len and H (s x q, column-major) on the host: the
+ * candidate formation, combine and block update products (FP64 MFMA for
+ * 2 <= q <= 128 when H^T fits 64 KB of LDS). */
+int kh_gemm_nn_dev(kh_context *ctx, int precision, const void *V, int s, const double *H,
+                   int q, void *W, int64_t len, int subtract);
 /* The same product on row-major blocks (element (r, c) at r * s + c), the
  * layout the LOBPCG normal products use for blocks of 4 or more columns. */
 int kh_spmm_rows_dev(kh_matrix *A, int transpose, int precision, int s,

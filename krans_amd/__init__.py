@@ -131,6 +131,8 @@ SIGNATURES = {
     "kh_matrix_jacobi_AAt": (I32, [P, P, P]),
     "kh_spmm_dev": (I32, [P, I32, I32, I32, P, P]),
     "kh_spmm_rows_dev": (I32, [P, I32, I32, I32, P, P]),
+    "kh_gemm_tn_dev": (I32, [P, I32, P, I32, P, I32, I64, P]),
+    "kh_gemm_nn_dev": (I32, [P, I32, P, I32, P, I32, P, I64, I32]),
     "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
     "kh_cgs2_dev_ex": (I32, [P, I32, P, I32, P, I64, I32, P, P]),

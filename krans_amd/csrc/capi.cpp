@@ -599,6 +599,35 @@ int kh_spmm_dev(kh_matrix *A, int transpose, int precision, int s, const void *x
   });
 }
 
+int kh_gemm_tn_dev(kh_context *ctx, int precision, const void *V, int s, const void *W, int q,
+                   int64_t len, double *H) {
+  return guard([&] {
+    need(ctx && V && W && H && s >= 1 && q >= 1 && len >= 1, "bad argument");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    kh::DBuf d((std::size_t)s * q * 8);
+    kh::gemm_tn(c, p, V, len, s, p, W, len, q, len, d.as<double>());
+    KH_CHECK(hipMemcpyAsync(H, d.p, (std::size_t)s * q * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int kh_gemm_nn_dev(kh_context *ctx, int precision, const void *V, int s, const double *H, int q,
+                   void *W, int64_t len, int subtract) {
+  return guard([&] {
+    need(ctx && V && H && W && s >= 1 && q >= 1 && len >= 1, "bad argument");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    kh::DBuf d((std::size_t)s * q * 8);
+    KH_CHECK(hipMemcpyAsync(d.p, H, (std::size_t)s * q * 8, hipMemcpyHostToDevice, c.stream));
+    if (subtract)
+      kh::gemm_nn_sub(c, p, V, len, s, d.as<double>(), q, p, W, len, len, nullptr);
+    else
+      kh::gemm_nn(c, p, V, len, s, d.as<double>(), q, p, W, len, len);
+    c.sync();
+  });
+}
+
 int kh_spmm_rows_dev(kh_matrix *A, int transpose, int precision, int s, const void *x,
                      void *y) {
   return guard([&] {

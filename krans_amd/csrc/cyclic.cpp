@@ -840,9 +840,24 @@ void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *
       break;
     const int it = (int)std::min<int64_t>(chunk, maxit - iters_done);
     const int *act = B.act.as<int>();
+    // four or more columns: A (A^T p) on row-major copies, one coalesced
+    // q-wide row per gather (inactive columns are computed too and ignored)
+    const bool rows = q >= 4 && A.has_rows();
+    if (rows) {
+      const std::size_t need = (std::size_t)q * (std::size_t)m * (p == Prec::F64 ? 8 : 4);
+      if (B.rows_m.bytes < need)
+        B.rows_m = DBuf(need);
+    }
     for (int k = 0; k < it; ++k) {
-      A.mtm(c, p, q, B.pv.p, m, B.tmp_n.p, n, act);
-      A.mm(c, p, q, B.tmp_n.p, n, B.Ap.p, m, act);
+      if (rows) {
+        cols_to_rows(c, p, m, q, B.pv.p, m, B.rows_m.p);
+        A.mm_rows(c, p, q, B.rows_m.p, B.tmp_n.p, true);
+        A.mm_rows(c, p, q, B.tmp_n.p, B.rows_m.p, false);
+        rows_to_cols(c, p, m, q, B.rows_m.p, B.Ap.p, m);
+      } else {
+        A.mtm(c, p, q, B.pv.p, m, B.tmp_n.p, n, act);
+        A.mm(c, p, q, B.tmp_n.p, n, B.Ap.p, m, act);
+      }
       cg_after_products(c, B, w, diag, maxit, breakdown_tol);
     }
     iters_done += it;
@@ -861,7 +876,7 @@ double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P) {
   b += ke * (m + n) + 8.0 * (2 * n + m + n);           // w_x, w_y, y0, best_v, tmp
   const double w = (double)std::min<int64_t>(64, std::max<int64_t>(8, std::min(ncv, std::max<int64_t>(k, 8))));
   b += w * 8.0 * (5.0 * n + 5.0 * m);                  // candidate windows
-  b += w * 8.0 * (4.0 * m + n);                        // CG state
+  b += w * 8.0 * (5.0 * m + n);                        // CG state (+ row-major block)
   b += (P.polish == Prec::F32 ? w * 8.0 * m : 0.0);
   return b + (double)(64 << 20);                       // scalars, partials, slack
 }

@@ -230,6 +230,7 @@ struct CgBatch {
   int64_t m = 0, n = 0;
   Prec p = Prec::F64;
   DBuf r, z, pv, Ap, tmp_n, st, act;
+  DBuf rows_m;  // row-major m x s block of the row-major product path
   bool have_prec = false;
   void ensure(int s_, int64_t m_, int64_t n_, Prec p_);
 };

@@ -1063,16 +1063,25 @@ template <> struct Mfma16<double> {
   static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
+// mode 0: the Gram pair (s == q). mode 1: H = Z^T AZ for Z with s columns
+// and AZ with q columns (s, q <= 128), all (s/16) x (q/16) tiles, A operand
+// from Z and B operand from AZ (the tall-skinny V^T W of the solvers).
+__device__ inline int g2_ntiles(int mode, int s, int q) {
+  const int ns = (s + 15) / 16, nq = (q + 15) / 16;
+  return mode == 0 ? nq * (nq + 1) / 2 + nq * nq : ns * nq;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kG2Threads)
     k_gram2_partial(int64_t len, int64_t rows_per_block, const T *__restrict__ Z,
-                    int64_t ldz, const T *__restrict__ AZ, int64_t ldaz, int q,
-                    T *__restrict__ part) {
+                    int64_t ldz, int s, const T *__restrict__ AZ, int64_t ldaz, int q,
+                    int mode, T *__restrict__ part) {
   using V4 = typename Mfma16<T>::V;
   __shared__ T Zs[kG2MaxQ * kG2Ld];
   __shared__ T As[kG2MaxQ * kG2Ld];
-  const int nt = (q + 15) / 16, QP = nt * 16;
-  const int ntiles = nt * (nt + 1) / 2 + nt * nt;
+  const int nt = (q + 15) / 16;
+  const int SP = ((s + 15) / 16) * 16, QP = nt * 16;
+  const int ntiles = g2_ntiles(mode, s, q);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
@@ -1084,8 +1093,15 @@ __global__ void __launch_bounds__(kG2Threads)
     const int t = wid + u * kG2Waves;
     int ti = 0, tj = 0;
     bool is_s = true;
-    if (t < ntiles)
-      g2_tile(t, nt, ti, tj, is_s);
+    if (t < ntiles) {
+      if (mode == 0) {
+        g2_tile(t, nt, ti, tj, is_s);
+      } else {
+        ti = t / nt;
+        tj = t % nt;
+        is_s = false;
+      }
+    }
     aoff[u] = (16 * ti + (lane & 15)) * kG2Ld + (lane >> 4);
     boff[u] = (16 * tj + (lane & 15)) * kG2Ld + (lane >> 4);
     bs[u] = is_s;
@@ -1104,7 +1120,7 @@ __global__ void __launch_bounds__(kG2Threads)
       const int e = threadIdx.x + u * kG2Threads;
       const int r = e % kG2Slab, c = e / kG2Slab;
       const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
-      za[u] = c < q ? Z[row + (int64_t)c * ldz] : (T)0;
+      za[u] = c < s ? Z[row + (int64_t)c * ldz] : (T)0;
       aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : (T)0;
     }
   };
@@ -1116,11 +1132,11 @@ __global__ void __launch_bounds__(kG2Threads)
     for (int u = 0; u < kLd; ++u) {
       const int e = threadIdx.x + u * kG2Threads;
       const int r = e % kG2Slab, c = e / kG2Slab;
-      if (c < QP) {
-        const bool ok = (rb + r) < r1 && c < q;
-        Zs[c * kG2Ld + r] = ok ? za[u] : (T)0;
-        As[c * kG2Ld + r] = ok ? aa[u] : (T)0;
-      }
+      const bool inr = (rb + r) < r1;
+      if (c < SP)
+        Zs[c * kG2Ld + r] = (inr && c < s) ? za[u] : (T)0;
+      if (c < QP)
+        As[c * kG2Ld + r] = (inr && c < q) ? aa[u] : (T)0;
     }
     __syncthreads();
     if (rb + kG2Slab < r1)
@@ -1150,6 +1166,24 @@ __global__ void __launch_bounds__(kG2Threads)
 
 // SG[0 : q*q] = S, SG[q*q : 2*q*q] = G, both column-major q x q; S is
 // mirrored from its upper triangle.
+// mode 1: H (s x q, column-major) from the VᵀW tiles.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_tn2_finish(const T *__restrict__ part, int nblk, int s, int q, double *__restrict__ H) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (int64_t)s * q)
+    return;
+  const int i = (int)(e % s), j = (int)(e / s);
+  const int nq = (q + 15) / 16;
+  const int ntiles = ((s + 15) / 16) * nq;
+  const int t = (i / 16) * nq + j / 16;
+  const T *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b)
+    acc += (double)p[(int64_t)b * ntiles * 256];
+  H[e] = acc;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_gram2_finish(const T *__restrict__ part, int nblk, int q, double *__restrict__ SG) {
@@ -1178,6 +1212,74 @@ __global__ void __launch_bounds__(kBlock)
   for (int b = 0; b < nblk; ++b)
     acc += (double)p[(int64_t)b * ntiles * 256];
   SG[e] = acc;
+}
+
+// OUT(:, c) (= or -=) V H(:, c), c < q <= 128, on FP64 MFMA: computed as
+// OUT^T = H^T V^T so that element v of lane l lands in row l % 16 of a
+// 16-row group, i.e. 16 consecutive rows per column on the store. A wave owns
+// 16 rows and all ceil(q/16) column tiles; H^T sits in LDS (column j of the
+// output at j * sld), V is read once (FP32 V converted exactly), FP64
+// accumulation. Blocks stride over 64-row chunks.
+template <class TV, class TO, bool SUB>
+__global__ void __launch_bounds__(kBlock)
+    k_nn_mfma(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
+              const double *__restrict__ H, int q, TO *__restrict__ OUT, int64_t ldo, int sld) {
+  extern __shared__ double Hs[];  // [qp][sld]
+  const int nq = (q + 15) / 16, qp = nq * 16;
+  for (int e = threadIdx.x; e < qp * sld; e += kBlock) {
+    const int j = e / sld, k = e - j * sld;
+    Hs[e] = (j < q && k < s) ? H[k + (int64_t)j * s] : 0.0;
+  }
+  __syncthreads();
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int s4 = (s + 3) & ~3;
+  for (int64_t c0 = (int64_t)blockIdx.x * 64; c0 < len; c0 += (int64_t)gridDim.x * 64) {
+    const int64_t row0 = c0 + 16 * wid;
+    const int64_t rc = (row0 + li) < len ? (row0 + li) : (len - 1);
+    doublex4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < s4; k0 += 16) {
+      double b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u + lk;
+        const int kc = k < s ? k : s - 1;
+        const double x = (double)V[rc + (int64_t)kc * ldv];
+        b[u] = k < s ? x : 0.0;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kb = k0 + 4 * u;
+        if (kb < s4) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (t < nq) {
+              const double a = Hs[(16 * t + li) * sld + kb + lk];
+              acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[u], acc[t], 0, 0, 0);
+            }
+        }
+      }
+    }
+    const int64_t i = row0 + li;
+    if (i < len) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t < nq) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int j = 16 * t + Mfma16<double>::row(lane, v);
+            if (j < q) {
+              TO *o = OUT + i + (int64_t)j * ldo;
+              *o = SUB ? (TO)((double)*o - acc[t][v]) : (TO)acc[t][v];
+            }
+          }
+        }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- level 1 --
@@ -1703,9 +1805,31 @@ void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const
     launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out);
 }
 
+// k_nn_mfma when the block fits it (q <= 128 output columns, H^T in 64 KB
+// of LDS); returns false otherwise.
+template <class TV, class TO, bool SUB>
+static bool try_nn_mfma(Context &ctx, const TV *V, int64_t ldv, int s, const double *H, int q,
+                        TO *out, int64_t ldo, int64_t len) {
+  if (q < 2 || q > 128 || s < 1 || len <= 0)
+    return false;
+  const int sld = ((s + 3) & ~3) + 1;
+  const int qp = ((q + 15) / 16) * 16;
+  const std::size_t lds = (std::size_t)qp * sld * sizeof(double);
+  if (lds > 65536)
+    return false;
+  const int64_t chunks = (len + 63) / 64;
+  const unsigned grid = (unsigned)std::min<int64_t>(chunks, 4096);
+  hipLaunchKernelGGL((k_nn_mfma<TV, TO, SUB>), dim3(grid), dim3(kBlock), lds, ctx.stream, len, V,
+                     ldv, s, H, q, out, ldo, sld);
+  KH_CHECK(hipGetLastError());
+  return true;
+}
+
 template <class TV, class TO>
 static void launch_gemm_nn(Context &ctx, const TV *V, int64_t ldv, int s, const double *H,
                            int q, TO *out, int64_t ldo, int64_t len) {
+  if (try_nn_mfma<TV, TO, false>(ctx, V, ldv, s, H, q, out, ldo, len))
+    return;
   if (q == 1) {
     dim3 grid(grid_for(len, 4), 1);
     hipLaunchKernelGGL((k_gemm_nn<TV, TO>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
@@ -1895,15 +2019,16 @@ void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void 
   KH_CHECK(hipGetLastError());
 }
 
+// mode 0: SG = [Z^T Z | Z^T AZ] (s == q); mode 1: SG = Z^T AZ (s x q).
 template <class T>
-static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, const T *AZ, int64_t ldaz, int q,
-                         int64_t len, double *SG) {
-  if (q <= 0 || len <= 0)
+static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *AZ,
+                         int64_t ldaz, int q, int64_t len, double *SG, int mode) {
+  if (q <= 0 || s <= 0 || len <= 0)
     return;
-  if (q > kG2MaxQ)
-    throw std::invalid_argument("gram2_mfma: q above the fused-kernel limit");
-  const int nt = (q + 15) / 16;
-  const int ntiles = nt * (nt + 1) / 2 + nt * nt;
+  if (q > kG2MaxQ || s > kG2MaxQ)
+    throw std::invalid_argument("gram2_mfma: more than 128 columns");
+  const int ns = (s + 15) / 16, nq = (q + 15) / 16;
+  const int ntiles = mode == 0 ? nq * (nq + 1) / 2 + nq * nq : ns * nq;
   // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
   // unless the partials would not fit the context's reduction buffer
   const int64_t maxblk = std::max<int64_t>(
@@ -1916,25 +2041,31 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, const T *AZ, int
   const std::size_t need = (std::size_t)nblk64 * ntiles * 256;  // elements
   T *part = reinterpret_cast<T *>(ctx.partials((need * sizeof(T) + 7) / 8));
   {
-    TraceScope tr(ctx, sizeof(T) == 4 ? kTraceGram : kTraceNone,
-                  2.0 * (double)len * q * q * 2.0);
+    TraceScope tr(ctx, (mode == 0 && sizeof(T) == 4) ? kTraceGram : kTraceNone,
+                  2.0 * (double)len * s * q * (mode == 0 ? 2.0 : 1.0));
     hipLaunchKernelGGL(k_gram2_partial<T>, dim3((unsigned)nblk64), dim3(kG2Threads), 0,
-                       ctx.stream, len, rpb, Z, ldz, AZ, ldaz, q, part);
+                       ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, mode, part);
   }
-  const int64_t outs = 2 * (int64_t)q * q;
-  hipLaunchKernelGGL(k_gram2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, ctx.stream, part, (int)nblk64, q, SG);
+  if (mode == 0) {
+    const int64_t outs = 2 * (int64_t)q * q;
+    hipLaunchKernelGGL(k_gram2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, ctx.stream, part, (int)nblk64, q, SG);
+  } else {
+    const int64_t outs = (int64_t)s * q;
+    hipLaunchKernelGGL(k_tn2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, ctx.stream, part, (int)nblk64, s, q, SG);
+  }
   KH_CHECK(hipGetLastError());
 }
 
 void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
                 int64_t len, double *SG) {
-  launch_gram2(ctx, Z, ldz, AZ, ldaz, q, len, SG);
+  launch_gram2(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
 }
 
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
                 int q, int64_t len, double *SG) {
-  launch_gram2(ctx, Z, ldz, AZ, ldaz, q, len, SG);
+  launch_gram2(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
 }
 
 void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
@@ -1948,7 +2079,9 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
              len, H + (std::size_t)c * s);
     return;
   }
-  if (pv == Prec::F64 && pw == Prec::F64)
+  if (pv == Prec::F64 && pw == Prec::F64 && s <= kG2MaxQ && q <= kG2MaxQ)
+    launch_gram2(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H, 1);
+  else if (pv == Prec::F64 && pw == Prec::F64)
     launch_gemm_tn(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H);
   else if (pv == Prec::F32 && pw == Prec::F32)
     launch_gemm_tn(ctx, (const float *)V, ldv, s, (const float *)W, ldw, q, len, H);
@@ -1968,10 +2101,12 @@ static void launch_gemm_nn_sub(Context &ctx, const TV *V, int64_t ldv, int s,
                         norms ? norms + c : nullptr);
     return;
   }
-  dim3 grid(grid_for(len, 2), (q + kQT - 1) / kQT);
-  hipLaunchKernelGGL((k_ts_nn<TV, TW, true>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
-                     s, H, q, W, ldw);
-  KH_CHECK(hipGetLastError());
+  if (!try_nn_mfma<TV, TW, true>(ctx, V, ldv, s, H, q, W, ldw, len)) {
+    dim3 grid(grid_for(len, 2), (q + kQT - 1) / kQT);
+    hipLaunchKernelGGL((k_ts_nn<TV, TW, true>), grid, dim3(kBlock), 0, ctx.stream, len, V, ldv,
+                       s, H, q, W, ldw);
+    KH_CHECK(hipGetLastError());
+  }
   if (norms) {
     const int nb = reduce_blocks(len);
     double *part = ctx.partials((std::size_t)nb * q);

@@ -166,6 +166,7 @@ private:
   std::vector<double> hbuf_;
   Blk tmpq_;  // m x q scratch for apply_normal
   DBuf zr_, tr_;  // row-major n x q and m x q blocks of the row-major path
+  DBuf vblk_, rhsblk_, wblk_, tnblk_;  // FP64 candidate refinement blocks
   DBuf hproj_;  // locked^T X coefficients of a block projection
 
   void random_col(void *dst, Prec p);
@@ -734,6 +735,12 @@ DeviceBasis LobpcgDriver::run() {
   const double cand_thr =
       bp_ == Prec::F32 ? std::max(accept_thr, 1e-5 * std::max(1.0, S_.frob_norm_est)) : accept_thr;
   w64_ = DBuf((std::size_t)m_ * 8);
+  if (bp_ == Prec::F32) {
+    vblk_ = DBuf((std::size_t)B * n_ * 8);
+    tnblk_ = DBuf((std::size_t)B * n_ * 8);
+    rhsblk_ = DBuf((std::size_t)B * m_ * 8);
+    wblk_ = DBuf((std::size_t)B * m_ * 8);
+  }
 
   Blk X, AX, Rb, W, Pb, Z, AZ, scratch, Xn;
   X.init(n_, B, bp_);
@@ -813,81 +820,104 @@ DeviceBasis LobpcgDriver::run() {
     rb(c_, sc_ + 200, B, avo.data());
     int64_t nconv = 0;
     double best_unacc = std::numeric_limits<double>::infinity();
-    for (int j = 0; j < B; ++j) {
-      const double Av_orig = avo[(std::size_t)j];
-      if (Av_orig <= cand_thr) {
-        double *v = vc_.as<double>();
-        to64(X.col(j), bp_, v);
-        project_out(v, Prec::F64);
-        const double nv = nrm(Prec::F64, v, n_);
-        bool reject = false;
-        if (nv <= std::max(1e-14, 10.0 * P_.gs_tol)) {
-          reject = true;
-        } else {
-          axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nv, v);
-          if (locked_.count > 0) {
-            gemv_t(c_, Prec::F64, locked_.col(0), n_, (int)locked_.count, Prec::F64, v, n_,
-                   sc_ + 300);
-            std::vector<double> dd((std::size_t)locked_.count);
-            rb(c_, sc_ + 300, (int)locked_.count, dd.data());
-            for (double x : dd)
-              if (std::fabs(x) >= 1.0 - 1e-8) {
-                reject = true;
-                break;
-              }
-          }
-        }
-        if (!reject && bp_ == Prec::F32) {
-          // FP32 iterate: one FP64 projector pass v <- v - A^T (A A^T)^{-1} A v
-          // (iterative refinement), then the reference's FP64 acceptance test.
-          // A v' = A v - A A^T w is the CG residual, so CG only has to reduce
-          // ||A v|| to a tenth of the acceptance threshold; diag(A A^T)
-          // preconditions it when the operator provides it.
-          A_.mv(c_, Prec::F64, v, tm64_.p);
-          ++S_.spmv;
-          const double av = nrm(Prec::F64, tm64_.p, m_);
-          const double rel =
-              av > 0.0 ? std::min(1e-2, std::max(1e-12, 0.1 * accept_thr / av)) : 1e-12;
-          std::vector<CgState> st;
-          cg_solve_ops(c_, A_, cg_, 1, tm64_.p, w64_.p, Prec::F64, A_.diag_AAt(), 1000, rel,
-                       1e-30, st);
-          S_.spmv += st[0].k;
-          S_.spmtv += st[0].k;
-          A_.mtv(c_, Prec::F64, w64_.p, tn64_.p);
-          ++S_.spmtv;
-          axpby(c_, Prec::F64, n_, -1.0, tn64_.p, 1.0, v);
-          project_out(v, Prec::F64);
-          const double nr = nrm(Prec::F64, v, n_);
-          if (!(nr > 1e-14))
-            reject = true;
-          else
-            axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nr, v);
-        }
-        double Av_acc = std::numeric_limits<double>::infinity();
-        if (!reject) {
-          A_.mv(c_, Prec::F64, v, tm64_.p);
-          ++S_.spmv;
-          Av_acc = nrm(Prec::F64, tm64_.p, m_);
-          if (Av_acc > accept_thr)
-            reject = true;
-        }
-        if (!reject) {
-          dcopy(locked_.col(locked_.count), v, (std::size_t)n_ * 8);
-          locked_.count += 1;
-          S_.theta.push_back(theta[(std::size_t)j]);
-          S_.residuals_Av.push_back(Av_acc);
-          ++locked_total;
-          ++nconv;
-          if (P_.verbose)
-            std::fprintf(stderr, "[LOBPCG-HIP] accept: ||A v||=%.3e locked=%lld/%lld\n", Av_acc,
-                        (long long)locked_total, (long long)k_target);
-          if (locked_total >= k_target)
-            break;
-        } else {
-          best_unacc = std::min(best_unacc, Av_orig);
-        }
-      } else {
+    // the reference's acceptance of one FP64 vector v (:560-605): residual
+    // against the threshold, then lock
+    auto try_accept = [&](double *v, int j, double Av_orig) {
+      A_.mv(c_, Prec::F64, v, tm64_.p);
+      ++S_.spmv;
+      const double Av_acc = nrm(Prec::F64, tm64_.p, m_);
+      if (Av_acc > accept_thr) {
         best_unacc = std::min(best_unacc, Av_orig);
+        return;
+      }
+      dcopy(locked_.col(locked_.count), v, (std::size_t)n_ * 8);
+      locked_.count += 1;
+      S_.theta.push_back(theta[(std::size_t)j]);
+      S_.residuals_Av.push_back(Av_acc);
+      ++locked_total;
+      ++nconv;
+      if (P_.verbose)
+        std::fprintf(stderr, "[LOBPCG-HIP] accept: ||A v||=%.3e locked=%lld/%lld\n", Av_acc,
+                     (long long)locked_total, (long long)k_target);
+    };
+    // v = x_j in FP64, projected against the locked vectors and normalised;
+    // false if it is (numerically) in their span
+    auto prepare = [&](double *v, const void *xj) {
+      if (xj)
+        to64(xj, bp_, v);
+      project_out(v, Prec::F64);
+      const double nv = nrm(Prec::F64, v, n_);
+      if (!(nv > std::max(1e-14, 10.0 * P_.gs_tol)))
+        return false;
+      axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nv, v);
+      if (locked_.count > 0) {
+        gemv_t(c_, Prec::F64, locked_.col(0), n_, (int)locked_.count, Prec::F64, v, n_,
+               sc_ + 300);
+        std::vector<double> dd((std::size_t)locked_.count);
+        rb(c_, sc_ + 300, (int)locked_.count, dd.data());
+        for (double x : dd)
+          if (std::fabs(x) >= 1.0 - 1e-8)
+            return false;
+      }
+      return true;
+    };
+    if (bp_ == Prec::F64) {
+      for (int j = 0; j < B; ++j) {
+        const double Av_orig = avo[(std::size_t)j];
+        double *v = vc_.as<double>();
+        if (Av_orig <= cand_thr && prepare(v, X.col(j)))
+          try_accept(v, j, Av_orig);
+        else
+          best_unacc = std::min(best_unacc, Av_orig);
+        if (locked_total >= k_target)
+          break;
+      }
+    } else {
+      // FP32 iterates become candidates at cand_thr and are refined in FP64
+      // by one projector pass v <- v - A^T (A A^T)^{-1} A v (iterative
+      // refinement) before the reference's acceptance test. The candidates of
+      // an iteration are refined together: one multi-column CG (row-major
+      // products, one gather per nonzero for all columns), diag(A A^T)
+      // preconditioned when the operator provides it. A v' is the CG
+      // residual, so CG only has to bring ||A v|| to a tenth of the threshold.
+      std::vector<int> cand;
+      for (int j = 0; j < B; ++j) {
+        const double Av_orig = avo[(std::size_t)j];
+        if (Av_orig <= cand_thr &&
+            prepare(vblk_.as<double>() + (std::size_t)cand.size() * n_, X.col(j)))
+          cand.push_back(j);
+        else
+          best_unacc = std::min(best_unacc, Av_orig);
+      }
+      const int nc = (int)cand.size();
+      if (nc > 0) {
+        double *Vb64 = vblk_.as<double>();
+        A_.mm(c_, Prec::F64, nc, Vb64, n_, rhsblk_.p, m_, nullptr);
+        S_.spmv += nc;
+        col_norms(c_, Prec::F64, rhsblk_.p, m_, nc, m_, sc_ + 200);
+        std::vector<double> av((std::size_t)nc);
+        rb(c_, sc_ + 200, nc, av.data());
+        double rel = 1e-2;
+        for (double a : av)
+          rel = std::min(rel, a > 0.0 ? std::max(1e-12, 0.1 * accept_thr / a) : 1e-12);
+        std::vector<CgState> st;
+        cg_solve_ops(c_, A_, cg_, nc, rhsblk_.p, wblk_.p, Prec::F64, A_.diag_AAt(), 1000, rel,
+                     1e-30, st);
+        for (const auto &x : st) {
+          S_.spmv += x.k;
+          S_.spmtv += x.k;
+        }
+        A_.mtm(c_, Prec::F64, nc, wblk_.p, m_, tnblk_.p, n_, nullptr);
+        S_.spmtv += nc;
+        sub_cols(c_, Prec::F64, n_, nc, Vb64, n_, tnblk_.p, n_, Vb64, n_, nullptr);
+        for (int t = 0; t < nc && locked_total < k_target; ++t) {
+          const int j = cand[(std::size_t)t];
+          double *v = Vb64 + (std::size_t)t * n_;
+          if (prepare(v, nullptr))
+            try_accept(v, j, avo[(std::size_t)j]);
+          else
+            best_unacc = std::min(best_unacc, avo[(std::size_t)j]);
+        }
       }
     }
     S_.locked = locked_total;

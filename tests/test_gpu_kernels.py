@@ -151,3 +151,50 @@ def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
         outs.append((dw.get(), h, nrm))
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (64, 64, 300_001, 0), (97, 33, 70_000, 0),
+                                        (128, 128, 40_000, 0), (130, 9, 5000, 0),
+                                        (40, 24, 100_000, 1)])
+def test_gemm_tn_blocks(ctx, s, q, n, prec):
+    """V^T W of the block Gram-Schmidt steps: FP64 MFMA tiles for s, q <=
+    128 (FP64), FP64-accumulated tiles otherwise; within 1e-13 of numpy."""
+    print(f"[case] gemm_tn s={s} q={q} n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(s * 1000 + q)
+    V = rng.standard_normal((n, s))
+    W = rng.standard_normal((n, q))
+    if prec == 1:
+        V, W = V.astype(np.float32), W.astype(np.float32)
+    dV, dW = Dev(ctx, V), Dev(ctx, W)
+    H = np.zeros((s, q), order="F")
+    K._check(K.lib().kh_gemm_tn_dev(ctx.h, prec, dV.p, s, dW.p, q, n,
+                                    H.ctypes.data_as(C.c_void_p)))
+    V64, W64 = V.astype(np.float64), W.astype(np.float64)
+    ref = V64.T @ W64
+    scale = np.sqrt(np.outer(np.sum(V64 * V64, 0), np.sum(W64 * W64, 0)))
+    assert np.all(np.abs(H - ref) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("s,q,n,prec,sub", [(5, 7, 1000, 0, 1), (64, 64, 300_001, 0, 1),
+                                            (97, 33, 70_000, 0, 0), (64, 128, 20_000, 0, 1),
+                                            (3, 1, 5000, 0, 1), (200, 40, 9000, 0, 1),
+                                            (40, 24, 100_000, 1, 0), (96, 32, 50_000, 1, 1)])
+def test_gemm_nn_blocks(ctx, s, q, n, prec, sub):
+    """W = V H / W -= V H (FP64 MFMA for 2 <= q <= 128, the tile kernel
+    otherwise); FP64 within 1e-13 of numpy, FP32 outputs within FP32 rounding."""
+    print(f"[case] gemm_nn s={s} q={q} n={n} prec={prec} sub={sub}", flush=True)
+    rng = np.random.default_rng(s * 7 + q)
+    V = rng.standard_normal((n, s))
+    H = np.asfortranarray(rng.standard_normal((s, q)))
+    W = rng.standard_normal((n, q))
+    if prec == 1:
+        V, W = V.astype(np.float32), W.astype(np.float32)
+    dV, dW = Dev(ctx, V), Dev(ctx, W)
+    K._check(K.lib().kh_gemm_nn_dev(ctx.h, prec, dV.p, s, H.ctypes.data_as(C.c_void_p), q,
+                                    dW.p, n, sub))
+    got = dW.get().astype(np.float64)
+    VH = V.astype(np.float64) @ H
+    ref = W.astype(np.float64) - VH if sub else VH
+    scale = np.abs(V.astype(np.float64)) @ np.abs(H) + (np.abs(W) if sub else 0)
+    tol = 1e-13 if prec == 0 else 2 * np.finfo(np.float32).eps
+    assert np.all(np.abs(got - ref) <= tol * scale + 1e-300)
