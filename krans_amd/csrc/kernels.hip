@@ -78,7 +78,18 @@ __global__ void __launch_bounds__(kBlock)
        i += (int64_t)gridDim.x * kBlock) {
     const int32_t p0 = ptr[i], p1 = ptr[i + 1];
     A acc = (A)0;
-    for (int32_t p = p0; p < p1; ++p)
+    int32_t p = p0;
+    // four gathers in flight per lane; the sum keeps ascending order
+    for (; p + 4 <= p1; p += 4) {
+      const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
+      const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+      const T x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
+      acc += (A)v0 * (A)x0;
+      acc += (A)v1 * (A)x1;
+      acc += (A)v2 * (A)x2;
+      acc += (A)v3 * (A)x3;
+    }
+    for (; p < p1; ++p)
       acc += (A)val[p] * (A)x[idx[p]];
     T out = (T)acc;
     if (z)
@@ -134,7 +145,17 @@ __global__ void __launch_bounds__(kBlock)
         continue;
       const T *xc = x + (int64_t)c * ldx;
       A acc = (A)0;
-      for (int32_t p = p0; p < p1; ++p)
+      int32_t p = p0;
+      for (; p + 4 <= p1; p += 4) {  // four gathers in flight, ascending sum
+        const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
+        const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+        const T x0 = xc[j0], x1 = xc[j1], x2 = xc[j2], x3 = xc[j3];
+        acc += (A)v0 * (A)x0;
+        acc += (A)v1 * (A)x1;
+        acc += (A)v2 * (A)x2;
+        acc += (A)v3 * (A)x3;
+      }
+      for (; p < p1; ++p)
         acc += (A)val[p] * (A)xc[idx[p]];
       y[i + (int64_t)c * ldy] = (T)acc;
     }
@@ -863,7 +884,18 @@ __global__ void __launch_bounds__(kBlock)
     const int c = (int)(t - i * s);
     const int32_t p0 = ptr[i], p1 = ptr[i + 1];
     A acc = (A)0;
-    for (int32_t p = p0; p < p1; ++p)
+    int32_t p = p0;
+    for (; p + 4 <= p1; p += 4) {  // four gathers in flight, ascending sum
+      const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
+      const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+      const T x0 = X[(int64_t)j0 * s + c], x1 = X[(int64_t)j1 * s + c];
+      const T x2 = X[(int64_t)j2 * s + c], x3 = X[(int64_t)j3 * s + c];
+      acc += (A)v0 * (A)x0;
+      acc += (A)v1 * (A)x1;
+      acc += (A)v2 * (A)x2;
+      acc += (A)v3 * (A)x3;
+    }
+    for (; p < p1; ++p)
       acc += (A)val[p] * (A)X[(int64_t)idx[p] * s + c];
     Y[t] = (T)acc;
   }

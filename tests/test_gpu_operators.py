@@ -91,6 +91,18 @@ def test_callback_operator_same_as_csr(solver, precision):
     r1 = run(H, K.HipVectorFactory(A.n, 9), opt, precision)
     r2 = run(F, K.HipVectorFactory(A.n, 9), opt, precision)
     assert r1.right_null.shape[1] == 4
+    if (solver, precision) == ("lobpcg", "mixed"):
+        # the FP64 refinement of FP32 LOBPCG candidates is preconditioned with
+        # diag(A A^T) when the operand provides it (a CSR matrix does, a
+        # callback operator does not), so the two runs differ in rounding:
+        # both must still meet the threshold and span the same null space
+        thr = r1.accept_threshold
+        S = A.to_scipy()
+        for r in (r1, r2):
+            assert r.right_null.shape[1] == 4
+            assert np.linalg.norm(S @ r.right_null, axis=0).max() <= thr
+        assert principal_sine(r1.right_null, r2.right_null) <= 1e-9
+        return
     assert np.array_equal(r1.right_null.view(np.uint64), r2.right_null.view(np.uint64))
 
 
