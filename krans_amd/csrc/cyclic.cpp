@@ -840,9 +840,9 @@ void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *
       break;
     const int it = (int)std::min<int64_t>(chunk, maxit - iters_done);
     const int *act = B.act.as<int>();
-    // four or more columns: A (A^T p) on row-major copies, one coalesced
+    // two or more columns: A (A^T p) on row-major copies, one coalesced
     // q-wide row per gather (inactive columns are computed too and ignored)
-    const bool rows = q >= 4 && A.has_rows();
+    const bool rows = q >= 2 && A.has_rows();
     if (rows) {
       const std::size_t need = (std::size_t)q * (std::size_t)m * (p == Prec::F64 ? 8 : 4);
       if (B.rows_m.bytes < need)

@@ -176,6 +176,9 @@ private:
   bool orthonormalize(Blk &X, int i0, int cnt, double tol, bool reorth, Blk &scratch);
   void mgs_dev(Blk &X, int cnt, bool reorth, double *dn);
   void apply_normal(const Blk &Z, int q, Blk &AZ);
+  // Y = op(A) X for q FP64 columns (ld = input / output length), through
+  // row-major copies for two or more columns
+  void block_apply(int q, const void *X, void *Y, bool trans);
   void gram(const Blk &Z, const Blk &AZ, int q, dense::Mat &S, dense::Mat &G);
   void combine(const Blk &Z, int z0, int q, const dense::Mat &Y, int row0, int p,
                Blk &out, int o0);
@@ -357,6 +360,25 @@ void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
   S_.spmv += q;
   S_.spmtv += q;
   S_.t_sparse += since(t0);
+}
+
+void LobpcgDriver::block_apply(int q, const void *X, void *Y, bool trans) {
+  const int64_t nin = trans ? m_ : n_, nout = trans ? n_ : m_;
+  if (q >= 2 && A_.has_rows()) {
+    const std::size_t bin = (std::size_t)nin * q * 8, bout = (std::size_t)nout * q * 8;
+    if (zr_.bytes < bin)
+      zr_ = DBuf(bin);
+    if (tr_.bytes < bout)
+      tr_ = DBuf(bout);
+    cols_to_rows(c_, Prec::F64, nin, q, X, nin, zr_.p);
+    A_.mm_rows(c_, Prec::F64, q, zr_.p, tr_.p, trans);
+    rows_to_cols(c_, Prec::F64, nout, q, tr_.p, Y, nout);
+    return;
+  }
+  if (trans)
+    A_.mtm(c_, Prec::F64, q, X, nin, Y, nout, nullptr);
+  else
+    A_.mm(c_, Prec::F64, q, X, nin, Y, nout, nullptr);
 }
 
 // :67-80  S = Z^T Z, G = Z^T AZ (full q x q, symmetric to rounding)
@@ -892,14 +914,17 @@ DeviceBasis LobpcgDriver::run() {
       const int nc = (int)cand.size();
       if (nc > 0) {
         double *Vb64 = vblk_.as<double>();
-        A_.mm(c_, Prec::F64, nc, Vb64, n_, rhsblk_.p, m_, nullptr);
+        block_apply(nc, Vb64, rhsblk_.p, false);
         S_.spmv += nc;
         col_norms(c_, Prec::F64, rhsblk_.p, m_, nc, m_, sc_ + 200);
         std::vector<double> av((std::size_t)nc);
         rb(c_, sc_ + 200, nc, av.data());
+        // CG target: a tenth of the final-polish target (:808-811), so the
+        // reference's block polish finds nothing left to do for these vectors
+        const double polish_target = std::max(1e-14, std::min(1e-10, 0.01 * accept_thr));
         double rel = 1e-2;
         for (double a : av)
-          rel = std::min(rel, a > 0.0 ? std::max(1e-12, 0.1 * accept_thr / a) : 1e-12);
+          rel = std::min(rel, a > 0.0 ? std::max(1e-12, 0.1 * polish_target / a) : 1e-12);
         std::vector<CgState> st;
         cg_solve_ops(c_, A_, cg_, nc, rhsblk_.p, wblk_.p, Prec::F64, A_.diag_AAt(), 1000, rel,
                      1e-30, st);
@@ -907,7 +932,7 @@ DeviceBasis LobpcgDriver::run() {
           S_.spmv += x.k;
           S_.spmtv += x.k;
         }
-        A_.mtm(c_, Prec::F64, nc, wblk_.p, m_, tnblk_.p, n_, nullptr);
+        block_apply(nc, wblk_.p, tnblk_.p, true);
         S_.spmtv += nc;
         sub_cols(c_, Prec::F64, n_, nc, Vb64, n_, tnblk_.p, n_, Vb64, n_, nullptr);
         for (int t = 0; t < nc && locked_total < k_target; ++t) {
