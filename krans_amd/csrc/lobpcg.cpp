@@ -919,12 +919,14 @@ DeviceBasis LobpcgDriver::run() {
         col_norms(c_, Prec::F64, rhsblk_.p, m_, nc, m_, sc_ + 200);
         std::vector<double> av((std::size_t)nc);
         rb(c_, sc_ + 200, nc, av.data());
-        // CG target: a tenth of the final-polish target (:808-811), so the
-        // reference's block polish finds nothing left to do for these vectors
-        const double polish_target = std::max(1e-14, std::min(1e-10, 0.01 * accept_thr));
+        // CG target: a tenth of the acceptance threshold. A tighter target
+        // (say the final polish's 0.01 thr) would also resolve singular
+        // directions just above the threshold when the null space is only
+        // numerical (mat3 of the reference's data: sigma ~ 6.5e-10), and the
+        // projection would then remove the candidate altogether.
         double rel = 1e-2;
         for (double a : av)
-          rel = std::min(rel, a > 0.0 ? std::max(1e-12, 0.1 * polish_target / a) : 1e-12);
+          rel = std::min(rel, a > 0.0 ? std::max(1e-12, 0.1 * accept_thr / a) : 1e-12);
         std::vector<CgState> st;
         cg_solve_ops(c_, A_, cg_, nc, rhsblk_.p, wblk_.p, Prec::F64, A_.diag_AAt(), 1000, rel,
                      1e-30, st);
