@@ -142,6 +142,7 @@ private:
                 int64_t maxit, double rel_tol, std::vector<CgState> &st);
   void upload_mask(const std::vector<int> &mask);
   DBuf mask_;
+  DBuf rin_, rout_;  // row-major copies of block_product
   std::vector<int> slot_of_;
 };
 
@@ -372,7 +373,7 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
         o.hit = true;
     }
     // v' = v - A^T w; GS vs basis; normalise; ||A v'||
-    A_.mtm(c_, Prec::F64, qa, Wm64, m_, Tn, n_, nullptr);
+    block_product(c_, A_, Prec::F64, qa, Wm64, m_, Tn, n_, true, nullptr, rin_, rout_);
     S_.spmtv += qa;
     sub_cols(c_, Prec::F64, n_, qa, Vw, n_, Tn, n_, Vp, n_, nullptr);
     gs_basis_block(Vp, qa);
@@ -391,7 +392,7 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
     upload_doubles(c_, sc.data(), qa, wsc_ + 1024);
     scale_cols(c_, Prec::F64, n_, qa, Vp, n_, wsc_ + 1024, true);
     upload_mask(ok);
-    A_.mm(c_, Prec::F64, qa, Vp, n_, AVp, m_, mask_.as<int>());
+    block_product(c_, A_, Prec::F64, qa, Vp, n_, AVp, m_, false, mask_.as<int>(), rin_, rout_);
     for (int a = 0; a < qa; ++a)
       S_.spmv += ok[(std::size_t)a];
     col_norms(c_, Prec::F64, AVp, m_, qa, m_, wsc_ + 2048);
@@ -702,7 +703,7 @@ DeviceBasis CyclicDriver::run() {
       upload_doubles(c_, sc.data(), wn, wsc_ + 1024);
       scale_cols(c_, Prec::F64, n_, wn, Wv, n_, wsc_ + 1024, true);
       upload_mask(valid);
-      A_.mm(c_, Prec::F64, wn, Wv, n_, AV, m_, mask_.as<int>());
+      block_product(c_, A_, Prec::F64, wn, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
       col_norms(c_, Prec::F64, AV, m_, wn, m_, wsc_ + 2048);
       std::vector<double> rr(wn);
       readback(c_, wsc_ + 2048, wn, rr.data());
@@ -820,6 +821,28 @@ DeviceBasis CyclicDriver::run() {
 
 }  // namespace
 
+void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, int64_t ldx,
+                   void *Y, int64_t ldy, bool trans, const int *mask, DBuf &rin, DBuf &rout) {
+  if (q <= 0)
+    return;
+  const int64_t nin = trans ? A.rows() : A.cols(), nout = trans ? A.cols() : A.rows();
+  if (q >= 2 && A.has_rows()) {
+    const std::size_t e = p == Prec::F64 ? 8 : 4;
+    if (rin.bytes < (std::size_t)nin * q * e)
+      rin = DBuf((std::size_t)nin * q * e);
+    if (rout.bytes < (std::size_t)nout * q * e)
+      rout = DBuf((std::size_t)nout * q * e);
+    cols_to_rows(c, p, nin, q, X, ldx, rin.p);
+    A.mm_rows(c, p, q, rin.p, rout.p, trans);
+    rows_to_cols(c, p, nout, q, rout.p, Y, ldy);
+    return;
+  }
+  if (trans)
+    A.mtm(c, p, q, X, ldx, Y, ldy, mask);
+  else
+    A.mm(c, p, q, X, ldx, Y, ldy, mask);
+}
+
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
                   Prec p, const double *diag, int64_t maxit, double rel_tol,
                   double breakdown_tol, std::vector<CgState> &st) {
@@ -877,6 +900,7 @@ double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P) {
   const double w = (double)std::min<int64_t>(64, std::max<int64_t>(8, std::min(ncv, std::max<int64_t>(k, 8))));
   b += w * 8.0 * (5.0 * n + 5.0 * m);                  // candidate windows
   b += w * 8.0 * (5.0 * m + n);                        // CG state (+ row-major block)
+  b += w * 8.0 * (m + n);                              // row-major block products
   b += (P.polish == Prec::F32 ? w * 8.0 * m : 0.0);
   return b + (double)(64 << 20);                       // scalars, partials, slack
 }

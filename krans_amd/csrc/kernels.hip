@@ -1197,12 +1197,15 @@ __global__ void __launch_bounds__(kG2Threads)
 }
 
 // SG[0 : q*q] = S, SG[q*q : 2*q*q] = G, both column-major q x q; S is
-// mirrored from its upper triangle.
-// mode 1: H (s x q, column-major) from the VᵀW tiles.
+// mirrored from its upper triangle. One wave per output, as k_tn2_finish.
+// mode 1: H (s x q, column-major) from the VᵀW tiles. One wave per output:
+// lane l sums the partials of blocks l, l+64, ... in order, then a fixed
+// shuffle tree (deterministic for a given block count).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_tn2_finish(const T *__restrict__ part, int nblk, int s, int q, double *__restrict__ H) {
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (e >= (int64_t)s * q)
     return;
   const int i = (int)(e % s), j = (int)(e / s);
@@ -1211,15 +1214,18 @@ __global__ void __launch_bounds__(kBlock)
   const int t = (i / 16) * nq + j / 16;
   const T *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
   double acc = 0.0;
-  for (int b = 0; b < nblk; ++b)
+  for (int b = lane; b < nblk; b += 64)
     acc += (double)p[(int64_t)b * ntiles * 256];
-  H[e] = acc;
+  acc = wave_sum(acc);
+  if (lane == 0)
+    H[e] = acc;
 }
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_gram2_finish(const T *__restrict__ part, int nblk, int q, double *__restrict__ SG) {
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   const int64_t qq = (int64_t)q * q;
   if (e >= 2 * qq)
     return;
@@ -1241,9 +1247,11 @@ __global__ void __launch_bounds__(kBlock)
     t = nt * (nt + 1) / 2 + ti * nt + tj;
   const T *p = part + (int64_t)t * 256 + (i % 16) * 16 + (j % 16);
   double acc = 0.0;
-  for (int b = 0; b < nblk; ++b)
+  for (int b = lane; b < nblk; b += 64)
     acc += (double)p[(int64_t)b * ntiles * 256];
-  SG[e] = acc;
+  acc = wave_sum(acc);
+  if (lane == 0)
+    SG[e] = acc;
 }
 
 // OUT(:, c) (= or -=) V H(:, c), c < q <= 128, on FP64 MFMA: computed as
@@ -2079,11 +2087,11 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
                        ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, mode, part);
   }
   if (mode == 0) {
-    const int64_t outs = 2 * (int64_t)q * q;
+    const int64_t outs = 2 * (int64_t)q * q * 64;  // one wave per output
     hipLaunchKernelGGL(k_gram2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, ctx.stream, part, (int)nblk64, q, SG);
   } else {
-    const int64_t outs = (int64_t)s * q;
+    const int64_t outs = (int64_t)s * q * 64;  // one wave per output
     hipLaunchKernelGGL(k_tn2_finish<T>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, ctx.stream, part, (int)nblk64, s, q, SG);
   }

@@ -364,21 +364,7 @@ void LobpcgDriver::apply_normal(const Blk &Z, int q, Blk &AZ) {
 
 void LobpcgDriver::block_apply(int q, const void *X, void *Y, bool trans) {
   const int64_t nin = trans ? m_ : n_, nout = trans ? n_ : m_;
-  if (q >= 2 && A_.has_rows()) {
-    const std::size_t bin = (std::size_t)nin * q * 8, bout = (std::size_t)nout * q * 8;
-    if (zr_.bytes < bin)
-      zr_ = DBuf(bin);
-    if (tr_.bytes < bout)
-      tr_ = DBuf(bout);
-    cols_to_rows(c_, Prec::F64, nin, q, X, nin, zr_.p);
-    A_.mm_rows(c_, Prec::F64, q, zr_.p, tr_.p, trans);
-    rows_to_cols(c_, Prec::F64, nout, q, tr_.p, Y, nout);
-    return;
-  }
-  if (trans)
-    A_.mtm(c_, Prec::F64, q, X, nin, Y, nout, nullptr);
-  else
-    A_.mm(c_, Prec::F64, q, X, nin, Y, nout, nullptr);
+  block_product(c_, A_, Prec::F64, q, X, nin, Y, nout, trans, nullptr, zr_, tr_);
 }
 
 // :67-80  S = Z^T Z, G = Z^T AZ (full q x q, symmetric to rounding)

@@ -142,6 +142,13 @@ struct CyclicStats {
 // rows), each column the recurrence of cg_normal_eq_prec
 // (CyclicLanczosSolver.cpp:48-113); the host polls the device-resident
 // states once per chunk of iterations. Returns the per-column states.
+// Y = op(A) X on q columns (column-major, leading dimensions ldx / ldy; op =
+// A or A^T). With two or more columns and row-major products available the
+// block goes through row-major copies (rin / rout grown as needed); every
+// column is then computed, so columns that `mask` marks inactive may be
+// overwritten. Otherwise op's mm / mtm with the mask.
+void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, int64_t ldx,
+                   void *Y, int64_t ldy, bool trans, const int *mask, DBuf &rin, DBuf &rout);
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
                   Prec p, const double *diag, int64_t maxit, double rel_tol,
                   double breakdown_tol, std::vector<CgState> &st);
