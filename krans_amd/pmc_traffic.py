@@ -3,7 +3,11 @@ WRITE_SIZE, both in KiB per dispatch), each scaled by what the same counter
 reads for the plain copy kernel of known size that bench.py launches first
 (k_axpby<double> copying 256 MiB: 256 MiB read, 256 MiB written).
 
-    python -m krans_amd.pmc_traffic <fetch.db> <write.db> <kernel-substring>
+    python -m krans_amd.pmc_traffic <fetch.db> <write.db> <kernel-substring> [--largest]
+
+With --largest only the dispatches whose FETCH_SIZE is at least half the
+largest one count (one launch shape out of several, e.g. the 96-column Gram
+pass among the 32-column ones).
 
 Prints JSON: per-launch algorithmic bytes are not known here; the caller
 compares the HBM bytes with its own algorithmic count.
@@ -31,14 +35,27 @@ def copy_factor(db, counter):
     return COPY_BYTES / (kib * 1024.0), kib
 
 
-def main(fetch_db, write_db, like):
+def main(fetch_db, write_db, like, largest=False):
     ff, fk = copy_factor(fetch_db, "FETCH_SIZE")
     wf, wk = copy_factor(write_db, "WRITE_SIZE")
     f = per_dispatch(fetch_db, "FETCH_SIZE", like)
     w = per_dispatch(write_db, "WRITE_SIZE", like)
+    if largest and f:
+        top = max(r[2] for r in f)
+        keep = {r[0] for r in f if r[2] >= 0.5 * top}
+        f = [r for r in f if r[0] in keep]
+        # the write run dispatches carry the same ids when the program repeats
+        # itself exactly; fall back to the largest writes otherwise
+        w2 = [r for r in w if r[0] in keep]
+        if w2:
+            w = w2
+        elif w:
+            wt = max(r[2] for r in w)
+            w = [r for r in w if r[2] >= 0.5 * wt]
     fetch = sum(r[2] for r in f) * 1024.0 * ff / max(1, len(f))
     write = sum(r[2] for r in w) * 1024.0 * wf / max(1, len(w))
-    out = {"kernel_like": like, "dispatches_fetch_run": len(f), "dispatches_write_run": len(w),
+    out = {"kernel_like": like, "largest_only": largest,
+           "dispatches_fetch_run": len(f), "dispatches_write_run": len(w),
            "copy_probe_fetch_KiB": fk, "copy_probe_write_KiB": wk,
            "fetch_scale": ff, "write_scale": wf,
            "hbm_read_bytes_per_launch": fetch, "hbm_write_bytes_per_launch": write,
@@ -47,4 +64,4 @@ def main(fetch_db, write_db, like):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1], sys.argv[2], sys.argv[3])
+    main(sys.argv[1], sys.argv[2], sys.argv[3], "--largest" in sys.argv[4:])
