@@ -186,6 +186,7 @@ private:
   double estimate_fro();
   bool orthonormalize_vs(Blk &B, int k, int nd, Blk &scratch);
   void block_rr_polish(double target, int max_sweeps);
+  void projector_polish(double target);
   bool two_vec_sharpen(int idx, double target, int max_iters);
   void mgs_locked(double tol);
 };
@@ -530,6 +531,62 @@ bool LobpcgDriver::orthonormalize_vs(Blk &B, int k, int nd, Blk &scratch) {
   KH_CHECK(hipMemcpyAsync(B.col(k), scratch.col(0), (std::size_t)nd * n_ * 8,
                           hipMemcpyDeviceToDevice, c_.stream));
   return true;
+}
+
+// Mixed precision only, ahead of the reference's polish: one FP64 projector
+// pass v <- v - A^T (A A^T)^{-1} A v on all locked vectors at once (one
+// multi-column CG, row-major products), aimed at a tenth of the polish target.
+// A vector is replaced only if its projection keeps most of its norm (an
+// exact null vector moves by about its residual; a vector of a numerical null
+// space, whose singular values sit above the target, would be projected
+// away) and its residual drops. The reference's block and 2x2 polish then
+// find nothing to do for the vectors that reached the target.
+void LobpcgDriver::projector_polish(double target) {
+  const int k = (int)locked_.count;
+  if (k == 0 || k > 1000)  // norms live in the device scalar slab
+    return;
+  double *nrm_d = sc_ + 60000, *inv_d = sc_ + 62000;
+  const std::size_t nm = (std::size_t)m_ * k * 8, nn = (std::size_t)n_ * k * 8;
+  DBuf AV(nm), Wk(nm), T(nn), Vn(nn);
+  block_apply(k, locked_.col(0), AV.p, false);
+  S_.spmv += k;
+  col_norms(c_, Prec::F64, AV.p, m_, k, m_, nrm_d);
+  std::vector<double> r0((std::size_t)k);
+  rb(c_, nrm_d, k, r0.data());
+  double rel = 1e-2;
+  bool any = false;
+  for (double a : r0)
+    if (a > target) {
+      any = true;
+      rel = std::min(rel, std::max(1e-12, 0.1 * target / a));
+    }
+  if (!any)
+    return;
+  std::vector<CgState> st;
+  cg_solve_ops(c_, A_, cg_, k, AV.p, Wk.p, Prec::F64, A_.diag_AAt(), 1000, rel, 1e-30, st);
+  for (const auto &x : st) {
+    S_.spmv += x.k;
+    S_.spmtv += x.k;
+  }
+  block_apply(k, Wk.p, T.p, true);
+  S_.spmtv += k;
+  sub_cols(c_, Prec::F64, n_, k, locked_.col(0), n_, T.p, n_, Vn.p, n_, nullptr);
+  col_norms(c_, Prec::F64, Vn.p, n_, k, n_, nrm_d);
+  std::vector<double> nv((std::size_t)k), inv((std::size_t)k);
+  rb(c_, nrm_d, k, nv.data());
+  for (int j = 0; j < k; ++j)
+    inv[(std::size_t)j] = nv[(std::size_t)j] > 0.5 ? 1.0 / nv[(std::size_t)j] : 0.0;
+  up(c_, inv.data(), k, inv_d);
+  scale_cols(c_, Prec::F64, n_, k, Vn.p, n_, inv_d, false);
+  block_apply(k, Vn.p, AV.p, false);
+  S_.spmv += k;
+  col_norms(c_, Prec::F64, AV.p, m_, k, m_, nrm_d);
+  std::vector<double> r1((std::size_t)k);
+  rb(c_, nrm_d, k, r1.data());
+  for (int j = 0; j < k; ++j)
+    if (nv[(std::size_t)j] > 0.5 && r1[(std::size_t)j] < r0[(std::size_t)j])
+      KH_CHECK(hipMemcpyAsync(locked_.col(j), static_cast<double *>(Vn.p) + (std::size_t)j * n_,
+                              (std::size_t)n_ * 8, hipMemcpyDeviceToDevice, c_.stream));
 }
 
 // :286-440 on the locked FP64 vectors
@@ -1035,6 +1092,8 @@ DeviceBasis LobpcgDriver::run() {
   if (locked_.count > 0) {  // final polish (:806-831)
     const double desired = accept_thr;
     const double target = std::max(1e-14, std::min(1e-10, 0.01 * desired));
+    if (bp_ == Prec::F32)
+      projector_polish(target);
     block_rr_polish(target, 80);
     for (int i = 0; i < (int)locked_.count; ++i) {
       A_.mv(c_, Prec::F64, locked_.col(i), tm64_.p);
