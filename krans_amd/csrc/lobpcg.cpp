@@ -199,6 +199,8 @@ void LobpcgDriver::random_col(void *dst, Prec p) {
   convert(c_, Prec::F64, v64_.p, p, dst, n_);
   c_.sync();
   S_.t_host += since(t0);
+  S_.t_rng += since(t0);
+  ++S_.random_cols;
 }
 
 double LobpcgDriver::nrm(Prec p, const void *x, int64_t len) {
@@ -1076,6 +1078,7 @@ DeviceBasis LobpcgDriver::run() {
       ++S_.p_drops;
     }
     S_.t_host += since(th);
+    S_.t_gep += since(th);
     const int take = std::min<int>(na, (int)sol.theta.size());
     // P_new = [W P] Y[na:, :take]; X_new = X Y[:na, :take] + P_new (:718-759)
     combine(Z, na, na + np_, sol.Y, na, take, Xn, 0);
@@ -1089,6 +1092,12 @@ DeviceBasis LobpcgDriver::run() {
     full_rr();
   }
   S_.converged_to_target = (S_.locked >= k_target);
+  if (P_.verbose)
+    std::fprintf(stderr,
+                 "[LOBPCG-HIP] done: it=%lld locked=%lld random columns=%lld (%.3f s) "
+                 "tiny GEPs %.3f s host total %.3f s P drops %lld\n",
+                 (long long)its, (long long)S_.locked, (long long)S_.random_cols, S_.t_rng,
+                 S_.t_gep, S_.t_host, (long long)S_.p_drops);
   if (locked_.count > 0) {  // final polish (:806-831)
     const double desired = accept_thr;
     const double target = std::max(1e-14, std::min(1e-10, 0.01 * desired));

@@ -1,16 +1,65 @@
-// Prefetching implementation of the reference-compatible random stream.
+// Prefetching implementation of the reference-compatible random stream: the
+// values of std::normal_distribution<double>(0, 1) driven by
+// std::mt19937(seed), as libstdc++ produces them, bit for bit.
+//
+// libstdc++'s normal_distribution (bits/random.tcc) is the Marsaglia polar
+// method: each attempt draws two uniforms u = generate_canonical<double, 53>
+// (two 32-bit outputs w0, w1: (w0 + w1 * 2^32) / 2^64, clamped below 1),
+// forms x = 2u1 - 1, y = 2u2 - 1, r2 = x^2 + y^2, rejects r2 > 1 or r2 == 0,
+// and otherwise returns y * m and then x * m with m = sqrt(-2 log(r2) / r2)
+// (each result then goes through * stddev + mean, i.e. * 1.0 + 0.0). Every
+// attempt consumes exactly four generator outputs, so attempt c reads outputs
+// 4c..4c+3 whether it is accepted or not. The producer thread therefore draws
+// the raw outputs sequentially (cheap) and converts a batch of attempts on
+// worker threads, each keeping its accepted pairs in order; concatenating the
+// workers' outputs in order reproduces the sequential stream exactly. The
+// expressions below are those of libstdc++ term by term, without contraction.
 #include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <thread>
 
 #include "solver.hpp"
 
 namespace kh {
 
 namespace {
-constexpr std::size_t kChunk = std::size_t(1) << 15;
+constexpr std::size_t kPairs = std::size_t(1) << 18;  // attempts per batch
+
+inline double canonical53(uint32_t w0, uint32_t w1) {
+#pragma clang fp contract(off)
+  double sum = 0.0, tmp = 1.0;
+  sum += (double)w0 * tmp;
+  tmp *= 4294967296.0;
+  sum += (double)w1 * tmp;
+  tmp *= 4294967296.0;
+  double r = sum / tmp;
+  if (r >= 1.0)
+    r = std::nextafter(1.0, 0.0);
+  return r;
+}
+
+// Attempts [c0, c1) of the batch `w` (4 outputs each) into `out`, in order.
+void convert(const uint32_t *w, std::size_t c0, std::size_t c1, std::vector<double> &out) {
+#pragma clang fp contract(off)
+  out.clear();
+  for (std::size_t c = c0; c < c1; ++c) {
+    const double x = 2.0 * canonical53(w[4 * c], w[4 * c + 1]) - 1.0;
+    const double y = 2.0 * canonical53(w[4 * c + 2], w[4 * c + 3]) - 1.0;
+    const double r2 = x * x + y * y;
+    if (r2 > 1.0 || r2 == 0.0)
+      continue;
+    const double mult = std::sqrt(-2 * std::log(r2) / r2);
+    const double a = y * mult, b = x * mult;
+    out.push_back(a * 1.0 + 0.0);
+    out.push_back(b * 1.0 + 0.0);
+  }
+}
 }  // namespace
 
 RefStream::RefStream(unsigned seed, std::size_t prefetch)
-    : gen_(seed), dist_(0.0, 1.0), ring_(std::max(prefetch, kChunk)), cap_(ring_.size()) {
+    : gen_(seed), dist_(0.0, 1.0), ring_(std::max<std::size_t>(prefetch, 4 * kPairs)),
+      cap_(ring_.size()) {
   th_ = std::thread([this] { produce(); });
 }
 
@@ -24,29 +73,58 @@ RefStream::~RefStream() {
 }
 
 void RefStream::produce() {
-  std::vector<double> tmp(kChunk);
-  while (true) {
-    std::size_t room;
-    {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const std::size_t nt = std::min<std::size_t>(8, std::max<std::size_t>(1, hw / 2));
+  // double buffering: the generator outputs of batch b+1 are drawn on this
+  // thread while the workers convert batch b
+  std::vector<uint32_t> words[2] = {std::vector<uint32_t>(4 * kPairs),
+                                    std::vector<uint32_t>(4 * kPairs)};
+  std::vector<std::vector<double>> outs(nt);
+  auto draw = [this](std::vector<uint32_t> &w) {
+    for (auto &x : w)
+      x = (uint32_t)gen_();
+  };
+  auto push = [this](const std::vector<double> &o) {
+    std::size_t done = 0;
+    while (done < o.size()) {
       std::unique_lock<std::mutex> lk(mu_);
       cv_.wait(lk, [&] { return stop_ || count_ < cap_; });
       if (stop_)
-        return;
-      room = std::min(kChunk, cap_ - count_);
-    }
-    // generate outside the lock: gen_/dist_ belong to this thread only
-    for (std::size_t i = 0; i < room; ++i)
-      tmp[i] = dist_(gen_);
-    {
-      std::lock_guard<std::mutex> lk(mu_);
+        return false;
+      const std::size_t room = std::min(cap_ - count_, o.size() - done);
       std::size_t tail = (head_ + count_) % cap_;
       for (std::size_t i = 0; i < room; ++i) {
-        ring_[tail] = tmp[i];
+        ring_[tail] = o[done + i];
         tail = (tail + 1 == cap_) ? 0 : tail + 1;
       }
       count_ += room;
+      done += room;
+      lk.unlock();
+      cv_.notify_all();
     }
-    cv_.notify_all();
+    return true;
+  };
+  int cur = 0;
+  draw(words[cur]);
+  while (true) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (stop_)
+        return;
+    }
+    const uint32_t *w = words[cur].data();
+    std::vector<std::thread> ths;
+    for (std::size_t t = 0; t < nt; ++t) {
+      const std::size_t c0 = kPairs * t / nt, c1 = kPairs * (t + 1) / nt;
+      ths.emplace_back([&outs, w, t, c0, c1] { convert(w, c0, c1, outs[t]); });
+    }
+    draw(words[cur ^ 1]);
+    for (auto &t : ths)
+      t.join();
+    for (const auto &o : outs)
+      if (!push(o))
+        return;
+    cur ^= 1;
   }
 }
 

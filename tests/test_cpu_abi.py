@@ -41,3 +41,16 @@ def test_rng_stream_matches_oracle_without_gpu():
     g = O.Factory(42)
     a = np.concatenate([f.create_random_of_size(7), f.create_random_of_size(9)])
     assert np.array_equal(a, g.randn(16))
+
+
+def test_rng_stream_long_and_ragged_matches_oracle():
+    """The prefetching stream converts polar-method attempts in parallel
+    batches of 2^18; draws of ragged sizes that cross several batch and ring
+    boundaries must still be libstdc++'s sequential stream bit for bit."""
+    import numpy as np
+    import oracle as O
+    for seed in (1, 42, 2**31 + 5):
+        f = K.HipVectorFactory(10, seed)
+        a = np.concatenate([f.create_random_of_size(s) for s in (1, 3, 262_143, 777, 1_500_001)])
+        b = O.Factory(seed).randn(a.size)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), seed
