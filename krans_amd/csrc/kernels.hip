@@ -54,6 +54,14 @@ template <class T> struct Acc;
 template <> struct Acc<double> { using type = double; };
 template <> struct Acc<float> { using type = float; };
 
+// one thread per row, no cap (kernels that do not stride over the grid)
+inline unsigned grid_rows(int64_t rows) {
+  const int64_t g = (rows + kBlock - 1) / kBlock;
+  if (g > ((int64_t)1 << 31) - 1)
+    throw std::length_error("more rows than one launch covers");
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
 inline int grid_for(int64_t len, int per_thread = 1) {
   int64_t g = (len + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
   if (g < 1)
@@ -64,7 +72,44 @@ inline int grid_for(int64_t len, int per_thread = 1) {
 }
 
 // ----------------------------------------------------------------- sparse --
-// One lane per row: sequential ascending sum, no contraction.
+// sum of val[p] * x(idx[p]) over p in [p0, p1), ascending from 0 with no
+// contraction (the rounding of a sequential loop). Entries go in groups of
+// four, then pairs, then a last single one; the index, value and x loads of
+// a group are issued before its sums, so rows of two or four entries cost
+// three dependent memory latencies (row pointers, entries, x) instead of two
+// per entry, and no gather is issued for an entry that does not exist (a
+// gather costs up to one cache line per lane).
+template <class T, class XAt>
+__device__ __forceinline__ typename Acc<T>::type row_dot(int32_t p0, int32_t p1,
+                                                         const int32_t *__restrict__ idx,
+                                                         const T *__restrict__ val, XAt xat) {
+#pragma clang fp contract(off)
+  using A = typename Acc<T>::type;
+  A acc = (A)0;
+  int32_t p = p0;
+  for (; p + 4 <= p1; p += 4) {
+    const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
+    const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+    const T x0 = xat(j0), x1 = xat(j1), x2 = xat(j2), x3 = xat(j3);
+    acc += (A)v0 * (A)x0;
+    acc += (A)v1 * (A)x1;
+    acc += (A)v2 * (A)x2;
+    acc += (A)v3 * (A)x3;
+  }
+  for (; p + 2 <= p1; p += 2) {
+    const int32_t j0 = idx[p], j1 = idx[p + 1];
+    const T v0 = val[p], v1 = val[p + 1];
+    const T x0 = xat(j0), x1 = xat(j1);
+    acc += (A)v0 * (A)x0;
+    acc += (A)v1 * (A)x1;
+  }
+  if (p < p1)
+    acc += (A)val[p] * (A)xat(idx[p]);
+  return acc;
+}
+
+// One lane per row, one row per thread (the grid covers all rows, so every
+// row's chain of loads is in flight at once up to the occupancy limit).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_spmv_lane(int64_t rows, const int32_t *__restrict__ ptr,
@@ -72,30 +117,15 @@ __global__ void __launch_bounds__(kBlock)
                 const T *__restrict__ x, T *__restrict__ y,
                 const T *__restrict__ z, const double *__restrict__ beta) {
 #pragma clang fp contract(off)
-  using A = typename Acc<T>::type;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rows)
+    return;
   const T b = z ? (T)(*beta) : (T)0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < rows;
-       i += (int64_t)gridDim.x * kBlock) {
-    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
-    A acc = (A)0;
-    int32_t p = p0;
-    // four gathers in flight per lane; the sum keeps ascending order
-    for (; p + 4 <= p1; p += 4) {
-      const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
-      const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-      const T x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
-      acc += (A)v0 * (A)x0;
-      acc += (A)v1 * (A)x1;
-      acc += (A)v2 * (A)x2;
-      acc += (A)v3 * (A)x3;
-    }
-    for (; p < p1; ++p)
-      acc += (A)val[p] * (A)x[idx[p]];
-    T out = (T)acc;
-    if (z)
-      out = out + (-b) * z[i];
-    y[i] = out;
-  }
+  const T zi = z ? z[i] : (T)0;
+  T out = (T)row_dot<T>(ptr[i], ptr[i + 1], idx, val, [=](int32_t j) { return x[j]; });
+  if (z)
+    out = out + (-b) * zi;
+  y[i] = out;
 }
 
 // G lanes per row (G in 4,16,64): strided partial sums, shuffle tree.
@@ -128,37 +158,23 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Multi-RHS gather, one lane per row, columns c < s with active[c] != 0.
+// Multi-RHS gather, one lane per row (one row per thread), columns c < s
+// with active[c] != 0.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_spmm_lane(int64_t rows, const int32_t *__restrict__ ptr,
                 const int32_t *__restrict__ idx, const T *__restrict__ val,
                 int s, const T *__restrict__ x, int64_t ldx, T *__restrict__ y,
                 int64_t ldy, const int *__restrict__ active) {
-#pragma clang fp contract(off)
-  using A = typename Acc<T>::type;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < rows;
-       i += (int64_t)gridDim.x * kBlock) {
-    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
-    for (int c = 0; c < s; ++c) {
-      if (active && !active[c])
-        continue;
-      const T *xc = x + (int64_t)c * ldx;
-      A acc = (A)0;
-      int32_t p = p0;
-      for (; p + 4 <= p1; p += 4) {  // four gathers in flight, ascending sum
-        const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
-        const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-        const T x0 = xc[j0], x1 = xc[j1], x2 = xc[j2], x3 = xc[j3];
-        acc += (A)v0 * (A)x0;
-        acc += (A)v1 * (A)x1;
-        acc += (A)v2 * (A)x2;
-        acc += (A)v3 * (A)x3;
-      }
-      for (; p < p1; ++p)
-        acc += (A)val[p] * (A)xc[idx[p]];
-      y[i + (int64_t)c * ldy] = (T)acc;
-    }
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rows)
+    return;
+  const int32_t p0 = ptr[i], p1 = ptr[i + 1];
+  for (int c = 0; c < s; ++c) {
+    if (active && !active[c])
+      continue;
+    const T *xc = x + (int64_t)c * ldx;
+    y[i + (int64_t)c * ldy] = (T)row_dot<T>(p0, p1, idx, val, [=](int32_t j) { return xc[j]; });
   }
 }
 
@@ -875,29 +891,14 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_spmm_rows(int64_t rows, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
                 const T *__restrict__ val, int s, const T *__restrict__ X, T *__restrict__ Y) {
-#pragma clang fp contract(off)
-  using A = typename Acc<T>::type;
   const int64_t total = rows * (int64_t)s;
   for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kBlock) {
     const int64_t i = t / s;
     const int c = (int)(t - i * s);
-    const int32_t p0 = ptr[i], p1 = ptr[i + 1];
-    A acc = (A)0;
-    int32_t p = p0;
-    for (; p + 4 <= p1; p += 4) {  // four gathers in flight, ascending sum
-      const int32_t j0 = idx[p], j1 = idx[p + 1], j2 = idx[p + 2], j3 = idx[p + 3];
-      const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-      const T x0 = X[(int64_t)j0 * s + c], x1 = X[(int64_t)j1 * s + c];
-      const T x2 = X[(int64_t)j2 * s + c], x3 = X[(int64_t)j3 * s + c];
-      acc += (A)v0 * (A)x0;
-      acc += (A)v1 * (A)x1;
-      acc += (A)v2 * (A)x2;
-      acc += (A)v3 * (A)x3;
-    }
-    for (; p < p1; ++p)
-      acc += (A)val[p] * (A)X[(int64_t)idx[p] * s + c];
-    Y[t] = (T)acc;
+    const T *xc = X + c;
+    Y[t] = (T)row_dot<T>(ptr[i], ptr[i + 1], idx, val,
+                         [=](int32_t j) { return xc[(int64_t)j * s]; });
   }
 }
 
@@ -1650,7 +1651,7 @@ static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
     return;
   switch (lpr) {
   case 1:
-    hipLaunchKernelGGL(k_spmv_lane<T>, dim3(grid_for(rows)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_spmv_lane<T>, dim3(grid_rows(rows)), dim3(kBlock), 0, s,
                        rows, ptr, idx, val, x, y, z, beta);
     break;
   case 4:
@@ -1696,7 +1697,7 @@ static void launch_spmm(Context &ctx, int64_t rows, const int32_t *ptr,
   if (rows <= 0 || s <= 0)
     return;
   if (lpr == 1)
-    hipLaunchKernelGGL(k_spmm_lane<T>, dim3(grid_for(rows)), dim3(kBlock), 0, st, rows,
+    hipLaunchKernelGGL(k_spmm_lane<T>, dim3(grid_rows(rows)), dim3(kBlock), 0, st, rows,
                        ptr, idx, val, s, x, ldx, y, ldy, act);
   else if (lpr == 4)
     hipLaunchKernelGGL((k_spmm_group<T, 4>), dim3(grid_for(rows * 4)), dim3(kBlock), 0,
@@ -2021,7 +2022,7 @@ void spmm_rows(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s, cons
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
   const int64_t total = rows * (int64_t)s;
   int64_t g = (total + kBlock - 1) / kBlock;
-  g = std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+  g = std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)1 << 30));
   if (p == Prec::F64)
     hipLaunchKernelGGL(k_spmm_rows<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, rows,
                        ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(), s,
