@@ -902,49 +902,49 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Column-major (ld) <-> row-major (stride s) block transposes through LDS,
-// 64 x 64 tiles: both global sides are read and written in 256-byte runs.
+// Column-major (ld) <-> row-major (stride s) block transposes through LDS.
+// A block owns 64 rows and a group of up to 64 columns (blockIdx.y): on the
+// row-major side its rows form one contiguous run when s <= 64 (runs of 64
+// elements otherwise), read or written flat; on the column-major side each
+// column's 64 rows are one run. Tile stride 65 keeps the column-wise LDS
+// accesses off a single bank.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_cols_to_rows(int64_t len, int s, const T *__restrict__ X, int64_t ldx, T *__restrict__ R) {
-  __shared__ T tile[64][65];
+  __shared__ T tile[64 * 65];
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int c0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
-  for (int cc = ty; cc < 64; cc += 4) {
-    const int c = c0 + cc;
-    const int64_t r = r0 + tx;
-    if (c < s && r < len)
-      tile[cc][tx] = X[r + (int64_t)c * ldx];
+  const int cw = s - c0 < 64 ? s - c0 : 64;
+  const int rows = len - r0 < 64 ? (int)(len - r0) : 64;
+  for (int e = threadIdx.x; e < cw * 64; e += kBlock) {
+    const int c = e >> 6, r = e & 63;
+    if (r < rows)
+      tile[r * 65 + c] = X[r0 + r + (int64_t)(c0 + c) * ldx];
   }
   __syncthreads();
-  for (int rr = ty; rr < 64; rr += 4) {
-    const int64_t r = r0 + rr;
-    const int c = c0 + tx;
-    if (c < s && r < len)
-      R[r * s + c] = tile[tx][rr];
+  for (int e = threadIdx.x; e < rows * cw; e += kBlock) {
+    const int r = e / cw, c = e - r * cw;
+    R[(r0 + r) * s + c0 + c] = tile[r * 65 + c];
   }
 }
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_rows_to_cols(int64_t len, int s, const T *__restrict__ R, T *__restrict__ X, int64_t ldx) {
-  __shared__ T tile[64][65];
+  __shared__ T tile[64 * 65];
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int c0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int rr = ty; rr < 64; rr += 4) {
-    const int64_t r = r0 + rr;
-    const int c = c0 + tx;
-    if (c < s && r < len)
-      tile[rr][tx] = R[r * s + c];
+  const int cw = s - c0 < 64 ? s - c0 : 64;
+  const int rows = len - r0 < 64 ? (int)(len - r0) : 64;
+  for (int e = threadIdx.x; e < rows * cw; e += kBlock) {
+    const int r = e / cw, c = e - r * cw;
+    tile[r * 65 + c] = R[(r0 + r) * s + c0 + c];
   }
   __syncthreads();
-  for (int cc = ty; cc < 64; cc += 4) {
-    const int c = c0 + cc;
-    const int64_t r = r0 + tx;
-    if (c < s && r < len)
-      X[r + (int64_t)c * ldx] = tile[tx][cc];
+  for (int e = threadIdx.x; e < cw * 64; e += kBlock) {
+    const int c = e >> 6, r = e & 63;
+    if (r < rows)
+      X[r0 + r + (int64_t)(c0 + c) * ldx] = tile[r * 65 + c];
   }
 }
 
