@@ -165,6 +165,7 @@ private:
   double *sc_ = nullptr;
   std::vector<double> hbuf_;
   Blk tmpq_;  // m x q scratch for apply_normal
+  int defer_ = 0;  // consecutive iterations whose few candidates wait to be refined together
   DBuf zr_, tr_;  // row-major n x q and m x q blocks of the row-major path
   DBuf vblk_, rhsblk_, wblk_, tnblk_;  // FP64 candidate refinement blocks
   DBuf hproj_;  // locked^T X coefficients of a block projection
@@ -947,15 +948,31 @@ DeviceBasis LobpcgDriver::run() {
       // products, one gather per nonzero for all columns), diag(A A^T)
       // preconditioned when the operator provides it. A v' is the CG
       // residual, so CG only has to bring ||A v|| to a tenth of the threshold.
+      //
+      // An iteration with fewer than kBatch candidates leaves them in the
+      // block (they keep converging in FP32) for up to kMaxDefer iterations,
+      // so that later candidates join the same refinement: on matrices whose
+      // gathers are random (the [R | RG] family) a product costs about the
+      // same for one column as for several. Candidates that would complete
+      // the target, and those of the last allowed iteration, are refined at
+      // once.
+      constexpr int kBatch = 4, kMaxDefer = 3;
+      int ncand = 0;
+      for (int j = 0; j < B; ++j)
+        ncand += avo[(std::size_t)j] <= cand_thr;
+      const bool defer = ncand > 0 && ncand < kBatch && defer_ < kMaxDefer &&
+                         locked_total + ncand < k_target && its < P_.max_iters;
+      defer_ = defer ? defer_ + 1 : 0;
       std::vector<int> cand;
       for (int j = 0; j < B; ++j) {
         const double Av_orig = avo[(std::size_t)j];
-        if (Av_orig <= cand_thr &&
+        if (!defer && Av_orig <= cand_thr &&
             prepare(vblk_.as<double>() + (std::size_t)cand.size() * n_, X.col(j)))
           cand.push_back(j);
         else
           best_unacc = std::min(best_unacc, Av_orig);
       }
+      S_.deferred += defer ? 1 : 0;
       const int nc = (int)cand.size();
       if (nc > 0) {
         double *Vb64 = vblk_.as<double>();
@@ -1095,9 +1112,9 @@ DeviceBasis LobpcgDriver::run() {
   if (P_.verbose)
     std::fprintf(stderr,
                  "[LOBPCG-HIP] done: it=%lld locked=%lld random columns=%lld (%.3f s) "
-                 "tiny GEPs %.3f s host total %.3f s P drops %lld\n",
+                 "tiny GEPs %.3f s host total %.3f s P drops %lld deferred refinements %lld\n",
                  (long long)its, (long long)S_.locked, (long long)S_.random_cols, S_.t_rng,
-                 S_.t_gep, S_.t_host, (long long)S_.p_drops);
+                 S_.t_gep, S_.t_host, (long long)S_.p_drops, (long long)S_.deferred);
   if (locked_.count > 0) {  // final polish (:806-831)
     const double desired = accept_thr;
     const double target = std::max(1e-14, std::min(1e-10, 0.01 * desired));
