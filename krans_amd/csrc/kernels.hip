@@ -1197,6 +1197,69 @@ __global__ void __launch_bounds__(kG2Threads)
   }
 }
 
+// H = V^T W for narrow FP64 blocks (s, q <= 16: one 16 x 16 MFMA tile), the
+// window-against-basis products of the cyclic solver at small k. A block of
+// 8 waves stages 256-row slabs of both operands in LDS (16 coalesced loads
+// in flight per thread, the next slab loading while this one is consumed);
+// wave w runs the MFMA steps of slab rows [32w, 32w + 32), and the 8 wave
+// tiles are summed in wave order at the end. Output layout as
+// k_gram2_partial with one tile (256 values per block), for k_tn2_finish.
+constexpr int kTnSlab = 256, kTnLd = kTnSlab + 1;
+
+__global__ void __launch_bounds__(kG2Threads)
+    k_tn_narrow(int64_t len, int64_t rows_per_block, const double *__restrict__ V, int64_t ldv,
+                int s, const double *__restrict__ W, int64_t ldw, int q,
+                double *__restrict__ part) {
+  __shared__ double Vs[16 * kTnLd];
+  __shared__ double Ws[16 * kTnLd];
+  __shared__ double red[kG2Waves * 256];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
+  // element u of this thread: column 2u + (tid >= 256), row tid % 256
+  const int rl = threadIdx.x & (kTnSlab - 1), chi = threadIdx.x >> 8;
+  double va[8], wa[8];
+  auto load = [&](int64_t rb) {
+    const int64_t row = (rb + rl) < r1 ? (rb + rl) : (r1 - 1);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = 2 * u + chi;
+      va[u] = c < s ? V[row + (int64_t)c * ldv] : 0.0;
+      wa[u] = c < q ? W[row + (int64_t)c * ldw] : 0.0;
+    }
+  };
+  doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
+  const int ai = (lane & 15) * kTnLd + (lane >> 4) + 32 * wid;
+  if (r0 < r1)
+    load(r0);
+  for (int64_t rb = r0; rb < r1; rb += kTnSlab) {
+    __syncthreads();  // previous slab consumed
+    const bool inr = (rb + rl) < r1;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = 2 * u + chi;
+      Vs[c * kTnLd + rl] = inr ? va[u] : 0.0;
+      Ws[c * kTnLd + rl] = inr ? wa[u] : 0.0;
+    }
+    __syncthreads();
+    if (rb + kTnSlab < r1)
+      load(rb + kTnSlab);
+#pragma unroll
+    for (int k = 0; k < 32; k += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[ai + k], Ws[ai + k], acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    red[wid * 256 + Mfma16<double>::row(lane, v) * 16 + (lane & 15)] = acc[v];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    double t = 0.0;
+    for (int w = 0; w < kG2Waves; ++w)
+      t += red[w * 256 + threadIdx.x];
+    part[(int64_t)blockIdx.x * 256 + threadIdx.x] = t;
+  }
+}
+
 // SG[0 : q*q] = S, SG[q*q : 2*q*q] = G, both column-major q x q; S is
 // mirrored from its upper triangle. One wave per output, as k_tn2_finish.
 // mode 1: H (s x q, column-major) from the VᵀW tiles. One wave per output:
@@ -2070,6 +2133,20 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     throw std::invalid_argument("gram2_mfma: more than 128 columns");
   const int ns = (s + 15) / 16, nq = (q + 15) / 16;
   const int ntiles = mode == 0 ? nq * (nq + 1) / 2 + nq * nq : ns * nq;
+  if (mode == 1 && sizeof(T) == 8 && s <= 16 && q <= 16) {
+    // one tile: 256-row slabs, about 2048 blocks
+    int64_t rpb = (len + 2047) / 2048;
+    rpb = std::max<int64_t>((rpb + kTnSlab - 1) / kTnSlab * kTnSlab, kTnSlab);
+    const int64_t nblk = (len + rpb - 1) / rpb;
+    double *part = ctx.partials((std::size_t)nblk * 256);
+    hipLaunchKernelGGL(k_tn_narrow, dim3((unsigned)nblk), dim3(kG2Threads), 0, ctx.stream, len,
+                       rpb, (const double *)Z, ldz, s, (const double *)AZ, ldaz, q, part);
+    const int64_t outs = (int64_t)s * q * 64;  // one wave per output
+    hipLaunchKernelGGL(k_tn2_finish<double>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, ctx.stream, part, (int)nblk, s, q, SG);
+    KH_CHECK(hipGetLastError());
+    return;
+  }
   // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
   // unless the partials would not fit the context's reduction buffer
   const int64_t maxblk = std::max<int64_t>(

@@ -39,5 +39,6 @@ if __name__ == "__main__":
     ap.add_argument("--title", default="rocprofv3 kernel summary")
     ap.add_argument("--cmd", default="")
     ap.add_argument("--detail", nargs="*", default=[])
+    ap.add_argument("--top", type=int, default=25)
     a = ap.parse_args()
-    print(summary(a.db, a.title, a.cmd, detail=a.detail), end="")
+    print(summary(a.db, a.title, a.cmd, top=a.top, detail=a.detail), end="")

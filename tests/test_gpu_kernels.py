@@ -153,12 +153,14 @@ def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (64, 64, 300_001, 0), (97, 33, 70_000, 0),
+@pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (16, 16, 1_000_003, 0), (11, 15, 257, 0),
+                                        (64, 64, 300_001, 0), (97, 33, 70_000, 0),
                                         (128, 128, 40_000, 0), (130, 9, 5000, 0),
                                         (40, 24, 100_000, 1)])
 def test_gemm_tn_blocks(ctx, s, q, n, prec):
     """V^T W of the block Gram-Schmidt steps: FP64 MFMA tiles for s, q <=
-    128 (FP64), FP64-accumulated tiles otherwise; within 1e-13 of numpy."""
+    128 (FP64; s, q <= 16 through the narrow 256-row-slab kernel),
+    FP64-accumulated tiles otherwise; within 1e-13 of numpy."""
     print(f"[case] gemm_tn s={s} q={q} n={n} prec={prec}", flush=True)
     rng = np.random.default_rng(s * 1000 + q)
     V = rng.standard_normal((n, s))
