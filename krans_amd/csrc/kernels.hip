@@ -1318,13 +1318,16 @@ __global__ void __launch_bounds__(kBlock)
     SG[e] = acc;
 }
 
-// OUT(:, c) (= or -=) V H(:, c), c < q <= 128, on FP64 MFMA: computed as
+// OUT(:, c) (= or -=) V H(:, c), c < q <= 16 * NQ, on FP64 MFMA: computed as
 // OUT^T = H^T V^T so that element v of lane l lands in row l % 16 of a
 // 16-row group, i.e. 16 consecutive rows per column on the store. A wave owns
-// 16 rows and all ceil(q/16) column tiles; H^T sits in LDS (column j of the
-// output at j * sld), V is read once (FP32 V converted exactly), FP64
-// accumulation. Blocks stride over 64-row chunks.
-template <class TV, class TO, bool SUB>
+// G groups of 16 rows per chunk and all ceil(q/16) column tiles; H^T sits in
+// LDS (column j of the output at j * sld), V is read once (FP32 V converted
+// exactly), FP64 accumulation. With HOIST the old OUT values of the
+// subtracting form are loaded with the chunk's V, so a chunk waits for memory
+// once (narrow outputs; wide ones would need too many registers). Blocks
+// stride over 64G-row chunks.
+template <class TV, class TO, bool SUB, int G, int NQ, bool HOIST>
 __global__ void __launch_bounds__(kBlock)
     k_nn_mfma(int64_t len, const TV *__restrict__ V, int64_t ldv, int s,
               const double *__restrict__ H, int q, TO *__restrict__ OUT, int64_t ldo, int sld) {
@@ -1338,50 +1341,78 @@ __global__ void __launch_bounds__(kBlock)
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int s4 = (s + 3) & ~3;
-  for (int64_t c0 = (int64_t)blockIdx.x * 64; c0 < len; c0 += (int64_t)gridDim.x * 64) {
-    const int64_t row0 = c0 + 16 * wid;
-    const int64_t rc = (row0 + li) < len ? (row0 + li) : (len - 1);
-    doublex4 acc[8];
+  for (int64_t c0 = (int64_t)blockIdx.x * 64 * G; c0 < len;
+       c0 += (int64_t)gridDim.x * 64 * G) {
+    int64_t ig[G], rc[G];
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < G; ++g) {
+      ig[g] = c0 + 64 * g + 16 * wid + li;
+      rc[g] = ig[g] < len ? ig[g] : (len - 1);
+    }
+    double old[G][NQ][4];
+    if (SUB && HOIST) {
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int t = 0; t < NQ; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int j = 16 * t + Mfma16<double>::row(lane, v);
+            old[g][t][v] = (t < nq && j < q && ig[g] < len)
+                               ? (double)OUT[ig[g] + (int64_t)j * ldo] : 0.0;
+          }
+    }
+    doublex4 acc[G][NQ];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int t = 0; t < NQ; ++t)
+        acc[g][t] = doublex4{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < s4; k0 += 16) {
-      double b[4];
+      double b[G][4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + 4 * u + lk;
-        const int kc = k < s ? k : s - 1;
-        const double x = (double)V[rc + (int64_t)kc * ldv];
-        b[u] = k < s ? x : 0.0;
-      }
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + 4 * u + lk;
+          const int kc = k < s ? k : s - 1;
+          const double x = (double)V[rc[g] + (int64_t)kc * ldv];
+          b[g][u] = k < s ? x : 0.0;
+        }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int kb = k0 + 4 * u;
         if (kb < s4) {
 #pragma unroll
-          for (int t = 0; t < 8; ++t)
+          for (int t = 0; t < NQ; ++t)
             if (t < nq) {
               const double a = Hs[(16 * t + li) * sld + kb + lk];
-              acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[u], acc[t], 0, 0, 0);
+#pragma unroll
+              for (int g = 0; g < G; ++g)
+                acc[g][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[g][u], acc[g][t], 0, 0, 0);
             }
         }
       }
     }
-    const int64_t i = row0 + li;
-    if (i < len) {
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (t < nq) {
+    for (int g = 0; g < G; ++g) {
+      const int64_t i = ig[g];
+      if (i < len) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int j = 16 * t + Mfma16<double>::row(lane, v);
-            if (j < q) {
-              TO *o = OUT + i + (int64_t)j * ldo;
-              *o = SUB ? (TO)((double)*o - acc[t][v]) : (TO)acc[t][v];
+        for (int t = 0; t < NQ; ++t)
+          if (t < nq) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int j = 16 * t + Mfma16<double>::row(lane, v);
+              if (j < q) {
+                TO *o = OUT + i + (int64_t)j * ldo;
+                const double ov = HOIST ? old[g][t][v] : (SUB ? (double)*o : 0.0);
+                *o = SUB ? (TO)(ov - acc[g][t][v]) : (TO)acc[g][t][v];
+              }
             }
           }
-        }
+      }
     }
   }
 }
@@ -1921,10 +1952,17 @@ static bool try_nn_mfma(Context &ctx, const TV *V, int64_t ldv, int s, const dou
   const std::size_t lds = (std::size_t)qp * sld * sizeof(double);
   if (lds > 65536)
     return false;
-  const int64_t chunks = (len + 63) / 64;
-  const unsigned grid = (unsigned)std::min<int64_t>(chunks, 4096);
-  hipLaunchKernelGGL((k_nn_mfma<TV, TO, SUB>), dim3(grid), dim3(kBlock), lds, ctx.stream, len, V,
-                     ldv, s, H, q, out, ldo, sld);
+  if (q <= 32) {  // two 16-row groups per wave
+    const int64_t chunks = (len + 127) / 128;
+    const unsigned grid = (unsigned)std::min<int64_t>(chunks, 4096);
+    hipLaunchKernelGGL((k_nn_mfma<TV, TO, SUB, 2, 2, true>), dim3(grid), dim3(kBlock), lds, ctx.stream,
+                       len, V, ldv, s, H, q, out, ldo, sld);
+  } else {
+    const int64_t chunks = (len + 63) / 64;
+    const unsigned grid = (unsigned)std::min<int64_t>(chunks, 4096);
+    hipLaunchKernelGGL((k_nn_mfma<TV, TO, SUB, 1, 8, false>), dim3(grid), dim3(kBlock), lds, ctx.stream,
+                       len, V, ldv, s, H, q, out, ldo, sld);
+  }
   KH_CHECK(hipGetLastError());
   return true;
 }

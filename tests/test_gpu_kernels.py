@@ -177,7 +177,8 @@ def test_gemm_tn_blocks(ctx, s, q, n, prec):
     assert np.all(np.abs(H - ref) <= 1e-13 * scale)
 
 
-@pytest.mark.parametrize("s,q,n,prec,sub", [(5, 7, 1000, 0, 1), (64, 64, 300_001, 0, 1),
+@pytest.mark.parametrize("s,q,n,prec,sub", [(5, 7, 1000, 0, 1), (16, 15, 1_000_003, 0, 1),
+                                            (16, 16, 129, 0, 0), (64, 64, 300_001, 0, 1),
                                             (97, 33, 70_000, 0, 0), (64, 128, 20_000, 0, 1),
                                             (3, 1, 5000, 0, 1), (200, 40, 9000, 0, 1),
                                             (40, 24, 100_000, 1, 0), (96, 32, 50_000, 1, 1)])
