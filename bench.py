@@ -101,26 +101,16 @@ def main():
         sine = float(np.linalg.norm(R, 2))
         return float(res.max() / A.fro()), sine
 
-    trace = {"launches": 0, "bytes": 0.0, "ms": 0.0}
-
-    def run(precision, steps, warmup, traced=False):
+    def run(precision, steps, warmup):
         lat, found, chk, info = [], [], None, None
         seed = 42
         for i in range(warmup + steps):
             vf = K.HipVectorFactory(A.n, seed)
             H.ctx.sync()
-            timed = i >= warmup
-            if traced and timed:
-                H.ctx.trace_begin("vtw")
             t = time.perf_counter()
             r = K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision=precision)
             H.ctx.sync()
             dt = time.perf_counter() - t
-            if traced and timed:
-                n_l, b_l, ms_l = H.ctx.trace_end()
-                trace["launches"] += n_l
-                trace["bytes"] += b_l
-                trace["ms"] += ms_l
             if args.verbose:
                 print(f"[{precision}] solve {i}: {dt:.3f}s found={r.right_null.shape[1]} "
                       f"restarts={r.restarts} cg={r.cg_solves} times={r.times}",
@@ -135,7 +125,7 @@ def main():
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else args.steps
     copy_probe(K, H.ctx)
-    lat_m, found_m, chk_m, info_m = run("mixed", args.steps, args.warmup, traced=True)
+    lat_m, found_m, chk_m, info_m = run("mixed", args.steps, args.warmup)
     if fp64_steps > 0:
         lat_d, found_d, chk_d, info_d = run("fp64", fp64_steps, args.warmup)
     else:
@@ -154,19 +144,29 @@ def main():
         hi = min(lo + 1, len(xs) - 1)
         return xs[lo] + (xs[hi] - xs[lo]) * (pos - lo)
 
-    # roofline of the dominant kernel of the timed mixed solves: the FP32
-    # reorthogonalisation product V^T w (k_vtw_partial<float,float>), timed
-    # per launch with HIP events on the library stream during the timed solves
-    achieved = trace["bytes"] / (trace["ms"] * 1e-3) / 1e9 if trace["ms"] > 0 else None
-    avg_bytes = trace["bytes"] / max(1, trace["launches"])
-    avg_ms = trace["ms"] / max(1, trace["launches"])
-    peak = 8000.0
-    # per-class breakdown of one more (untimed) mixed solve: every launch of
-    # the reorthogonalisation products, updates and sparse products
+    # One more (untimed) mixed solve with every launch of the
+    # reorthogonalisation products, updates and sparse products bracketed by
+    # HIP events (per-class breakdown), recording the (rows, columns) shape
+    # of each launch of the dominant kernel, the FP32 V^T w product
+    # k_vtw_partial<float,float>. Its roofline: each shape the solve used is
+    # replayed back to back between two events on the library stream (an
+    # event pair around a single launch would add the dispatch gap), and the
+    # per-shape durations are weighted by the solve's launch counts.
     classes = ["vtw", "vh_sub", "spmv"]
     vf = K.HipVectorFactory(A.n, 42)
     H.ctx.trace_begin_classes(classes)
     K.find_nullspace_cyclic_lanczos_ex(H, vf, opt, precision="mixed")
+    shapes = H.ctx.trace_shapes("vtw")
+    vt_launches, vt_bytes, vt_ms = 0, 0.0, 0.0
+    for ln, sc, cnt in shapes:
+        nb = (ln + 2047) // 2048
+        vt_launches += cnt
+        vt_bytes += cnt * (4.0 * ln * sc + 4.0 * ln + 8.0 * nb * sc)
+        vt_ms += cnt * H.ctx.time_vtw(ln, sc, "fp32", reps=10)
+    achieved = vt_bytes / (vt_ms * 1e-3) / 1e9 if vt_ms > 0 else None
+    avg_bytes = vt_bytes / max(1, vt_launches)
+    avg_ms = vt_ms / max(1, vt_launches)
+    peak = 8000.0
     per_class = {}
     for k_, (n_l, b_l, ms_l) in H.ctx.trace_classes_end(classes).items():
         gbs = b_l / (ms_l * 1e-3) / 1e9 if ms_l > 0 else None
@@ -220,8 +220,11 @@ def main():
             "peak": peak, "unit": "GB/s",
             "frac": None if achieved is None else round(achieved / peak, 4),
             "traffic": traffic,
-            "launches": trace["launches"], "avg_bytes_per_launch": round(avg_bytes),
+            "launches": vt_launches, "shapes": len(shapes),
+            "avg_bytes_per_launch": round(avg_bytes),
             "avg_us_per_launch": round(1e3 * avg_ms, 3),
+            "timing": "each (rows, columns) shape of one mixed solve replayed back to back "
+                      "between HIP events, weighted by its launch count",
             "per_class_mixed_solve": per_class,
             "spmv_fp32": {"avg_us": round(1e3 * ms_spmv, 2),
                           "GBps": round(spmv_bytes(A, 4) / (ms_spmv * 1e-3) / 1e9, 1),
@@ -332,7 +335,9 @@ def variant(args):
             vf = K.HipVectorFactory(A.n, 42)
             H.ctx.sync()
             timed = i >= args.warmup
-            traced = timed and prec == "mixed" and trace is None
+            # the Gram class (about 300 launches of ~0.7 ms) is traced inside
+            # the timed solve; sparse products are timed separately below
+            traced = timed and prec == "mixed" and trace is None and cls == "gram"
             if traced:
                 H.ctx.trace_begin(cls)
             t = time.perf_counter()
@@ -357,16 +362,28 @@ def variant(args):
         out[prec] = ent
         del r0
 
-    n_l, amount, ms = trace
     if cls == "gram":
+        n_l, amount, ms = trace
         achieved = amount / (ms * 1e-3) / 1e12 if ms > 0 else None
         peak, unit, bound = 157.3, "TFLOP/s", "mfma"
         kern = "k_gram2_partial<float> (Z^T [Z AZ] in one pass, FP32 MFMA 16x16x4)"
         traffic = load_traffic("k_gram2_partial<float>")
     else:
-        achieved = amount / (ms * 1e-3) / 1e9 if ms > 0 else None
+        # the FP32 sparse product replayed back to back between two HIP events
+        # (A and A^T, averaged)
+        ms_a = H.time_spmv(transpose=False, precision="fp32", reps=100)
+        ms_t = H.time_spmv(transpose=True, precision="fp32", reps=100)
+        if args.matrix_free and cfg == "c3a":
+            per = 4.0 * (A.n + A.m)  # x read, y written; no matrix to read
+            kern = "k_sd_mv / k_sd_mtv<float> (stencil products, FP32)"
+        else:
+            per = spmv_bytes(A, 4)
+            kern = "k_spmv_lane<float> (CSR and CSC-twin gathers, FP32)"
+        ms = 0.5 * (ms_a + ms_t)
+        n_l = 200
+        achieved = per / (ms * 1e-3) / 1e9
+        ms *= n_l
         peak, unit, bound = 8000.0, "GB/s", "hbm"
-        kern = "k_spmv_* / k_spmm_* (CSR and CSC-twin gathers, mixed solve)"
         traffic = None
     line = {
         "metric": "null vectors/sec at ||Au||/||A||_F<=1e-12, mixed vs FP64, 1 MI355X",
@@ -386,7 +403,10 @@ def variant(args):
                      "peak": peak, "unit": unit,
                      "frac": None if achieved is None else round(achieved / peak, 4),
                      "traffic": traffic, "launches": n_l,
-                     "avg_us_per_launch": round(1e3 * ms / max(1, n_l), 3)},
+                     "avg_us_per_launch": round(1e3 * ms / max(1, n_l), 3),
+                     **({} if cls == "gram" else
+                        {"timing": "100 launches each of y = A x and y = A^T x back to back "
+                                   "between HIP events"})},
         "cpu_baseline": None,
         "setup_s": round(setup_s, 2),
     }

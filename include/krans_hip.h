@@ -283,6 +283,19 @@ int kh_trace_class(kh_context *ctx, int kernel_class, int64_t *launches, double 
  * HIP events on the library stream. */
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps,
                  double *avg_ms);
+/* Launch shapes of one traced class (call before kh_trace_end): the distinct
+ * (len, s) pairs its launches recorded (rows and columns of the V^T w
+ * products of class 1; classes that record no shape report none), with the
+ * number of launches of each, at most `max` pairs in order of first use.
+ * Observability of this build; the reference has no counterpart. */
+int kh_trace_shapes(kh_context *ctx, int kernel_class, int max, int64_t *len, int32_t *s,
+                    int64_t *count, int32_t *n_shapes);
+/* Average device time (ms) of one V^T w partial-product launch (the kernel of
+ * class 1 without its finishing sum) on len x s zero blocks of precision
+ * 0 (FP64) or 1 (FP32), over `reps` back-to-back launches between two HIP
+ * events on the context's stream; 0 < s <= 512. Observability of this build;
+ * the reference has no counterpart. */
+int kh_time_vtw(kh_context *ctx, int precision, int64_t len, int s, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

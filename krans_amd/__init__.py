@@ -142,6 +142,9 @@ SIGNATURES = {
     "kh_trace_end": (I32, [P, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
     "kh_trace_begin_mask": (I32, [P, C.c_uint]),
     "kh_trace_class": (I32, [P, I32, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
+    "kh_trace_shapes": (I32, [P, I32, I32, C.POINTER(I64), C.POINTER(I32), C.POINTER(I64),
+                              C.POINTER(I32)]),
+    "kh_time_vtw": (I32, [P, I32, I64, I32, I32, C.POINTER(D)]),
 }
 
 _lib = None
@@ -222,6 +225,25 @@ class HipContext:
         for k in classes:
             mask |= 1 << self.TRACE_CLASSES[k]
         _check(lib().kh_trace_begin_mask(self.h, mask))
+
+    def trace_shapes(self, kernel_class, max_shapes=4096):
+        """[(len, s, launches)] of the traced class's launches (kh_trace_shapes);
+        call before trace_end."""
+        L = (I64 * max_shapes)()
+        S = (I32 * max_shapes)()
+        N = (I64 * max_shapes)()
+        n = I32()
+        _check(lib().kh_trace_shapes(self.h, self.TRACE_CLASSES[kernel_class], max_shapes, L, S,
+                                     N, C.byref(n)))
+        return [(L[i], S[i], N[i]) for i in range(n.value)]
+
+    def time_vtw(self, length, s, precision="fp32", reps=20):
+        """Average device ms of one V^T w partial launch at length x s,
+        back-to-back (kh_time_vtw)."""
+        out = D()
+        _check(lib().kh_time_vtw(self.h, PREC[precision], int(length), int(s), int(reps),
+                                 C.byref(out)))
+        return out.value
 
     def trace_classes_end(self, classes):
         """{class: (launches, algorithmic bytes or flops, device ms)}, then

@@ -776,6 +776,41 @@ int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms) 
   });
 }
 
+int kh_trace_shapes(kh_context *ctx, int kernel_class, int max, int64_t *len, int32_t *s,
+                    int64_t *count, int32_t *n_shapes) {
+  return guard([&] {
+    need(ctx && n_shapes && max >= 0, "bad argument");
+    need(max == 0 || (len && s && count), "null argument");
+    auto &c = *ctx->ctx;
+    int n = 0;
+    for (auto &r : c.trace) {
+      if (r.cls != kernel_class || r.s <= 0)
+        continue;
+      int i = 0;
+      while (i < n && !(len[i] == r.len && s[i] == r.s))
+        ++i;
+      if (i < n) {
+        ++count[i];
+      } else if (n < max) {
+        len[n] = r.len;
+        s[n] = r.s;
+        count[n] = 1;
+        ++n;
+      } else {
+        throw std::length_error("kh_trace_shapes: more distinct shapes than max");
+      }
+    }
+    *n_shapes = n;
+  });
+}
+
+int kh_time_vtw(kh_context *ctx, int precision, int64_t len, int s, int reps, double *avg_ms) {
+  return guard([&] {
+    need(ctx && avg_ms, "null argument");
+    *avg_ms = kh::time_vtw(*ctx->ctx, prec_of(precision), len, s, reps);
+  });
+}
+
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps, double *avg_ms) {
   return guard([&] {
     need(A && avg_ms && reps >= 1, "bad argument");

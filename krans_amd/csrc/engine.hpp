@@ -46,6 +46,8 @@ struct TraceRec {
   double bytes;
   hipEvent_t a, b;
   int cls;
+  int64_t len = 0;  // launch shape, where the class records one (rows, columns)
+  int s = 0;
 };
 
 struct Context {
@@ -151,6 +153,10 @@ void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void 
 void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
          int64_t len, double *out, bool do_sqrt = false);
 // out[j] = V(:,j) . w for j < s (FP64 accumulate).
+// Average duration (ms) of one V^T w partial launch (k_vtw_partial, no
+// finishing sum) at len x s in precision p, over reps back-to-back launches
+// on the context's stream (V, w zero-filled, allocated here).
+double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps);
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
             const void *w, int64_t len, double *out);
 // w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.

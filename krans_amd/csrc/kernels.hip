@@ -1711,8 +1711,11 @@ struct TraceScope {
   hipEvent_t a = nullptr;
   double bytes;
   int cls;
-  TraceScope(Context &ctx, int cls_, double b)
-      : c(ctx), on(cls_ > 0 && ((ctx.trace_mask >> cls_) & 1u)), bytes(b), cls(cls_) {
+  int64_t len = 0;
+  int s = 0;
+  TraceScope(Context &ctx, int cls_, double b, int64_t len_ = 0, int s_ = 0)
+      : c(ctx), on(cls_ > 0 && ((ctx.trace_mask >> cls_) & 1u)), bytes(b), cls(cls_), len(len_),
+        s(s_) {
     if (on) {
       a = c.take_event();
       KH_CHECK(hipEventRecord(a, c.stream));
@@ -1722,7 +1725,7 @@ struct TraceScope {
     if (on) {
       hipEvent_t b2 = c.take_event();
       (void)hipEventRecord(b2, c.stream);
-      c.trace.push_back({bytes, a, b2, cls});
+      c.trace.push_back({bytes, a, b2, cls, len, s});
     }
   }
 };
@@ -1852,14 +1855,53 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
   for (int c0 = 0; c0 < s; c0 += 512) {  // LDS holds 512 x 4 partials
     const int sc = std::min(512, s - c0);
     double *part = ctx.partials((std::size_t)nb * sc);
-    TraceScope tr(ctx, kTraceVtW,
-                  (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc);
-    hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock),
-                       (std::size_t)sc * kWaves * sizeof(double), ctx.stream, len,
-                       V + (int64_t)c0 * ld, ld, sc, w, part, nb);
+    {
+      TraceScope tr(ctx, kTraceVtW,
+                    (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc,
+                    len, sc);
+      hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock),
+                         (std::size_t)sc * kWaves * sizeof(double), ctx.stream, len,
+                         V + (int64_t)c0 * ld, ld, sc, w, part, nb);
+    }
     hipLaunchKernelGGL(k_finish, dim3(sc), dim3(kBlock), 0, ctx.stream, part, nb, out + c0, 0);
   }
   KH_CHECK(hipGetLastError());
+}
+
+template <class T>
+static double time_vtw_t(Context &ctx, int64_t len, int s, int reps) {
+  if (len <= 0 || s <= 0 || s > 512 || reps < 1)
+    throw std::invalid_argument("time_vtw: 0 < s <= 512, len > 0, reps >= 1");
+  const int nb = vtw_blocks(len);
+  double *part = ctx.partials((std::size_t)nb * s);
+  DBuf V((std::size_t)len * s * sizeof(T)), w((std::size_t)len * sizeof(T));
+  KH_CHECK(hipMemsetAsync(V.p, 0, V.bytes, ctx.stream));
+  KH_CHECK(hipMemsetAsync(w.p, 0, w.bytes, ctx.stream));
+  auto launch = [&] {
+    hipLaunchKernelGGL((k_vtw_partial<T, T>), dim3(nb), dim3(kBlock),
+                       (std::size_t)s * kWaves * sizeof(double), ctx.stream, len,
+                       (const T *)V.p, len, s, (const T *)w.p, part, nb);
+  };
+  launch();
+  hipEvent_t e0, e1;
+  KH_CHECK(hipEventCreate(&e0));
+  KH_CHECK(hipEventCreate(&e1));
+  KH_CHECK(hipEventRecord(e0, ctx.stream));
+  for (int r = 0; r < reps; ++r)
+    launch();
+  KH_CHECK(hipEventRecord(e1, ctx.stream));
+  KH_CHECK(hipGetLastError());
+  KH_CHECK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  KH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
+double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps) {
+  return p == Prec::F64 ? time_vtw_t<double>(ctx, len, s, reps)
+                        : time_vtw_t<float>(ctx, len, s, reps);
 }
 
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,

@@ -201,3 +201,27 @@ def test_gemm_nn_blocks(ctx, s, q, n, prec, sub):
     scale = np.abs(V.astype(np.float64)) @ np.abs(H) + (np.abs(W) if sub else 0)
     tol = 1e-13 if prec == 0 else 2 * np.finfo(np.float32).eps
     assert np.all(np.abs(got - ref) <= tol * scale + 1e-300)
+
+
+def test_trace_shapes_and_vtw_timer(ctx):
+    """The roofline plumbing: a traced CGS2 records the (rows, columns) shape
+    of its V^T w launches, trace_end clears them, and the back-to-back timer
+    returns a positive duration for that shape (and rejects s > 512)."""
+    n, s = 100_003, 7
+    print(f"[case] trace shapes n={n} s={s}", flush=True)
+    rng = np.random.default_rng(5)
+    V, _ = np.linalg.qr(rng.standard_normal((n, s)))
+    dV, dw = Dev(ctx, V.astype(np.float32)), Dev(ctx, rng.standard_normal(n).astype(np.float32))
+    h = np.zeros(2 * s)
+    nrm = C.c_double()
+    ctx.trace_begin("vtw")
+    K._check(K.lib().kh_cgs2_dev(ctx.h, 1, dV.p, s, dw.p, n, h.ctypes.data_as(C.c_void_p),
+                                 C.byref(nrm)))
+    shapes = ctx.trace_shapes("vtw")
+    launches, _, _ = ctx.trace_end()
+    assert launches >= 1 and shapes and all(ln == n and sc == s for ln, sc, _ in shapes)
+    assert sum(c for _, _, c in shapes) == launches
+    assert ctx.trace_shapes("vtw") == []  # cleared by trace_end
+    assert 0.0 < ctx.time_vtw(n, s, "fp32", reps=5) < 100.0
+    with pytest.raises(ValueError):
+        ctx.time_vtw(n, 513, "fp32")
