@@ -212,7 +212,8 @@ def main():
         "mixed_over_fp64": round(value / fp64_value, 3) if fp64_value else None,
         "accuracy": {"max_res_rel": chk_m[0], "sine_vs_exact": chk_m[1],
                      "vectors": found_m[0], "restarts": info_m.restarts,
-                     "cg_solves": info_m.cg_solves, "phase_s": info_m.times},
+                     "cg_solves": info_m.cg_solves, "phase_s": info_m.times,
+                     "screened": info_m.screened},
         "roofline": {
             "kernel": "k_vtw_partial<float, float> (reorthogonalisation V^T w, FP32 in, FP64 acc)",
             "bound": "hbm",
@@ -356,7 +357,8 @@ def variant(args):
         ent = {"value": round(sum(found) / sum(lat), 4), "requested": k, "s_per_solve": round(sum(lat) / len(lat), 4),
                "vectors": found[0], "max_res_rel": res, "sine_vs_exact": sine}
         if solver == "cyclic":
-            ent.update(restarts=r0.restarts, cg_solves=r0.cg_solves, phase_s=r0.times)
+            ent.update(restarts=r0.restarts, cg_solves=r0.cg_solves, phase_s=r0.times,
+                       screened=r0.screened)
         else:
             ent.update(iters=r0.iters, converged=r0.converged_to_target, phase_s=r0.times)
         out[prec] = ent

@@ -66,6 +66,7 @@ typedef struct kh_cyclic_result {
   double frob_norm_est;
   int64_t converged_to_target, cg_solves, cg_maxit_hits, spmv, spmtv, cg_iters;
   double t_gk, t_tiny, t_cand, t_polish, t_fallback, t_rng;
+  int64_t screened; /* mixed precision: candidates rejected by the FP32 screen */
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and

@@ -56,6 +56,7 @@ class _CyclicRes(C.Structure):
         ("cg_iters", C.c_int64),
         ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
+        ("screened", C.c_int64),
     ]
 
 
@@ -490,6 +491,7 @@ class CyclicLanczosResult:
     spmtv: int = 0
     cg_iters: int = 0
     times: dict = field(default_factory=dict)
+    screened: int = 0  # mixed precision: candidates rejected by the FP32 screen
 
 
 def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
@@ -519,7 +521,7 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         restarts=res.restarts, krylov_steps_total=res.krylov_steps_total,
         frob_norm_est=res.frob_norm_est, converged_to_target=bool(res.converged_to_target),
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
-        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times)
+        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened)
 
 
 def find_nullspace_cyclic_lanczos(A, vf, opt, precision="fp64"):

@@ -376,6 +376,7 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     res->t_polish = S.t.polish + SL.t.polish;
     res->t_fallback = S.t.fallback + SL.t.fallback;
     res->t_rng = S.t.rng + SL.t.rng;
+    res->screened = S.screened + SL.screened;
   });
 }
 
@@ -493,6 +494,7 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->t_polish = r.stats.t.polish;
     res->t_fallback = r.stats.t.fallback;
     res->t_rng = r.stats.t.rng;
+    res->screened = r.stats.screened;
     if (latency_s)
       *latency_s = r.latency;
     if (queued_s)

@@ -117,6 +117,9 @@ private:
   std::vector<double> hbuf_;
   // window blocks
   DBuf Wv_, AV_, Vc_, AVc_, Vs_, Rs_, Vp_, AVp_, Wm_, Tn_, Wm64_;
+  DBuf Wv32_, AV32_;      // FP32 screen of a window (mixed precision)
+  bool screen_ = false;   // screen candidate windows in FP32 first
+  int screen_miss_ = 0;   // consecutive windows the screen did not thin out
   int wcap_ = 0;
   CgBatch cg_;
 
@@ -198,6 +201,10 @@ void CyclicDriver::ensure_window(int w) {
   Wm64_ = DBuf(cm);
   if (P_.polish == Prec::F32)
     Wm_ = DBuf((std::size_t)wcap_ * m_ * 4 * 2);  // FP32 w and rhs
+  if (kp_ == Prec::F32) {
+    Wv32_ = DBuf((std::size_t)wcap_ * n_ * 4);
+    AV32_ = DBuf((std::size_t)wcap_ * m_ * 4);
+  }
 }
 
 void CyclicDriver::random_vec(double *d) {
@@ -504,6 +511,9 @@ DeviceBasis CyclicDriver::run() {
   S_.frob_norm_est = estimate_fro();
   accept_thr_ = std::max(P_.tol_abs, P_.tol_rel * std::max(1.0, S_.frob_norm_est));
   prescreen_thr_ = std::max(0.0, P_.prescreen_factor) * std::max(1.0, S_.frob_norm_est);
+  // unit vectors have ||A v|| <= ||A||_2 <= ||A||_F: a screen below a tenth
+  // of the norm can reject candidates
+  screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1;
 
   double *y0 = y0_.as<double>();
   if (P_.y0_host)
@@ -639,8 +649,11 @@ DeviceBasis CyclicDriver::run() {
         return std::abs(ev[a] - P_.target_tau) < std::abs(ev[b] - P_.target_tau);
       });
     }
-    for (double x : ev)
+    double smax = 0.0;  // largest singular value of the projected problem
+    for (double x : ev) {
       finite = finite && std::isfinite(x);
+      smax = std::max(smax, std::fabs(x));
+    }
     for (double x : V.a)
       finite = finite && std::isfinite(x);
     S_.t.tiny += since(t0);
@@ -673,7 +686,8 @@ DeviceBasis CyclicDriver::run() {
       int wn = single_next ? 1 : (int)std::min<int64_t>((int64_t)(cl.size() - pos), test_cap - tested);
       wn = std::min(wn, 64);
       ensure_window(wn);
-      // coefficient block C (ny_used x wn)
+      // coefficient block C (ny_used x wn); Y_{ny_used-1} .. Y_0 are
+      // contiguous (reversed), so the coefficients are stored in reverse order
       std::vector<double> Ch((std::size_t)ny_used * wn, 0.0);
       for (int c = 0; c < wn; ++c) {
         const int jb = cl[pos + (std::size_t)c];
@@ -682,49 +696,134 @@ DeviceBasis CyclicDriver::run() {
       }
       upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
       double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
-      // Y_{ny_used-1} .. Y_0 are contiguous (reversed), so the coefficients
-      // are stored in reverse order
-      gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F64, Wv, n_, n_);
-      gs_basis_block(Wv, wn);
-      col_norms(c_, Prec::F64, Wv, n_, wn, n_, wsc_);
-      std::vector<double> nv(wn);
-      readback(c_, wsc_, wn, nv.data());
       std::vector<ColOut> out((std::size_t)wn);
-      std::vector<double> sc(wn, 0.0);
-      std::vector<int> valid(wn, 0);
-      for (int c = 0; c < wn; ++c) {
-        out[(std::size_t)c].nv = nv[(std::size_t)c];
-        if (std::isfinite(nv[(std::size_t)c]) && nv[(std::size_t)c] >= 1e-8) {
-          out[(std::size_t)c].valid = true;
-          valid[(std::size_t)c] = 1;
-          sc[(std::size_t)c] = 1.0 / nv[(std::size_t)c];
+      // Mixed precision: the window is screened in FP32 first (Y C, two
+      // Gram-Schmidt passes against the FP32 copy of the basis, ||A v||).
+      // A column whose FP32 residual clears the prescreen threshold by more
+      // than the FP32 error margin is rejected as the FP64 test would reject
+      // it (its FP32 vector becomes the best-so-far restart candidate when
+      // it is the best); every other column (possible null vectors, columns
+      // near the basis span) is evaluated in FP64 exactly as without the
+      // screen. Screening switches off for the solve when it keeps failing to
+      // reject columns (thresholds above the spectrum).
+      std::vector<int> sub;             // window columns evaluated in FP64
+      std::vector<char> screened((std::size_t)wn, 0);
+      float *W32 = nullptr;
+      if (screen_) {
+        W32 = Wv32_.as<float>();
+        float *AV32 = AV32_.as<float>();
+        gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F32, W32, n_, n_);
+        const int nb = (int)basis_.count;
+        if (nb > 0) {
+          const void *B32 = VBb + (std::size_t)ycap_ * ldy_ * ke;
+          for (int pass = 0; pass < 2; ++pass) {
+            gemm_tn_f32_mfma(c_, static_cast<const float *>(B32), ldy_, nb, W32, n_, wn, n_, H_);
+            gemm_nn_sub(c_, Prec::F32, B32, ldy_, nb, H_, wn, Prec::F32, W32, n_, n_);
+          }
         }
+        col_norms(c_, Prec::F32, W32, n_, wn, n_, wsc_);
+        std::vector<double> nv32(wn);
+        readback(c_, wsc_, wn, nv32.data());
+        std::vector<double> sc32(wn, 0.0);
+        std::vector<int> ok32(wn, 0);
+        for (int c = 0; c < wn; ++c)
+          if (std::isfinite(nv32[(std::size_t)c]) && nv32[(std::size_t)c] >= 1e-5) {
+            ok32[(std::size_t)c] = 1;
+            sc32[(std::size_t)c] = 1.0 / nv32[(std::size_t)c];
+          }
+        upload_doubles(c_, sc32.data(), wn, wsc_ + 1024);
+        scale_cols(c_, Prec::F32, n_, wn, W32, n_, wsc_ + 1024, true);
+        upload_mask(ok32);
+        block_product(c_, A_, Prec::F32, wn, W32, n_, AV32, m_, false, mask_.as<int>(), rin_,
+                      rout_);
+        col_norms(c_, Prec::F32, AV32, m_, wn, m_, wsc_ + 2048);
+        std::vector<double> r32(wn);
+        readback(c_, wsc_ + 2048, wn, r32.data());
+        // FP32 error of ||A v|| for a unit v: a few units of 6e-8 times
+        // ||A||_2, estimated (from below) by the largest singular value of
+        // the projected problem; the margin allows 1e-5 of twice that
+        const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
+        for (int c = 0; c < wn; ++c) {
+          const double r = r32[(std::size_t)c];
+          if (ok32[(std::size_t)c] && std::isfinite(r) && r > prescreen_thr_ + margin) {
+            screened[(std::size_t)c] = 1;
+            ColOut &o = out[(std::size_t)c];
+            o.valid = true;
+            o.nv = nv32[(std::size_t)c];
+            o.res_raw = o.res = r;
+          } else {
+            sub.push_back(c);
+          }
+        }
+        S_.screened += (int64_t)(wn - (int)sub.size());
+        if (wn >= 4) {
+          screen_miss_ = (int)sub.size() * 4 > wn * 3 ? screen_miss_ + 1 : 0;
+          if (screen_miss_ >= 3)
+            screen_ = false;
+        }
+        if (!sub.empty() && (int)sub.size() < wn) {  // compact the FP64 coefficients
+          std::vector<double> Cs((std::size_t)ny_used * sub.size());
+          for (std::size_t t = 0; t < sub.size(); ++t)
+            std::copy(Ch.begin() + (std::ptrdiff_t)((std::size_t)sub[t] * ny_used),
+                      Ch.begin() + (std::ptrdiff_t)((std::size_t)(sub[t] + 1) * ny_used),
+                      Cs.begin() + (std::ptrdiff_t)(t * ny_used));
+          upload_doubles(c_, Cs.data(), (int)Cs.size(), C_);
+        }
+      } else {
+        for (int c = 0; c < wn; ++c)
+          sub.push_back(c);
       }
-      upload_doubles(c_, sc.data(), wn, wsc_ + 1024);
-      scale_cols(c_, Prec::F64, n_, wn, Wv, n_, wsc_ + 1024, true);
-      upload_mask(valid);
-      block_product(c_, A_, Prec::F64, wn, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
-      col_norms(c_, Prec::F64, AV, m_, wn, m_, wsc_ + 2048);
-      std::vector<double> rr(wn);
-      readback(c_, wsc_ + 2048, wn, rr.data());
-      std::vector<int> pol;
-      for (int c = 0; c < wn; ++c) {
-        ColOut &o = out[(std::size_t)c];
-        if (!o.valid)
-          continue;
-        ++S_.spmv;
-        o.res_raw = o.res = rr[(std::size_t)c];
-        if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
-            o.res_raw > accept_thr_)
-          pol.push_back(c);
-      }
-      slot_of_.assign((std::size_t)wn, -1);
-      for (std::size_t i = 0; i < pol.size(); ++i)
-        slot_of_[(std::size_t)pol[i]] = (int)i;
-      if (!pol.empty()) {
-        auto tp = Clock::now();
-        polish_batch(pol, out);
-        S_.t.polish += since(tp);
+      // FP64 evaluation of the columns in `sub` (window column sub[t] sits in
+      // column t of Wv / AV)
+      const int ws = (int)sub.size();
+      std::vector<int> t64((std::size_t)wn, -1);
+      for (int t = 0; t < ws; ++t)
+        t64[(std::size_t)sub[(std::size_t)t]] = t;
+      slot_of_.assign((std::size_t)std::max(ws, 1), -1);
+      if (ws > 0) {
+        gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, ws, Prec::F64, Wv, n_, n_);
+        gs_basis_block(Wv, ws);
+        col_norms(c_, Prec::F64, Wv, n_, ws, n_, wsc_);
+        std::vector<double> nv(ws);
+        readback(c_, wsc_, ws, nv.data());
+        std::vector<ColOut> o64((std::size_t)ws);
+        std::vector<double> sc(ws, 0.0);
+        std::vector<int> valid(ws, 0);
+        for (int t = 0; t < ws; ++t) {
+          o64[(std::size_t)t].nv = nv[(std::size_t)t];
+          if (std::isfinite(nv[(std::size_t)t]) && nv[(std::size_t)t] >= 1e-8) {
+            o64[(std::size_t)t].valid = true;
+            valid[(std::size_t)t] = 1;
+            sc[(std::size_t)t] = 1.0 / nv[(std::size_t)t];
+          }
+        }
+        upload_doubles(c_, sc.data(), ws, wsc_ + 1024);
+        scale_cols(c_, Prec::F64, n_, ws, Wv, n_, wsc_ + 1024, true);
+        upload_mask(valid);
+        block_product(c_, A_, Prec::F64, ws, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
+        col_norms(c_, Prec::F64, AV, m_, ws, m_, wsc_ + 2048);
+        std::vector<double> rr(ws);
+        readback(c_, wsc_ + 2048, ws, rr.data());
+        std::vector<int> pol;
+        for (int t = 0; t < ws; ++t) {
+          ColOut &o = o64[(std::size_t)t];
+          if (!o.valid)
+            continue;
+          ++S_.spmv;
+          o.res_raw = o.res = rr[(std::size_t)t];
+          if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
+              o.res_raw > accept_thr_)
+            pol.push_back(t);
+        }
+        for (std::size_t i = 0; i < pol.size(); ++i)
+          slot_of_[(std::size_t)pol[i]] = (int)i;
+        if (!pol.empty()) {
+          auto tp = Clock::now();
+          polish_batch(pol, o64);
+          S_.t.polish += since(tp);
+        }
+        for (int t = 0; t < ws; ++t)
+          out[(std::size_t)sub[(std::size_t)t]] = o64[(std::size_t)t];
       }
       // replay the sequential accept / stall / best logic over the window
       std::size_t consumed = (std::size_t)wn;
@@ -739,14 +838,18 @@ DeviceBasis CyclicDriver::run() {
         if (o.res_raw > prescreen_thr_ || polish_stalled) {
           if (o.res_raw < best_res) {
             best_res = o.res_raw;
-            KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)c * n_, (std::size_t)n_ * 8,
-                                    hipMemcpyDeviceToDevice, c_.stream));
+            if (screened[(std::size_t)c])
+              convert(c_, Prec::F32, W32 + (int64_t)c * n_, Prec::F64, best_v_.p, n_);
+            else
+              KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)t64[(std::size_t)c] * n_,
+                                      (std::size_t)n_ * 8, hipMemcpyDeviceToDevice, c_.stream));
             have_best_v = true;
           }
           continue;
         }
-        const int slot = slot_of_[(std::size_t)c];
-        const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)c * n_;
+        const int t = t64[(std::size_t)c];
+        const int slot = slot_of_[(std::size_t)t];
+        const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)t * n_;
         S_.cg_solves += o.passes;
         if (o.hit)
           S_.cg_maxit_hits += 1;  // one counted hit per column and pass set
@@ -826,21 +929,53 @@ void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, 
   if (q <= 0)
     return;
   const int64_t nin = trans ? A.rows() : A.cols(), nout = trans ? A.cols() : A.rows();
-  if (q >= 2 && A.has_rows()) {
-    const std::size_t e = p == Prec::F64 ? 8 : 4;
-    if (rin.bytes < (std::size_t)nin * q * e)
-      rin = DBuf((std::size_t)nin * q * e);
-    if (rout.bytes < (std::size_t)nout * q * e)
-      rout = DBuf((std::size_t)nout * q * e);
+  auto by_cols = [&] {
+    if (trans)
+      A.mtm(c, p, q, X, ldx, Y, ldy, mask);
+    else
+      A.mm(c, p, q, X, ldx, Y, ldy, mask);
+  };
+  if (q < 2 || !A.has_rows()) {
+    by_cols();
+    return;
+  }
+  const std::size_t e = p == Prec::F64 ? 8 : 4;
+  if (rin.bytes < (std::size_t)nin * q * e)
+    rin = DBuf((std::size_t)nin * q * e);
+  if (rout.bytes < (std::size_t)nout * q * e)
+    rout = DBuf((std::size_t)nout * q * e);
+  auto by_rows = [&] {
     cols_to_rows(c, p, nin, q, X, ldx, rin.p);
     A.mm_rows(c, p, q, rin.p, rout.p, trans);
     rows_to_cols(c, p, nout, q, rout.p, Y, ldy);
-    return;
+  };
+  std::atomic<signed char> *cache = A.block_path(p, trans, q);
+  signed char path = cache ? cache->load(std::memory_order_relaxed) : 2;
+  if (path == 0) {  // first product of this kind: time both (second run of each)
+    auto timed = [&](auto &&fn) {
+      fn();
+      hipEvent_t e0, e1;
+      KH_CHECK(hipEventCreate(&e0));
+      KH_CHECK(hipEventCreate(&e1));
+      KH_CHECK(hipEventRecord(e0, c.stream));
+      fn();
+      KH_CHECK(hipEventRecord(e1, c.stream));
+      KH_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      KH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return ms;
+    };
+    const float t_cols = timed(by_cols), t_rows = timed(by_rows);
+    path = t_rows < t_cols ? 2 : 1;
+    cache->store(path, std::memory_order_relaxed);
+    return;  // Y holds the (identical) product of the last run
   }
-  if (trans)
-    A.mtm(c, p, q, X, ldx, Y, ldy, mask);
+  if (path == 2)
+    by_rows();
   else
-    A.mm(c, p, q, X, ldx, Y, ldy, mask);
+    by_cols();
 }
 
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
@@ -902,6 +1037,7 @@ double cyclic_footprint(int64_t m, int64_t n, const CyclicParams &P) {
   b += w * 8.0 * (5.0 * m + n);                        // CG state (+ row-major block)
   b += w * 8.0 * (m + n);                              // row-major block products
   b += (P.polish == Prec::F32 ? w * 8.0 * m : 0.0);
+  b += (P.krylov == Prec::F32 ? w * 4.0 * (n + m) : 0.0);  // FP32 window screen
   return b + (double)(64 << 20);                       // scalars, partials, slack
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <atomic>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -106,6 +107,12 @@ struct DeviceCsr {
   double fro = 0.0;  // exact ||A||_F from the CSR values
   std::vector<double> diag_AAt_host;
   DBuf diag_AAt;  // row sums of squares, for the Jacobi preconditioner
+  // Block products of two or more columns: row-major (one gather of a
+  // contiguous row per nonzero, plus two transposes) or column by column,
+  // whichever was faster when first timed, per [FP64|FP32][A|A^T][q <= 8|q > 8]
+  // (0 = not timed yet, 1 = column by column, 2 = row-major). Both give
+  // bitwise the same products.
+  mutable std::atomic<signed char> block_path[2][2][2]{};
 };
 
 // Build from host canonical CSR (validated). Throws std::invalid_argument
@@ -187,6 +194,12 @@ void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int6
 // FP64 blocks: the same pass on FP64 MFMA (v_mfma_f64_16x16x4f64).
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
                 int q, int64_t len, double *SG);
+// H (s x q, FP64 out, device) = V^T W for FP32 V, W with s, q <= 128 on the
+// matrix cores (s, q <= 16: FP64 MFMA on the exactly converted values;
+// otherwise FP32 MFMA accumulation within 32-row slabs, FP64 across slabs,
+// about FP32 accuracy), for screening; other shapes go through gemm_tn.
+void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
+                      int64_t ldw, int q, int64_t len, double *H);
 void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
                int q, int64_t len, double *H);
 // W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.

@@ -1197,8 +1197,9 @@ __global__ void __launch_bounds__(kG2Threads)
   }
 }
 
-// H = V^T W for narrow FP64 blocks (s, q <= 16: one 16 x 16 MFMA tile), the
-// window-against-basis products of the cyclic solver at small k. A block of
+// H = V^T W for narrow blocks (s, q <= 16: one 16 x 16 MFMA tile) of FP64 or
+// FP32 (converted exactly) input, the window-against-basis products of the
+// cyclic solver at small k, on FP64 MFMA. A block of
 // 8 waves stages 256-row slabs of both operands in LDS (16 coalesced loads
 // in flight per thread, the next slab loading while this one is consumed);
 // wave w runs the MFMA steps of slab rows [32w, 32w + 32), and the 8 wave
@@ -1206,9 +1207,10 @@ __global__ void __launch_bounds__(kG2Threads)
 // k_gram2_partial with one tile (256 values per block), for k_tn2_finish.
 constexpr int kTnSlab = 256, kTnLd = kTnSlab + 1;
 
+template <class T>
 __global__ void __launch_bounds__(kG2Threads)
-    k_tn_narrow(int64_t len, int64_t rows_per_block, const double *__restrict__ V, int64_t ldv,
-                int s, const double *__restrict__ W, int64_t ldw, int q,
+    k_tn_narrow(int64_t len, int64_t rows_per_block, const T *__restrict__ V, int64_t ldv,
+                int s, const T *__restrict__ W, int64_t ldw, int q,
                 double *__restrict__ part) {
   __shared__ double Vs[16 * kTnLd];
   __shared__ double Ws[16 * kTnLd];
@@ -1218,14 +1220,16 @@ __global__ void __launch_bounds__(kG2Threads)
   const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
   // element u of this thread: column 2u + (tid >= 256), row tid % 256
   const int rl = threadIdx.x & (kTnSlab - 1), chi = threadIdx.x >> 8;
-  double va[8], wa[8];
+  // raw values stay in registers (in flight during the MFMA steps of the
+  // current slab) and are converted on the LDS store
+  T va[8], wa[8];
   auto load = [&](int64_t rb) {
     const int64_t row = (rb + rl) < r1 ? (rb + rl) : (r1 - 1);
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int c = 2 * u + chi;
-      va[u] = c < s ? V[row + (int64_t)c * ldv] : 0.0;
-      wa[u] = c < q ? W[row + (int64_t)c * ldw] : 0.0;
+      va[u] = c < s ? V[row + (int64_t)c * ldv] : (T)0;
+      wa[u] = c < q ? W[row + (int64_t)c * ldw] : (T)0;
     }
   };
   doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
@@ -1238,8 +1242,8 @@ __global__ void __launch_bounds__(kG2Threads)
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int c = 2 * u + chi;
-      Vs[c * kTnLd + rl] = inr ? va[u] : 0.0;
-      Ws[c * kTnLd + rl] = inr ? wa[u] : 0.0;
+      Vs[c * kTnLd + rl] = inr ? (double)va[u] : 0.0;
+      Ws[c * kTnLd + rl] = inr ? (double)wa[u] : 0.0;
     }
     __syncthreads();
     if (rb + kTnSlab < r1)
@@ -2213,14 +2217,14 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     throw std::invalid_argument("gram2_mfma: more than 128 columns");
   const int ns = (s + 15) / 16, nq = (q + 15) / 16;
   const int ntiles = mode == 0 ? nq * (nq + 1) / 2 + nq * nq : ns * nq;
-  if (mode == 1 && sizeof(T) == 8 && s <= 16 && q <= 16) {
+  if (mode == 1 && s <= 16 && q <= 16) {
     // one tile: 256-row slabs, about 2048 blocks
     int64_t rpb = (len + 2047) / 2048;
     rpb = std::max<int64_t>((rpb + kTnSlab - 1) / kTnSlab * kTnSlab, kTnSlab);
     const int64_t nblk = (len + rpb - 1) / rpb;
     double *part = ctx.partials((std::size_t)nblk * 256);
-    hipLaunchKernelGGL(k_tn_narrow, dim3((unsigned)nblk), dim3(kG2Threads), 0, ctx.stream, len,
-                       rpb, (const double *)Z, ldz, s, (const double *)AZ, ldaz, q, part);
+    hipLaunchKernelGGL(k_tn_narrow<T>, dim3((unsigned)nblk), dim3(kG2Threads), 0, ctx.stream,
+                       len, rpb, Z, ldz, s, AZ, ldaz, q, part);
     const int64_t outs = (int64_t)s * q * 64;  // one wave per output
     hipLaunchKernelGGL(k_tn2_finish<double>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, ctx.stream, part, (int)nblk, s, q, SG);
@@ -2259,6 +2263,16 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
 void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
                 int64_t len, double *SG) {
   launch_gram2(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
+}
+
+void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
+                      int64_t ldw, int q, int64_t len, double *H) {
+  if (s <= 0 || q <= 0)
+    return;
+  if (q > 4 && s <= kG2MaxQ && q <= kG2MaxQ)
+    launch_gram2(ctx, V, ldv, s, W, ldw, q, len, H, 1);
+  else
+    gemm_tn(ctx, Prec::F32, V, ldv, s, Prec::F32, W, ldw, q, len, H);
 }
 
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,

@@ -37,6 +37,12 @@ struct DeviceOp {
     (void)c, (void)p, (void)s, (void)X, (void)Y, (void)trans;
     throw std::logic_error("row-major block products not provided by this operator");
   }
+  // The cached choice between row-major and column-by-column block products
+  // for op(A) in precision p on q columns, or nullptr (no cache: row-major).
+  virtual std::atomic<signed char> *block_path(Prec p, bool trans, int q) const {
+    (void)p, (void)trans, (void)q;
+    return nullptr;
+  }
   // diag(A A^T) on the device (Jacobi preconditioner), or nullptr.
   virtual const double *diag_AAt() const { return nullptr; }
   virtual double fro() const { return 0.0; }
@@ -68,6 +74,9 @@ struct CsrOp final : DeviceOp {
   bool has_rows() const override { return true; }
   void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const override {
     spmm_rows(c, A, t != trans, p, s, X, Y);
+  }
+  std::atomic<signed char> *block_path(Prec p, bool trans, int q) const override {
+    return &A.block_path[p == Prec::F64 ? 0 : 1][t != trans ? 1 : 0][q > 8 ? 1 : 0];
   }
   const double *diag_AAt() const override { return t ? nullptr : A.diag_AAt.as<double>(); }
   double fro() const override { return A.fro; }
@@ -135,6 +144,7 @@ struct CyclicStats {
   int64_t cg_solves = 0, cg_maxit_hits = 0;
   int64_t spmv = 0, spmtv = 0;    // operator applications (column count)
   int64_t cg_iters = 0;           // CG iterations over all columns
+  int64_t screened = 0;           // candidates rejected by the FP32 prescreen (mixed)
   PhaseTimes t;
 };
 
@@ -144,9 +154,11 @@ struct CyclicStats {
 // states once per chunk of iterations. Returns the per-column states.
 // Y = op(A) X on q columns (column-major, leading dimensions ldx / ldy; op =
 // A or A^T). With two or more columns and row-major products available the
-// block goes through row-major copies (rin / rout grown as needed); every
-// column is then computed, so columns that `mask` marks inactive may be
-// overwritten. Otherwise op's mm / mtm with the mask.
+// block goes either through row-major copies (rin / rout grown as needed;
+// every column is computed, so columns that `mask` marks inactive may be
+// overwritten) or column by column through op's mm / mtm with the mask,
+// whichever the operator's cache says was faster; the first product of a
+// kind runs and times both (bitwise the same results). Otherwise mm / mtm.
 void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, int64_t ldx,
                    void *Y, int64_t ldy, bool trans, const int *mask, DBuf &rin, DBuf &rout);
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
