@@ -576,7 +576,9 @@ DeviceBasis CyclicDriver::run() {
       const Group &g0 = groups.front();
       if (passes == 2 && g0.count > 0) {
         // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
-        // fused; w -= V h2 with ||w||. Same arithmetic as two cgs_pass calls.
+        // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
+        // except that h2's partial sums are grouped per block of the fused
+        // kernel.
         gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
         gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
         gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);

@@ -383,6 +383,96 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// The middle of a classical Gram-Schmidt pair with the block's tile of V in
+// LDS: a block owns 128 rows and all s columns; thread t loads row t % 128 of
+// column half t / 128 (every load issued before any use), the tile is stored
+// to LDS (column-major, stride 129), threads < 128 then update their row's w
+// exactly as k_vh_sub does (sequential sum over all columns in order), and
+// thread c sums column c of the tile against the new w in row order. HBM sees
+// V once. With MULTI a block walks tiles blockIdx.x, blockIdx.x + gridDim.x,
+// ... (the next tile's loads in flight while the current one is processed)
+// and accumulates each column over its tiles in that order, so the partials
+// of V^T w are one per block: part[c * gridDim.x + block]. Without MULTI the
+// grid must cover every tile (one each; the prefetch and its registers are
+// compiled out).
+constexpr int kLdsRows = 128, kLdsStride = kLdsRows + 1;
+
+template <class TV, class TW, int HMAX, bool MULTI>
+__global__ void __launch_bounds__(kBlock)
+    k_vh_sub_vtw_lds(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                     const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
+                     int nb) {
+  extern __shared__ double sh[];
+  double *hs = sh;               // [s]
+  double *wn = hs + s;           // [kLdsRows]
+  TV *tile = reinterpret_cast<TV *>(wn + kLdsRows);  // [s][kLdsStride]
+  const int t = threadIdx.x, rl = t & (kLdsRows - 1), half = t / kLdsRows;
+  for (int j = t; j < s; j += kBlock)
+    hs[j] = h[j];
+  const int sh2 = (s + 1) >> 1;
+  const int c_lo = half ? sh2 : 0, c_hi = half ? s : sh2;
+  const int64_t ntiles = (len + kLdsRows - 1) / kLdsRows;
+  double colsum = 0.0;  // thread c's column over this block's tiles
+  // the next tile's loads are issued as soon as the current tile is in LDS,
+  // so they are in flight while its two passes run
+  TV x[HMAX];
+  TW wnext = (TW)0;
+  auto load = [&](int64_t ti) {
+    const int64_t r = ti * kLdsRows + rl;
+    const TV *base = V + (r < len ? r : len - 1);
+    // an opaque copy of ld (empty asm, scalar register) keeps the compiler
+    // from hoisting the HMAX column offsets out of the tile loop into vector
+    // registers
+    int64_t ldo = ld;
+    __asm__ volatile("" : "+s"(ldo));
+#pragma unroll
+    for (int u = 0; u < HMAX; ++u) {
+      const int c = c_lo + u;
+      const int cc = c < c_hi ? c : (c_hi > c_lo ? c_hi - 1 : 0);
+      x[u] = base[(int64_t)cc * ldo];
+    }
+    if (half == 0)
+      wnext = w[r < len ? r : len - 1];
+  };
+  if ((int64_t)blockIdx.x < ntiles)
+    load(blockIdx.x);
+  for (int64_t tile_i = blockIdx.x; tile_i < ntiles; tile_i += gridDim.x) {
+    const int64_t r = tile_i * kLdsRows + rl;
+    __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+    for (int u = 0; u < HMAX; ++u) {
+      const int c = c_lo + u;
+      if (c < c_hi)
+        tile[c * kLdsStride + rl] = x[u];
+    }
+    const TW wold = wnext;
+    __syncthreads();
+    if (MULTI && tile_i + (int64_t)gridDim.x < ntiles)
+      load(tile_i + gridDim.x);
+    if (half == 0) {
+      double acc = 0.0;
+#pragma unroll 2
+      for (int c = 0; c < s; ++c)
+        acc += (double)tile[c * kLdsStride + rl] * hs[c];
+      const TW nw = (TW)((double)wold - acc);
+      if (r < len)
+        w[r] = nw;
+      wn[rl] = r < len ? (double)nw : 0.0;
+    }
+    __syncthreads();
+    if (t < s) {  // s <= kBlock
+      const TV *tc = tile + t * kLdsStride;
+      double a = 0.0;
+#pragma unroll 4
+      for (int i = 0; i < kLdsRows; ++i)
+        a += (double)tc[i] * wn[i];
+      colsum += a;
+    }
+  }
+  if (t < s)
+    part[(int64_t)t * nb + blockIdx.x] = colsum;
+}
+
 // The middle of a classical Gram-Schmidt pair in one read of V: w -= V h
 // (the update of pass 1, exactly as k_vh_sub computes it) and then the block
 // partials of V^T w for the updated w (the projection of pass 2, exactly as
@@ -1956,6 +2046,39 @@ void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
 template <class TV, class TW>
 static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, const double *h,
                                 TW *w, int64_t len, double *out) {
+  // LDS-tiled form when its tile fits 64 KB and the columns fill it (below
+  // about 12 columns the two-sweep kernel is faster)
+  const std::size_t lds = (std::size_t)s * 8 + kLdsRows * 8 +
+                          (std::size_t)s * kLdsStride * sizeof(TV);
+  // one 128-row tile per block up to 16384 tiles (2^21 rows); beyond, 16384
+  // blocks each take every 16384-th tile, which bounds the partials per
+  // column that k_finish sums
+  const int64_t ntl = (len + kLdsRows - 1) / kLdsRows;
+  const int64_t nbl = std::min<int64_t>(ntl, 16384);
+  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * s <= ctx.part_cap) {
+    double *part = ctx.partials((std::size_t)nbl * s);
+    {
+      TraceScope tr(ctx, kTraceVhSub,
+                    (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * nbl * s);
+      const int hm = (s + 1) / 2;
+      const bool multi = ntl > nbl;
+#define KH_LDS_LAUNCH(H, M)                                                                    \
+  hipLaunchKernelGGL((k_vh_sub_vtw_lds<TV, TW, H, M>), dim3((unsigned)nbl), dim3(kBlock), lds, \
+                     ctx.stream, len, V, ld, s, h, w, part, (int)nbl)
+      if (hm <= 16)
+        { if (multi) KH_LDS_LAUNCH(16, true); else KH_LDS_LAUNCH(16, false); }
+      else if (hm <= 32)
+        { if (multi) KH_LDS_LAUNCH(32, true); else KH_LDS_LAUNCH(32, false); }
+      else if (hm <= 48)
+        { if (multi) KH_LDS_LAUNCH(48, true); else KH_LDS_LAUNCH(48, false); }
+      else
+        { if (multi) KH_LDS_LAUNCH(64, true); else KH_LDS_LAUNCH(64, false); }
+#undef KH_LDS_LAUNCH
+    }
+    hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl, out, 0);
+    KH_CHECK(hipGetLastError());
+    return;
+  }
   const int nb = vtw_blocks(len);
   double *part = ctx.partials((std::size_t)nb * s);
   {

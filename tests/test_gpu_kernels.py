@@ -132,8 +132,11 @@ def test_spmm_rows_bit_exact(s, transpose):
 @pytest.mark.parametrize("s,n,prec", [(1, 5000, 0), (33, 100_003, 0), (97, 300_000, 1),
                                       (7, 2049, 1)])
 def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
-    """The fused middle kernel (w -= V h1 and V^T w in one read of V) gives
-    bitwise the results of the separate update and projection sweeps."""
+    """The fused middle kernel (w -= V h1 and V^T w from one read of V) gives
+    the results of the separate update and projection sweeps: the update of
+    w is the same arithmetic; the second projection's partial sums are grouped
+    per 128-row block in the LDS-tiled form (12 <= s), so h2 and the final w
+    agree to rounding there and bitwise below 12 columns."""
     print(f"[case] cgs2 fused s={s} n={n} prec={prec}", flush=True)
     rng = np.random.default_rng(s + n)
     V, _ = np.linalg.qr(rng.standard_normal((n, s)))
@@ -149,8 +152,19 @@ def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
                                         h.ctypes.data_as(C.c_void_p),
                                         nrm.ctypes.data_as(C.c_void_p)))
         outs.append((dw.get(), h, nrm))
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    if s < 12:
+        for a, b in zip(outs[0], outs[1]):
+            assert np.array_equal(a, b)
+        return
+    (w0, h0, n0), (w1, h1, n1) = outs
+    eps = np.finfo(dt).eps
+    wn = float(np.linalg.norm(w.astype(np.float64)))
+    assert np.abs(w0.astype(np.float64) - w1.astype(np.float64)).max() <= 4 * eps * wn
+    assert np.abs(h0 - h1).max() <= 1e-13 * wn
+    assert abs(n0[0] - n1[0]) <= 16 * eps * n0[0]
+    # and both are orthogonal to V to working precision
+    for wx in (w0, w1):
+        assert np.abs(V.astype(np.float64).T @ wx.astype(np.float64)).max() <= 64 * eps * wn
 
 
 @pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (16, 16, 1_000_003, 0), (11, 15, 257, 0),
