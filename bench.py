@@ -148,7 +148,7 @@ def main():
     # reorthogonalisation products, updates and sparse products bracketed by
     # HIP events (per-class breakdown), recording the (rows, columns) shape
     # of each launch of the dominant kernel, the FP32 V^T w product
-    # k_vtw_partial<float,float>. Its roofline: each shape the solve used is
+    # k_vtw4_partial (16-byte loads). Its roofline: each shape the solve used is
     # replayed back to back between two events on the library stream (an
     # event pair around a single launch would add the dispatch gap), and the
     # per-shape durations are weighted by the solve's launch counts.
@@ -176,7 +176,7 @@ def main():
                          "device_ms": round(ms_l, 1)}
     ms_spmv = H.time_spmv(transpose=False, precision="fp32", reps=200)
     ms_spmtv = H.time_spmv(transpose=True, precision="fp32", reps=200)
-    traffic = load_traffic("k_vtw_partial<float, float>")
+    traffic = load_traffic("k_vtw4_partial")
 
     cpu = None
     if not args.no_cpu:
@@ -215,7 +215,8 @@ def main():
                      "cg_solves": info_m.cg_solves, "phase_s": info_m.times,
                      "screened": info_m.screened},
         "roofline": {
-            "kernel": "k_vtw_partial<float, float> (reorthogonalisation V^T w, FP32 in, FP64 acc)",
+            "kernel": "k_vtw4_partial (reorthogonalisation V^T w, FP32 in with 16-byte loads, "
+                      "FP64 accumulation)",
             "bound": "hbm",
             "achieved": None if achieved is None else round(achieved, 1),
             "peak": peak, "unit": "GB/s",

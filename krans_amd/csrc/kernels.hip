@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -303,6 +304,85 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int k = 0; k < kR; ++k)
       acc += (double)x[k] * wv[k];
+    acc = wave_sum(acc);
+    if (lane == 0)
+      red[c * kWaves + wid] = acc;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < s; c += kBlock) {
+    double t = 0.0;
+    for (int w2 = 0; w2 < kWaves; ++w2)
+      t += red[c * kWaves + w2];
+    part[(int64_t)c * nb + blockIdx.x] = t;
+  }
+}
+
+// FP32 V^T w with 16-byte loads: a block owns 2048 rows as 512 float4
+// groups; thread t owns groups t and t + 256 (rows 4t..4t+3 and
+// 1024+4t..1024+4t+3 of the block), two columns per step. Same block count
+// and partial layout as k_vtw_partial; len and ld multiples of 4, V and w
+// 16-byte aligned.
+__global__ void __launch_bounds__(kBlock)
+    k_vtw4_partial(int64_t len, const float *__restrict__ V, int64_t ld, int s,
+                   const float *__restrict__ w, double *__restrict__ part, int nb) {
+  constexpr int G = 2, CU = 2;
+  extern __shared__ double red[];  // [s][kWaves]
+  const int64_t nq = len >> 2;
+  const int64_t q0 = (int64_t)blockIdx.x * kBlock * G + threadIdx.x;
+  int64_t qq[G];
+  double wv[G][4];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int64_t q = q0 + (int64_t)g * kBlock;
+    qq[g] = q < nq ? q : nq - 1;
+    const float4 t = reinterpret_cast<const float4 *>(w)[qq[g]];
+    const bool ok = q < nq;
+    wv[g][0] = ok ? (double)t.x : 0.0;
+    wv[g][1] = ok ? (double)t.y : 0.0;
+    wv[g][2] = ok ? (double)t.z : 0.0;
+    wv[g][3] = ok ? (double)t.w : 0.0;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int s_main = s - s % CU;
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    float4 x[CU][G];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const float4 *vc = reinterpret_cast<const float4 *>(V + (int64_t)(c0 + u) * ld);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        x[u][g] = vc[qq[g]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        acc += (double)x[u][g].x * wv[g][0];
+        acc += (double)x[u][g].y * wv[g][1];
+        acc += (double)x[u][g].z * wv[g][2];
+        acc += (double)x[u][g].w * wv[g][3];
+      }
+      acc = wave_sum(acc);
+      if (lane == 0)
+        red[(c0 + u) * kWaves + wid] = acc;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const float4 *vc = reinterpret_cast<const float4 *>(V + (int64_t)c * ld);
+    float4 x[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      x[g] = vc[qq[g]];
+    double acc = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      acc += (double)x[g].x * wv[g][0];
+      acc += (double)x[g].y * wv[g][1];
+      acc += (double)x[g].z * wv[g][2];
+      acc += (double)x[g].w * wv[g][3];
+    }
     acc = wave_sum(acc);
     if (lane == 0)
       red[c * kWaves + wid] = acc;
@@ -1942,6 +2022,23 @@ inline int vtw_blocks(int64_t len) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// The partial kernel of V^T w: 16-byte loads for aligned FP32 blocks.
+template <class TV, class TW>
+static void launch_vtw_partial(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
+                               int64_t len, double *part, int nb) {
+  const std::size_t lds = (std::size_t)s * kWaves * sizeof(double);
+  if constexpr (std::is_same<TV, float>::value && std::is_same<TW, float>::value) {
+    if (len % 4 == 0 && ld % 4 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(w) % 16 == 0) {
+      hipLaunchKernelGGL(k_vtw4_partial, dim3(nb), dim3(kBlock), lds, ctx.stream, len, V, ld, s,
+                         w, part, nb);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock), lds, ctx.stream, len, V, ld,
+                     s, w, part, nb);
+}
+
 template <class TV, class TW>
 static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
                           int64_t len, double *out) {
@@ -1953,9 +2050,7 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
       TraceScope tr(ctx, kTraceVtW,
                     (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc,
                     len, sc);
-      hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock),
-                         (std::size_t)sc * kWaves * sizeof(double), ctx.stream, len,
-                         V + (int64_t)c0 * ld, ld, sc, w, part, nb);
+      launch_vtw_partial(ctx, V + (int64_t)c0 * ld, ld, sc, w, len, part, nb);
     }
     hipLaunchKernelGGL(k_finish, dim3(sc), dim3(kBlock), 0, ctx.stream, part, nb, out + c0, 0);
   }
@@ -1972,9 +2067,7 @@ static double time_vtw_t(Context &ctx, int64_t len, int s, int reps) {
   KH_CHECK(hipMemsetAsync(V.p, 0, V.bytes, ctx.stream));
   KH_CHECK(hipMemsetAsync(w.p, 0, w.bytes, ctx.stream));
   auto launch = [&] {
-    hipLaunchKernelGGL((k_vtw_partial<T, T>), dim3(nb), dim3(kBlock),
-                       (std::size_t)s * kWaves * sizeof(double), ctx.stream, len,
-                       (const T *)V.p, len, s, (const T *)w.p, part, nb);
+    launch_vtw_partial(ctx, (const T *)V.p, len, s, (const T *)w.p, len, part, nb);
   };
   launch();
   hipEvent_t e0, e1;
