@@ -574,6 +574,7 @@ DeviceBasis CyclicDriver::run() {
       else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
         groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
       const Group &g0 = groups.front();
+      bool stored = false;  // w / beta_j already written to its basis column
       if (passes == 2 && g0.count > 0) {
         // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
         // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
@@ -581,7 +582,13 @@ DeviceBasis CyclicDriver::run() {
         // kernel.
         gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
         gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
-        gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+        if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
+          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
+                               bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
+          stored = true;
+        } else {
+          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+        }
       } else {
         for (int pass = 0; pass < passes; ++pass)
           cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
@@ -590,7 +597,8 @@ DeviceBasis CyclicDriver::run() {
         dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
       if (nyc + nxc >= ncv)
         break;
-      scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
+      if (!stored)
+        scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
       if (from_y)
         ++nxc;
       else

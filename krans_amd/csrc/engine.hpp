@@ -167,6 +167,11 @@ double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps);
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
             const void *w, int64_t len, double *out);
 // w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
+// w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
+// with the norm followed by scale_copy_dev), the norm and copy in one launch.
+void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
+                          const double *h, Prec pw, void *w, int64_t len, double *nrm,
+                          void *dst);
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                 const double *h, Prec pw, void *w, int64_t len,
                 double *nrm2_out = nullptr);

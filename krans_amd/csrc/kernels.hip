@@ -231,7 +231,18 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ double lds[kWaves];
   const double *pc = part + (int64_t)blockIdx.x * nb;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kBlock)
+  int i = threadIdx.x;
+  // eight loads in flight before their (in-order) sums
+  for (; i + 7 * kBlock < nb; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = pc[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      acc += v[u];
+  }
+  for (; i < nb; i += kBlock)
     acc += pc[i];
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0)
@@ -1632,6 +1643,43 @@ __global__ void __launch_bounds__(kBlock)
     y[i] = x[i] * s;
 }
 
+// ||w|| from the block partials of k_vh_sub (every block forms the same sum
+// as k_finish does, in the same order) and y = x * (1 / ||w||) as
+// k_scale_copy_dev computes it; block 0 stores the norm. One launch for the
+// norm and the normalised copy of a Golub-Kahan step.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_norm_scale_copy(const double *__restrict__ part, int nb, double *__restrict__ nrm,
+                      int64_t len, const T *__restrict__ x, T *__restrict__ y) {
+  __shared__ double lds[kWaves];
+  __shared__ double beta;
+  double acc = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * kBlock < nb; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = part[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      acc += v[u];
+  }
+  for (; i < nb; i += kBlock)
+    acc += part[i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(fmax(0.0, acc));
+    beta = b;
+    if (blockIdx.x == 0)
+      *nrm = b;
+  }
+  __syncthreads();
+  const T sc = (T)(1.0 / beta);
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < len;
+       j += (int64_t)gridDim.x * kBlock)
+    y[j] = x[j] * sc;
+}
+
 template <class TI, class TO>
 __global__ void __launch_bounds__(kBlock)
     k_convert(int64_t len, const TI *__restrict__ x, TO *__restrict__ y) {
@@ -2107,7 +2155,8 @@ void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
 
 template <class TV, class TW>
 static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
-                              const double *h, TW *w, int64_t len, double *nrm) {
+                              const double *h, TW *w, int64_t len, double *nrm,
+                              TW *dst = nullptr) {
   const int nb = vtw_blocks(len);
   double *part = nrm ? ctx.partials(nb) : nullptr;
   {
@@ -2117,9 +2166,32 @@ static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
                      (std::size_t)std::max(s, 1) * sizeof(double), ctx.stream, len, V, ld, s, h,
                      w, part);
   }
-  if (nrm)
+  // every block of the fused norm-and-copy re-reads the nb partials: used up
+  // to 1024 partials (2^21 rows) on at most 2048 blocks (<= 16 MB of L2
+  // reads); longer vectors take the norm and the copy as two launches
+  if (nrm && dst && nb <= 1024) {
+    hipLaunchKernelGGL(k_norm_scale_copy<TW>, dim3(std::min(grid_for(len), 2048)), dim3(kBlock),
+                       0, ctx.stream, part, nb, nrm, len, (const TW *)w, dst);
+  } else if (nrm) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, part, nb, nrm, 1);
+    if (dst)
+      hipLaunchKernelGGL(k_scale_copy_dev<TW>, dim3(grid_for(len)), dim3(kBlock), 0, ctx.stream,
+                         len, (const double *)nrm, (const TW *)w, dst);
+  }
   KH_CHECK(hipGetLastError());
+}
+
+void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
+                          const double *h, Prec pw, void *w, int64_t len, double *nrm,
+                          void *dst) {
+  if (pv == Prec::F64 && pw == Prec::F64)
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm, (double *)dst);
+  else if (pv == Prec::F32 && pw == Prec::F32)
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm, (float *)dst);
+  else if (pv == Prec::F64)
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm, (float *)dst);
+  else
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm, (double *)dst);
 }
 
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
