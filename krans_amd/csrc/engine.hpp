@@ -176,9 +176,10 @@ void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                 const double *h, Prec pw, void *w, int64_t len,
                 double *nrm2_out = nullptr);
 // w -= V h, then out = V^T w for the updated w (FP64): the middle of a CGS
-// pair with one read of V; same results as gemv_n_sub followed by gemv_t.
+// pair with one read of V; the update as gemv_n_sub, out as gemv_t up to the
+// grouping of its partial sums. With nrm1, also ||w|| after the update.
 void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
-                  Prec pw, void *w, int64_t len, double *out);
+                  Prec pw, void *w, int64_t len, double *out, double *nrm1 = nullptr);
 // out = V h (overwrite)
 void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
             const double *h, Prec po, void *out, int64_t len);

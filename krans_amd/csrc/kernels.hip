@@ -492,7 +492,7 @@ template <class TV, class TW, int HMAX, bool MULTI>
 __global__ void __launch_bounds__(kBlock)
     k_vh_sub_vtw_lds(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
                      const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
-                     int nb) {
+                     int nb, double *__restrict__ npart) {
   extern __shared__ double sh[];
   double *hs = sh;               // [s]
   double *wn = hs + s;           // [kLdsRows]
@@ -504,6 +504,7 @@ __global__ void __launch_bounds__(kBlock)
   const int c_lo = half ? sh2 : 0, c_hi = half ? s : sh2;
   const int64_t ntiles = (len + kLdsRows - 1) / kLdsRows;
   double colsum = 0.0;  // thread c's column over this block's tiles
+  double nsum = 0.0;    // with npart: ||updated w||^2 over this block's rows (wave 2)
   // the next tile's loads are issued as soon as the current tile is in LDS,
   // so they are in flight while its two passes run
   TV x[HMAX];
@@ -559,9 +560,17 @@ __global__ void __launch_bounds__(kBlock)
         a += (double)tc[i] * wn[i];
       colsum += a;
     }
+    if (npart && t >= 128 && t < 192) {  // rows l and l + 64, then a fixed wave tree
+      const int l = t - 128;
+      double v = wn[l] * wn[l] + wn[l + 64] * wn[l + 64];
+      v = wave_sum(v);
+      nsum += v;
+    }
   }
   if (t < s)
     part[(int64_t)t * nb + blockIdx.x] = colsum;
+  if (npart && t == 128)
+    npart[blockIdx.x] = nsum;
 }
 
 // The middle of a classical Gram-Schmidt pair in one read of V: w -= V h
@@ -2210,7 +2219,7 @@ void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
 
 template <class TV, class TW>
 static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, const double *h,
-                                TW *w, int64_t len, double *out) {
+                                TW *w, int64_t len, double *out, double *nrm1) {
   // LDS-tiled form when its tile fits 64 KB and the columns fill it (below
   // about 12 columns the two-sweep kernel is faster)
   const std::size_t lds = (std::size_t)s * 8 + kLdsRows * 8 +
@@ -2220,8 +2229,9 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
   // column that k_finish sums
   const int64_t ntl = (len + kLdsRows - 1) / kLdsRows;
   const int64_t nbl = std::min<int64_t>(ntl, 16384);
-  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * s <= ctx.part_cap) {
-    double *part = ctx.partials((std::size_t)nbl * s);
+  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * (s + 1) <= ctx.part_cap) {
+    double *part = ctx.partials((std::size_t)nbl * (s + 1));
+    double *npart = nrm1 ? part + (std::size_t)nbl * s : nullptr;
     {
       TraceScope tr(ctx, kTraceVhSub,
                     (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * nbl * s);
@@ -2229,7 +2239,7 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
       const bool multi = ntl > nbl;
 #define KH_LDS_LAUNCH(H, M)                                                                    \
   hipLaunchKernelGGL((k_vh_sub_vtw_lds<TV, TW, H, M>), dim3((unsigned)nbl), dim3(kBlock), lds, \
-                     ctx.stream, len, V, ld, s, h, w, part, (int)nbl)
+                     ctx.stream, len, V, ld, s, h, w, part, (int)nbl, npart)
       if (hm <= 16)
         { if (multi) KH_LDS_LAUNCH(16, true); else KH_LDS_LAUNCH(16, false); }
       else if (hm <= 32)
@@ -2241,6 +2251,8 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
 #undef KH_LDS_LAUNCH
     }
     hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl, out, 0);
+    if (nrm1)
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, npart, (int)nbl, nrm1, 1);
     KH_CHECK(hipGetLastError());
     return;
   }
@@ -2256,22 +2268,25 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
   }
   hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
   KH_CHECK(hipGetLastError());
+  if (nrm1)  // after the finish: the dot reuses the partial buffer
+    dot(ctx, std::is_same<TW, double>::value ? Prec::F64 : Prec::F32, w,
+        std::is_same<TW, double>::value ? Prec::F64 : Prec::F32, w, len, nrm1, true);
 }
 
 void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
-                  Prec pw, void *w, int64_t len, double *out) {
+                  Prec pw, void *w, int64_t len, double *out, double *nrm1) {
   if (s <= 0)
     return;
   if (s > 1024)
     throw std::invalid_argument("gemv_n_sub_t: more than 1024 columns");
   if (pv == Prec::F64 && pw == Prec::F64)
-    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (double *)w, len, out);
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (double *)w, len, out, nrm1);
   else if (pv == Prec::F32 && pw == Prec::F32)
-    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (float *)w, len, out);
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (float *)w, len, out, nrm1);
   else if (pv == Prec::F64)
-    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (float *)w, len, out);
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (float *)w, len, out, nrm1);
   else
-    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out);
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out, nrm1);
 }
 
 // k_nn_mfma when the block fits it (q <= 128 output columns, H^T in 64 KB

@@ -262,12 +262,21 @@ void LobpcgDriver::mgs_dev(Blk &X, int cnt, bool reorth, double *dn) {
   for (int i = 0; i < cnt; ++i) {
     void *xi = X.col(i);
     const int passes = reorth ? 2 : 1;
-    for (int ps = 0; ps < passes; ++ps) {
-      if (i > 0) {
-        gemv_t(c_, X.p, X.col(0), n_, i, X.p, xi, n_, sc_ + 16);
-        gemv_n_sub(c_, X.p, X.col(0), n_, i, sc_ + 16, X.p, xi, n_, dn + 2 * i + ps);
-      } else {
-        dot(c_, X.p, xi, X.p, xi, n_, dn + 2 * i + ps, true);
+    if (i > 0 && i <= 496 && reorth) {
+      // both passes with three reads of the previous columns: h1 = X^T x;
+      // [x -= X h1, ||x||, h2 = X^T x] in one sweep; x -= X h2 with ||x||
+      gemv_t(c_, X.p, X.col(0), n_, i, X.p, xi, n_, sc_ + 16);
+      // h1 in sc_[16, 528), h2 in sc_[528, 1040): i <= 512 columns
+      gemv_n_sub_t(c_, X.p, X.col(0), n_, i, sc_ + 16, X.p, xi, n_, sc_ + 528, dn + 2 * i);
+      gemv_n_sub(c_, X.p, X.col(0), n_, i, sc_ + 528, X.p, xi, n_, dn + 2 * i + 1);
+    } else {
+      for (int ps = 0; ps < passes; ++ps) {
+        if (i > 0) {
+          gemv_t(c_, X.p, X.col(0), n_, i, X.p, xi, n_, sc_ + 16);
+          gemv_n_sub(c_, X.p, X.col(0), n_, i, sc_ + 16, X.p, xi, n_, dn + 2 * i + ps);
+        } else {
+          dot(c_, X.p, xi, X.p, xi, n_, dn + 2 * i + ps, true);
+        }
       }
     }
     scale_dev(c_, X.p, n_, dn + 2 * i + passes - 1, true, xi);
