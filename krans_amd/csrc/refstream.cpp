@@ -2,6 +2,8 @@
 // values of std::normal_distribution<double>(0, 1) driven by
 // std::mt19937(seed), as libstdc++ produces them, bit for bit.
 //
+// The generator is MT19937 with std::mt19937's seeding (checked output for
+// output against std::mt19937), run a state block at a time.
 // libstdc++'s normal_distribution (bits/random.tcc) is the Marsaglia polar
 // method: each attempt draws two uniforms u = generate_canonical<double, 53>
 // (two 32-bit outputs w0, w1: (w0 + w1 * 2^32) / 2^64, clamped below 1),
@@ -57,9 +59,40 @@ void convert(const uint32_t *w, std::size_t c0, std::size_t c1, std::vector<doub
 }
 }  // namespace
 
+// std::mt19937's seeding: x_0 = seed, x_i = 1812433253 (x_{i-1} ^ (x_{i-1} >> 30)) + i.
+RefStream::Mt19937::Mt19937(uint32_t seed) {
+  mt_[0] = seed;
+  for (uint32_t i = 1; i < 624; ++i)
+    mt_[i] = 1812433253u * (mt_[i - 1] ^ (mt_[i - 1] >> 30)) + i;
+}
+
+// One twist of the whole state (the first 227 words read the old words
+// 397 ahead, the rest the already twisted words 227 behind), then the
+// tempering of every word: outputs 624k .. 624k + 623 of the generator.
+void RefStream::Mt19937::block(uint32_t *out) {
+  constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu, kA = 0x9908b0dfu;
+  auto twist = [](uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t y = (a & kUpper) | (b & kLower);
+    return c ^ (y >> 1) ^ ((b & 1u) ? kA : 0u);
+  };
+  int i = 0;
+  for (; i < 624 - 397; ++i)
+    mt_[i] = twist(mt_[i], mt_[i + 1], mt_[i + 397]);
+  for (; i < 623; ++i)
+    mt_[i] = twist(mt_[i], mt_[i + 1], mt_[i + 397 - 624]);
+  mt_[623] = twist(mt_[623], mt_[0], mt_[396]);
+  for (int k = 0; k < 624; ++k) {
+    uint32_t y = mt_[k];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    out[k] = y;
+  }
+}
+
 RefStream::RefStream(unsigned seed, std::size_t prefetch)
-    : gen_(seed), dist_(0.0, 1.0), ring_(std::max<std::size_t>(prefetch, 4 * kPairs)),
-      cap_(ring_.size()) {
+    : gen_(seed), ring_(std::max<std::size_t>(prefetch, 4 * kPairs)), cap_(ring_.size()) {
   th_ = std::thread([this] { produce(); });
 }
 
@@ -80,9 +113,23 @@ void RefStream::produce() {
   std::vector<uint32_t> words[2] = {std::vector<uint32_t>(4 * kPairs),
                                     std::vector<uint32_t>(4 * kPairs)};
   std::vector<std::vector<double>> outs(nt);
-  auto draw = [this](std::vector<uint32_t> &w) {
-    for (auto &x : w)
-      x = (uint32_t)gen_();
+  // generator outputs come 624 at a time; `left` keeps the unused tail of
+  // the last block for the next batch
+  uint32_t blk[624];
+  int left = 0;
+  auto draw = [&](std::vector<uint32_t> &w) {
+    std::size_t k = 0;
+    const std::size_t n = w.size();
+    while (k < n) {
+      if (left == 0) {
+        gen_.block(blk);
+        left = 624;
+      }
+      const std::size_t take = std::min<std::size_t>((std::size_t)left, n - k);
+      std::copy(blk + (624 - left), blk + (624 - left) + take, w.data() + k);
+      k += take;
+      left -= (int)take;
+    }
   };
   auto push = [this](const std::vector<double> &o) {
     std::size_t done = 0;
@@ -92,11 +139,10 @@ void RefStream::produce() {
       if (stop_)
         return false;
       const std::size_t room = std::min(cap_ - count_, o.size() - done);
-      std::size_t tail = (head_ + count_) % cap_;
-      for (std::size_t i = 0; i < room; ++i) {
-        ring_[tail] = o[done + i];
-        tail = (tail + 1 == cap_) ? 0 : tail + 1;
-      }
+      const std::size_t tail = (head_ + count_) % cap_;
+      const std::size_t first = std::min(room, cap_ - tail);  // up to the ring's end
+      std::copy(o.begin() + (long)done, o.begin() + (long)(done + first), ring_.begin() + (long)tail);
+      std::copy(o.begin() + (long)(done + first), o.begin() + (long)(done + room), ring_.begin());
       count_ += room;
       done += room;
       lk.unlock();

@@ -98,10 +98,20 @@ public:
   RefStream &operator=(const RefStream &) = delete;
   void fill(double *out, std::size_t n);
 
+  // MT19937 (Matsumoto & Nishimura 1998) with std::mt19937's seeding: the
+  // same 32-bit outputs, produced 624 at a time (one twist of the state,
+  // then the tempering of every word).
+  class Mt19937 {
+  public:
+    explicit Mt19937(uint32_t seed);
+    void block(uint32_t *out);  // the next 624 outputs
+  private:
+    uint32_t mt_[624];
+  };
+
 private:
   void produce();
-  std::mt19937 gen_;
-  std::normal_distribution<double> dist_;
+  Mt19937 gen_;
   std::vector<double> ring_;
   std::size_t cap_ = 0, head_ = 0, count_ = 0;
   bool stop_ = false;
