@@ -168,6 +168,11 @@ void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
             const void *w, int64_t len, double *out);
 // w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
 // w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
+// R(:, j) = AX(:, j) - theta_j X(:, j), j < q (neg_theta: q device doubles,
+// -theta_j), columns with leading dimension ld: bitwise a copy of AX followed
+// by axpby(-theta_j, X, 1) per column, in one launch.
+void resid_block(Context &ctx, Prec p, int64_t len, int q, const void *AX, const void *X,
+                 int64_t ld, const double *neg_theta, void *R);
 // with the norm followed by scale_copy_dev), the norm and copy in one launch.
 void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                           const double *h, Prec pw, void *w, int64_t len, double *nrm,

@@ -1689,6 +1689,23 @@ __global__ void __launch_bounds__(kBlock)
     y[j] = x[j] * sc;
 }
 
+// R(:, j) = AX(:, j) + (T)(neg_theta[j]) X(:, j) for j < q: the LOBPCG
+// residual block in one launch, per element the rounding of a copy of AX
+// followed by axpby(-theta_j, X, 1) (product rounded, then the sum).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_resid_block(int64_t len, const T *__restrict__ AX, const T *__restrict__ X,
+                  const double *__restrict__ neg_theta, T *__restrict__ R, int64_t ld) {
+#pragma clang fp contract(off)
+  const int64_t off = (int64_t)blockIdx.y * ld;
+  const T a = (T)neg_theta[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * kBlock) {
+    const T prod = a * X[off + i];
+    R[off + i] = AX[off + i] + prod;
+  }
+}
+
 template <class TI, class TO>
 __global__ void __launch_bounds__(kBlock)
     k_convert(int64_t len, const TI *__restrict__ x, TO *__restrict__ y) {
@@ -2729,6 +2746,20 @@ void scale_copy_dev(Context &ctx, Prec p, int64_t len, const double *den, const 
   else
     hipLaunchKernelGGL(k_scale_copy_dev<float>, dim3(grid_for(len)), dim3(kBlock), 0,
                        ctx.stream, len, den, (const float *)x, (float *)y);
+  KH_CHECK(hipGetLastError());
+}
+
+void resid_block(Context &ctx, Prec p, int64_t len, int q, const void *AX, const void *X,
+                 int64_t ld, const double *neg_theta, void *R) {
+  if (len <= 0 || q <= 0)
+    return;
+  const dim3 grid(grid_for(len, 4), q);
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_resid_block<double>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (const double *)AX, (const double *)X, neg_theta, (double *)R, ld);
+  else
+    hipLaunchKernelGGL(k_resid_block<float>, grid, dim3(kBlock), 0, ctx.stream, len,
+                       (const float *)AX, (const float *)X, neg_theta, (float *)R, ld);
   KH_CHECK(hipGetLastError());
 }
 

@@ -819,10 +819,9 @@ DeviceBasis LobpcgDriver::run() {
     wblk_ = DBuf((std::size_t)B * m_ * 8);
   }
 
-  Blk X, AX, Rb, W, Pb, Z, AZ, scratch, Xn;
+  Blk X, AX, W, Pb, Z, AZ, scratch, Xn;
   X.init(n_, B, bp_);
   AX.init(n_, B, bp_);
-  Rb.init(n_, B, bp_);
   W.init(n_, B, bp_);
   Pb.init(n_, B, bp_);
   Z.init(n_, 3 * B, bp_);
@@ -873,10 +872,14 @@ DeviceBasis LobpcgDriver::run() {
   int64_t its = 0, locked_total = 0;
   while (locked_total < k_target && its < P_.max_iters) {
     ++its;
-    // residuals Rb = AX - theta X (:545-547)
-    for (int j = 0; j < B; ++j) {
-      dcopy(Rb.col(j), AX.col(j), cb);
-      axpby(c_, bp_, n_, -theta[(std::size_t)j], X.col(j), 1.0, Rb.col(j));
+    // residuals R = AX - theta X (:545-547), straight into W (W = R below,
+    // no preconditioner; W is not read before that)
+    {
+      std::vector<double> nt((std::size_t)B);
+      for (int j = 0; j < B; ++j)
+        nt[(std::size_t)j] = -theta[(std::size_t)j];
+      up(c_, nt.data(), B, sc_ + 50000);
+      resid_block(c_, bp_, n_, B, AX.col(0), X.col(0), n_, sc_ + 50000, W.col(0));
     }
     // ||A x_j|| for all j at once (X does not change within the loop)
     if (B >= 4 && A_.has_rows()) {
@@ -1053,24 +1056,17 @@ DeviceBasis LobpcgDriver::run() {
     }
 
     const int na = B - ini;
-    // W = R on the active slice (no M_inv), orthonormalised (:653-680)
-    dcopy(W.col(ini), Rb.col(ini), (std::size_t)na * cb);
-    {
-      dcopy(Z.col(0), W.col(ini), (std::size_t)na * cb);
-      if (!orthonormalize(Z, 0, na, P_.gs_tol, P_.gs_reorth, scratch)) {
-        for (int j = ini; j < B; ++j)
-          random_col(W.col(j), bp_);
-        for (int j = ini; j < B; ++j)
-          project_out(W.col(j), bp_);
-      } else {
-        dcopy(W.col(ini), Z.col(0), (std::size_t)na * cb);
-      }
+    // W = R on the active slice (no M_inv), orthonormalised in place (:653-680):
+    // on a failure W is replaced by random columns anyway, so the partially
+    // processed state orthonormalize leaves does not matter
+    if (!orthonormalize(W, ini, na, P_.gs_tol, P_.gs_reorth, scratch)) {
+      for (int j = ini; j < B; ++j)
+        random_col(W.col(j), bp_);
+      for (int j = ini; j < B; ++j)
+        project_out(W.col(j), bp_);
     }
-    if (its > 1) {  // (:683-691)
-      dcopy(Z.col(0), Pb.col(ini), (std::size_t)na * cb);
-      (void)orthonormalize(Z, 0, na, P_.gs_tol, P_.gs_reorth, scratch);
-      dcopy(Pb.col(ini), Z.col(0), (std::size_t)na * cb);
-    }
+    if (its > 1)  // (:683-691): P keeps whatever state orthonormalize leaves
+      (void)orthonormalize(Pb, ini, na, P_.gs_tol, P_.gs_reorth, scratch);
     // Z = [X_act | W_act | P_act] (:693-704)
     const int np0 = (its > 1) ? na : 0;
     dcopy(Z.col(0), X.col(ini), (std::size_t)na * cb);
