@@ -198,6 +198,9 @@ DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
     A->max_col = max_col;
     A->lpr_rows = pick_lanes(m ? (double)nnz / (double)m : 0.0, max_row);
     A->lpr_cols = pick_lanes(n ? (double)nnz / (double)n : 0.0, max_col);
+    // no row exceeds max_row, so nnz == m max_row means every row has max_row
+    A->uniform_rows = (m > 0 && nnz == (int64_t)m * max_row) ? max_row : 0;
+    A->uniform_cols = (n > 0 && nnz == (int64_t)n * max_col) ? max_col : 0;
     std::vector<float> va32((std::size_t)nnz), vt32((std::size_t)nnz);
     for (int64_t p = 0; p < nnz; ++p) {
       va32[(std::size_t)p] = (float)val[p];

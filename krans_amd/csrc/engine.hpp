@@ -104,6 +104,8 @@ struct DeviceCsr {
   DBuf perm;
   int lpr_rows = 1, lpr_cols = 1;  // lanes per row / per column (autotuned)
   int max_row = 0, max_col = 0;
+  // entries in every row / every column when all have the same count, else 0
+  int uniform_rows = 0, uniform_cols = 0;
   double fro = 0.0;  // exact ||A||_F from the CSR values
   std::vector<double> diag_AAt_host;
   DBuf diag_AAt;  // row sums of squares, for the Jacobi preconditioner

@@ -129,6 +129,108 @@ __global__ void __launch_bounds__(kBlock)
   y[i] = out;
 }
 
+// Rows of exactly R entries each (row pointers R i): row i's entries sit at
+// R i .. R i + R - 1, so the lane loads them without the row pointers (one
+// dependent memory latency fewer) and with vector loads where R allows. The
+// sums are those of row_dot (ascending from 0, no contraction): results are
+// bitwise those of k_spmv_lane / k_spmm_lane.
+template <class T, int R>
+__device__ __forceinline__ void ell_entries(int64_t i, const int32_t *__restrict__ idx,
+                                            const T *__restrict__ val, int32_t (&j)[R],
+                                            T (&v)[R]) {
+  const int64_t p = i * R;
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r += 4) {
+      const int4 a = *reinterpret_cast<const int4 *>(idx + p + r);
+      j[r] = a.x, j[r + 1] = a.y, j[r + 2] = a.z, j[r + 3] = a.w;
+    }
+  } else if constexpr (R % 2 == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r += 2) {
+      const int2 a = *reinterpret_cast<const int2 *>(idx + p + r);
+      j[r] = a.x, j[r + 1] = a.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      j[r] = idx[p + r];
+  }
+  if constexpr (std::is_same<T, float>::value && R % 4 == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r += 4) {
+      const float4 a = *reinterpret_cast<const float4 *>(val + p + r);
+      v[r] = a.x, v[r + 1] = a.y, v[r + 2] = a.z, v[r + 3] = a.w;
+    }
+  } else if constexpr (R % 2 == 0) {
+    using V2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+#pragma unroll
+    for (int r = 0; r < R; r += 2) {
+      const V2 a = *reinterpret_cast<const V2 *>(val + p + r);
+      v[r] = a.x, v[r + 1] = a.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      v[r] = val[p + r];
+  }
+}
+
+template <class T, int R>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv_ell(int64_t rows, const int32_t *__restrict__ idx, const T *__restrict__ val,
+               const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ z,
+               const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rows)
+    return;
+  const T b = z ? (T)(*beta) : (T)0;
+  const T zi = z ? z[i] : (T)0;
+  int32_t j[R];
+  T v[R], xv[R];
+  ell_entries<T, R>(i, idx, val, j, v);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    xv[r] = x[j[r]];
+  T out = (T)0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    out += v[r] * xv[r];
+  if (z)
+    out = out + (-b) * zi;
+  y[i] = out;
+}
+
+// Multi-RHS form: the row's entries are loaded once for all columns.
+template <class T, int R>
+__global__ void __launch_bounds__(kBlock)
+    k_spmm_ell(int64_t rows, const int32_t *__restrict__ idx, const T *__restrict__ val, int s,
+               const T *__restrict__ x, int64_t ldx, T *__restrict__ y, int64_t ldy,
+               const int *__restrict__ active) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rows)
+    return;
+  int32_t j[R];
+  T v[R];
+  ell_entries<T, R>(i, idx, val, j, v);
+  for (int c = 0; c < s; ++c) {
+    if (active && !active[c])
+      continue;
+    const T *xc = x + (int64_t)c * ldx;
+    T xv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      xv[r] = xc[j[r]];
+    T out = (T)0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      out += v[r] * xv[r];
+    y[i + (int64_t)c * ldy] = out;
+  }
+}
+
 // G lanes per row (G in 4,16,64): strided partial sums, shuffle tree.
 template <class T, int G>
 __global__ void __launch_bounds__(kBlock)
@@ -1987,13 +2089,37 @@ int reduce_blocks(int64_t len) {
   return (int)g;
 }
 
+// Entries per row when every row has the same count R and R is one the
+// fixed-length kernels take (1..4 or 8), else 0.
+static int ell_width(int uniform, const int32_t *idx, const void *val) {
+  if (!(uniform >= 1 && uniform <= 4) && uniform != 8)
+    return 0;
+  if (reinterpret_cast<uintptr_t>(idx) % 16 != 0 || reinterpret_cast<uintptr_t>(val) % 16 != 0)
+    return 0;
+  return uniform;
+}
+
+#define KH_ELL_LAUNCH(KERNEL, R, STREAM, ...)                                                \
+  hipLaunchKernelGGL((KERNEL<T, R>), dim3(grid_rows(rows)), dim3(kBlock), 0, STREAM, __VA_ARGS__)
+
 template <class T>
 static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
-                        const int32_t *idx, const T *val, int lpr, const T *x,
+                        const int32_t *idx, const T *val, int lpr, int ell, const T *x,
                         T *y, const T *z, const double *beta) {
   hipStream_t s = ctx.stream;
   if (rows <= 0)
     return;
+  if (ell > 0) {
+    switch (ell) {
+    case 1: KH_ELL_LAUNCH(k_spmv_ell, 1, s, rows, idx, val, x, y, z, beta); break;
+    case 2: KH_ELL_LAUNCH(k_spmv_ell, 2, s, rows, idx, val, x, y, z, beta); break;
+    case 3: KH_ELL_LAUNCH(k_spmv_ell, 3, s, rows, idx, val, x, y, z, beta); break;
+    case 4: KH_ELL_LAUNCH(k_spmv_ell, 4, s, rows, idx, val, x, y, z, beta); break;
+    default: KH_ELL_LAUNCH(k_spmv_ell, 8, s, rows, idx, val, x, y, z, beta); break;
+    }
+    KH_CHECK(hipGetLastError());
+    return;
+  }
   switch (lpr) {
   case 1:
     hipLaunchKernelGGL(k_spmv_lane<T>, dim3(grid_rows(rows)), dim3(kBlock), 0, s,
@@ -2019,28 +2145,46 @@ void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
           void *y, const void *z, const double *beta) {
   const int64_t rows = trans ? A.n : A.m;
   const double e = p == Prec::F64 ? 8.0 : 4.0;
-  TraceScope tr(ctx, kTraceSpmv,
-                (e + 4.0) * A.nnz + 4.0 * (rows + 1) + e * (trans ? A.m : A.n) +
-                    e * rows * (z ? 2.0 : 1.0));
   const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
   const int lpr = trans ? A.lpr_cols : A.lpr_rows;
-  if (p == Prec::F64)
-    launch_spmv<double>(ctx, rows, ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(),
-                        lpr, (const double *)x, (double *)y, (const double *)z, beta);
-  else
-    launch_spmv<float>(ctx, rows, ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(),
-                       lpr, (const float *)x, (float *)y, (const float *)z, beta);
+  const int uni = trans ? A.uniform_cols : A.uniform_rows;
+  // the fixed-length kernels read no row pointers
+  const void *vals = p == Prec::F64 ? (trans ? A.vt.p : A.va.p) : (trans ? A.vt32.p : A.va32.p);
+  const bool fixed = ell_width(uni, idx, vals) > 0;
+  TraceScope tr(ctx, kTraceSpmv,
+                (e + 4.0) * A.nnz + (fixed ? 0.0 : 4.0 * (rows + 1)) + e * (trans ? A.m : A.n) +
+                    e * rows * (z ? 2.0 : 1.0));
+  if (p == Prec::F64) {
+    const double *val = trans ? A.vt.as<double>() : A.va.as<double>();
+    launch_spmv<double>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val),
+                        (const double *)x, (double *)y, (const double *)z, beta);
+  } else {
+    const float *val = trans ? A.vt32.as<float>() : A.va32.as<float>();
+    launch_spmv<float>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val),
+                       (const float *)x, (float *)y, (const float *)z, beta);
+  }
 }
 
 template <class T>
 static void launch_spmm(Context &ctx, int64_t rows, const int32_t *ptr,
-                        const int32_t *idx, const T *val, int lpr, int s,
+                        const int32_t *idx, const T *val, int lpr, int ell, int s,
                         const T *x, int64_t ldx, T *y, int64_t ldy,
                         const int *act) {
   hipStream_t st = ctx.stream;
   if (rows <= 0 || s <= 0)
     return;
+  if (ell > 0) {
+    switch (ell) {
+    case 1: KH_ELL_LAUNCH(k_spmm_ell, 1, st, rows, idx, val, s, x, ldx, y, ldy, act); break;
+    case 2: KH_ELL_LAUNCH(k_spmm_ell, 2, st, rows, idx, val, s, x, ldx, y, ldy, act); break;
+    case 3: KH_ELL_LAUNCH(k_spmm_ell, 3, st, rows, idx, val, s, x, ldx, y, ldy, act); break;
+    case 4: KH_ELL_LAUNCH(k_spmm_ell, 4, st, rows, idx, val, s, x, ldx, y, ldy, act); break;
+    default: KH_ELL_LAUNCH(k_spmm_ell, 8, st, rows, idx, val, s, x, ldx, y, ldy, act); break;
+    }
+    KH_CHECK(hipGetLastError());
+    return;
+  }
   if (lpr == 1)
     hipLaunchKernelGGL(k_spmm_lane<T>, dim3(grid_rows(rows)), dim3(kBlock), 0, st, rows,
                        ptr, idx, val, s, x, ldx, y, ldy, act);
@@ -2062,12 +2206,16 @@ void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
   const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
   const int lpr = trans ? A.lpr_cols : A.lpr_rows;
-  if (p == Prec::F64)
-    launch_spmm<double>(ctx, rows, ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(),
-                        lpr, s, (const double *)x, ldx, (double *)y, ldy, active);
-  else
-    launch_spmm<float>(ctx, rows, ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(),
-                       lpr, s, (const float *)x, ldx, (float *)y, ldy, active);
+  const int uni = trans ? A.uniform_cols : A.uniform_rows;
+  if (p == Prec::F64) {
+    const double *val = trans ? A.vt.as<double>() : A.va.as<double>();
+    launch_spmm<double>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val), s,
+                        (const double *)x, ldx, (double *)y, ldy, active);
+  } else {
+    const float *val = trans ? A.vt32.as<float>() : A.va32.as<float>();
+    launch_spmm<float>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val), s,
+                       (const float *)x, ldx, (float *)y, ldy, active);
+  }
 }
 
 void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,

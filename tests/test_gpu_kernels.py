@@ -105,6 +105,52 @@ def test_gram_blocks(ctx, k, n, prec):
     assert np.abs(G - Gr).max() <= tol * scale.max()
 
 
+def _sequential_rows(A, X, dt, transpose):
+    """op(A) X with every row's sum taken sequentially from 0 in ascending
+    column order, each product and sum rounded to dt (no contraction)."""
+    S = A.to_scipy()
+    if transpose:
+        S = S.T.tocsr()
+        S.sort_indices()
+    rp, ci, va = S.indptr, S.indices, S.data.astype(dt)
+    lens = np.diff(rp)
+    out = np.zeros((S.shape[0], X.shape[1]), dt)
+    for r in range(int(lens.max(initial=0))):
+        rows = np.nonzero(lens > r)[0]
+        p = rp[rows] + r
+        out[rows] = out[rows] + va[p, None] * X[ci[p]]
+    return out
+
+
+@pytest.mark.parametrize("case,transpose", [("incidence", 0), ("incidence", 1),
+                                            ("uniform1", 0), ("uniform3", 0),
+                                            ("uniform8", 0), ("uniform5", 0),
+                                            ("uniform3", 1)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_spmm_uniform_rows_bit_exact(case, transpose, prec):
+    """Operators whose rows all have the same number of entries R (the
+    incidence: 2 per row, 4 per column; random rows of 1, 3, 8) take the
+    fixed-length kernels, which read no row pointers; their sums are the
+    sequential ones, bitwise, in FP64 and FP32, for one and several columns.
+    R = 5 and the ragged transposes take the pointer-based kernels."""
+    from helpers import random_sparse
+    from krans_amd import workloads as W
+    if case == "incidence":
+        A = W.regular_incidence(4, 512, seed=7)
+    else:
+        A = random_sparse(3000, 2000, int(case[-1]), seed=int(case[-1]))
+    print(f"[case] spmm uniform {case} t={transpose} prec={prec}", flush=True)
+    H = K.HipCsrMatrixAdapter.from_csr(A)
+    nin, nout = (A.m, A.n) if transpose else (A.n, A.m)
+    dt = np.float64 if prec == 0 else np.float32
+    for s in (1, 5):
+        X = np.random.default_rng(s).standard_normal((nin, s)).astype(dt)
+        dX, dY = Dev(H.ctx, X), Dev(H.ctx, np.zeros((nout, s), dt))
+        K._check(K.lib().kh_spmm_dev(H.h, transpose, prec, s, dX.p, dY.p))
+        Y, ref = dY.get(), _sequential_rows(A, X, dt, transpose)
+        assert Y.dtype == dt and np.array_equal(Y, ref), s
+
+
 @pytest.mark.parametrize("s,transpose", [(1, 0), (9, 0), (96, 0), (96, 1), (130, 1)])
 def test_spmm_rows_bit_exact(s, transpose):
     """Row-major block product (LOBPCG normal products): per entry the same
