@@ -581,85 +581,123 @@ __global__ void __launch_bounds__(kBlock)
 // column half t / 128 (every load issued before any use), the tile is stored
 // to LDS (column-major, stride 129), threads < 128 then update their row's w
 // exactly as k_vh_sub does (sequential sum over all columns in order), and
-// thread c sums column c of the tile against the new w in row order. HBM sees
-// V once. With MULTI a block walks tiles blockIdx.x, blockIdx.x + gridDim.x,
-// ... (the next tile's loads in flight while the current one is processed)
-// and accumulates each column over its tiles in that order, so the partials
-// of V^T w are one per block: part[c * gridDim.x + block]. Without MULTI the
-// grid must cover every tile (one each; the prefetch and its registers are
-// compiled out).
+// all threads sum the tile's columns against the new w: with cw the smallest
+// of 16, 32, 64, 128 not below s, thread t takes column t % cw over the
+// cw / 2 rows of row group t / cw (row order), so no chain is longer than
+// cw / 2 products. Each thread accumulates its (column, group) over the
+// block's tiles; the groups of a column are then summed in group order. HBM
+// sees V once. With MULTI a block walks tiles blockIdx.x, blockIdx.x +
+// gridDim.x, ... (the next tile's loads in flight while the current one is
+// processed), so the partials of V^T w are one per block: part[c * gridDim.x
+// + block]. Without MULTI the grid must cover every tile (one each; the
+// prefetch and its registers are compiled out).
 constexpr int kLdsRows = 128, kLdsStride = kLdsRows + 1;
 
-template <class TV, class TW, int HMAX, bool MULTI>
+// With VEC (FP32 V, len and ld multiples of 4, V 16-byte aligned) the tile
+// is loaded with 16-byte loads instead: thread t takes rows 4 (t % 32) ..
+// 4 (t % 32) + 3 of columns t / 32, t / 32 + 8, ... (HMAX / 4 loads).
+template <class TV, class TW, int HMAX, bool MULTI, bool VEC>
 __global__ void __launch_bounds__(kBlock)
     k_vh_sub_vtw_lds(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
                      const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
                      int nb, double *__restrict__ npart) {
+  static_assert(!VEC || (std::is_same<TV, float>::value && HMAX % 4 == 0),
+                "16-byte tile loads are for FP32 blocks");
+  constexpr int NV = VEC ? HMAX / 4 : 1;  // 16-byte loads per thread
   extern __shared__ double sh[];
-  double *hs = sh;               // [s]
-  double *wn = hs + s;           // [kLdsRows]
+  double *wn = sh;  // [kLdsRows]
   TV *tile = reinterpret_cast<TV *>(wn + kLdsRows);  // [s][kLdsStride]
   const int t = threadIdx.x, rl = t & (kLdsRows - 1), half = t / kLdsRows;
-  for (int j = t; j < s; j += kBlock)
-    hs[j] = h[j];
   const int sh2 = (s + 1) >> 1;
   const int c_lo = half ? sh2 : 0, c_hi = half ? s : sh2;
   const int64_t ntiles = (len + kLdsRows - 1) / kLdsRows;
-  double colsum = 0.0;  // thread c's column over this block's tiles
+  const int cw = s <= 16 ? 16 : s <= 32 ? 32 : s <= 64 ? 64 : 128;
+  const int col = t & (cw - 1), grp = t / cw, rpg = cw / 2;
+  double colsum = 0.0;  // (column col, row group grp) over this block's tiles
   double nsum = 0.0;    // with npart: ||updated w||^2 over this block's rows (wave 2)
   // the next tile's loads are issued as soon as the current tile is in LDS,
   // so they are in flight while its two passes run
-  TV x[HMAX];
+  TV x[VEC ? 1 : HMAX];
+  float4 x4[NV];
   TW wnext = (TW)0;
+  const int q4 = t & 31, cg = t >> 5;  // VEC: row quad, first column
   auto load = [&](int64_t ti) {
-    const int64_t r = ti * kLdsRows + rl;
-    const TV *base = V + (r < len ? r : len - 1);
     // an opaque copy of ld (empty asm, scalar register) keeps the compiler
-    // from hoisting the HMAX column offsets out of the tile loop into vector
+    // from hoisting the column offsets out of the tile loop into vector
     // registers
     int64_t ldo = ld;
     __asm__ volatile("" : "+s"(ldo));
+    if constexpr (VEC) {
+      int64_t r4 = ti * kLdsRows + 4 * q4;
+      if (r4 >= len)
+        r4 = len - 4;  // len % 4 == 0: rows r4 .. r4 + 3 exist
+      const TV *base = V + r4;
 #pragma unroll
-    for (int u = 0; u < HMAX; ++u) {
-      const int c = c_lo + u;
-      const int cc = c < c_hi ? c : (c_hi > c_lo ? c_hi - 1 : 0);
-      x[u] = base[(int64_t)cc * ldo];
+      for (int u = 0; u < NV; ++u) {
+        const int c = u * 8 + cg;
+        const int cc = c < s ? c : s - 1;
+        x4[u] = *reinterpret_cast<const float4 *>(base + (int64_t)cc * ldo);
+      }
+    } else {
+      const int64_t r = ti * kLdsRows + rl;
+      const TV *base = V + (r < len ? r : len - 1);
+#pragma unroll
+      for (int u = 0; u < HMAX; ++u) {
+        const int c = c_lo + u;
+        const int cc = c < c_hi ? c : (c_hi > c_lo ? c_hi - 1 : 0);
+        x[u] = base[(int64_t)cc * ldo];
+      }
     }
-    if (half == 0)
+    if (half == 0) {
+      const int64_t r = ti * kLdsRows + rl;
       wnext = w[r < len ? r : len - 1];
+    }
   };
   if ((int64_t)blockIdx.x < ntiles)
     load(blockIdx.x);
   for (int64_t tile_i = blockIdx.x; tile_i < ntiles; tile_i += gridDim.x) {
     const int64_t r = tile_i * kLdsRows + rl;
     __syncthreads();  // the previous tile's LDS reads are done
+    if constexpr (VEC) {
 #pragma unroll
-    for (int u = 0; u < HMAX; ++u) {
-      const int c = c_lo + u;
-      if (c < c_hi)
-        tile[c * kLdsStride + rl] = x[u];
+      for (int u = 0; u < NV; ++u) {
+        const int c = u * 8 + cg;
+        if (c < s) {
+          TV *d = tile + c * kLdsStride + 4 * q4;
+          d[0] = x4[u].x, d[1] = x4[u].y, d[2] = x4[u].z, d[3] = x4[u].w;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < HMAX; ++u) {
+        const int c = c_lo + u;
+        if (c < c_hi)
+          tile[c * kLdsStride + rl] = x[u];
+      }
     }
     const TW wold = wnext;
     __syncthreads();
     if (MULTI && tile_i + (int64_t)gridDim.x < ntiles)
       load(tile_i + gridDim.x);
     if (half == 0) {
+      // h[c] is the same for every lane: scalar loads, no LDS traffic
       double acc = 0.0;
 #pragma unroll 2
       for (int c = 0; c < s; ++c)
-        acc += (double)tile[c * kLdsStride + rl] * hs[c];
+        acc += (double)tile[c * kLdsStride + rl] * h[c];
       const TW nw = (TW)((double)wold - acc);
       if (r < len)
         w[r] = nw;
       wn[rl] = r < len ? (double)nw : 0.0;
     }
     __syncthreads();
-    if (t < s) {  // s <= kBlock
-      const TV *tc = tile + t * kLdsStride;
+    if (col < s) {  // s <= 128
+      const TV *tc = tile + col * kLdsStride + grp * rpg;
+      const double *wg = wn + grp * rpg;
       double a = 0.0;
 #pragma unroll 4
-      for (int i = 0; i < kLdsRows; ++i)
-        a += (double)tc[i] * wn[i];
+      for (int i = 0; i < rpg; ++i)
+        a += (double)tc[i] * wg[i];
       colsum += a;
     }
     if (npart && t >= 128 && t < 192) {  // rows l and l + 64, then a fixed wave tree
@@ -669,8 +707,17 @@ __global__ void __launch_bounds__(kBlock)
       nsum += v;
     }
   }
-  if (t < s)
-    part[(int64_t)t * nb + blockIdx.x] = colsum;
+  // the row groups of each column, in group order (the tile's LDS is free)
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(tile);  // [kBlock / cw][cw]
+  red[t] = colsum;
+  __syncthreads();
+  if (t < s) {
+    double a = 0.0;
+    for (int g = 0; g < kBlock / cw; ++g)
+      a += red[g * cw + t];
+    part[(int64_t)t * nb + blockIdx.x] = a;
+  }
   if (npart && t == 128)
     npart[blockIdx.x] = nsum;
 }
@@ -2387,13 +2434,18 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
                                 TW *w, int64_t len, double *out, double *nrm1) {
   // LDS-tiled form when its tile fits 64 KB and the columns fill it (below
   // about 12 columns the two-sweep kernel is faster)
-  const std::size_t lds = (std::size_t)s * 8 + kLdsRows * 8 +
-                          (std::size_t)s * kLdsStride * sizeof(TV);
-  // one 128-row tile per block up to 16384 tiles (2^21 rows); beyond, 16384
-  // blocks each take every 16384-th tile, which bounds the partials per
-  // column that k_finish sums
+  const std::size_t lds = kLdsRows * 8 + (std::size_t)s * kLdsStride * sizeof(TV);
   const int64_t ntl = (len + kLdsRows - 1) / kLdsRows;
-  const int64_t nbl = std::min<int64_t>(ntl, 16384);
+  constexpr bool kF32 = std::is_same<TV, float>::value;
+  const bool vec = kF32 && len % 4 == 0 && ld % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(V) % 16 == 0;
+  // FP32 blocks with 16-byte loads: at most 4096 blocks, each walking its
+  // tiles with the next tile's loads in flight (a few tiles per block at 2^21
+  // rows); otherwise one 128-row tile per block up to 16384 tiles (2^21
+  // rows), and beyond that 16384 blocks each take every 16384-th tile (the
+  // dword-load pipelined form needs many more registers). Both bound the
+  // partials per column that k_finish sums.
+  const int64_t nbl = std::min<int64_t>(ntl, vec ? 4096 : 16384);
   if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * (s + 1) <= ctx.part_cap) {
     double *part = ctx.partials((std::size_t)nbl * (s + 1));
     double *npart = nrm1 ? part + (std::size_t)nbl * s : nullptr;
@@ -2402,17 +2454,26 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
                     (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * nbl * s);
       const int hm = (s + 1) / 2;
       const bool multi = ntl > nbl;
-#define KH_LDS_LAUNCH(H, M)                                                                    \
-  hipLaunchKernelGGL((k_vh_sub_vtw_lds<TV, TW, H, M>), dim3((unsigned)nbl), dim3(kBlock), lds, \
-                     ctx.stream, len, V, ld, s, h, w, part, (int)nbl, npart)
+#define KH_LDS_LAUNCH(H, M, VC)                                                                \
+  hipLaunchKernelGGL((k_vh_sub_vtw_lds<TV, TW, H, M, VC>), dim3((unsigned)nbl), dim3(kBlock),  \
+                     lds, ctx.stream, len, V, ld, s, h, w, part, (int)nbl, npart)
+#define KH_LDS_PICK(H)                                                                         \
+  {                                                                                            \
+    if (vec) {                                                                                 \
+      if (multi) KH_LDS_LAUNCH(H, true, kF32); else KH_LDS_LAUNCH(H, false, kF32);             \
+    } else {                                                                                   \
+      if (multi) KH_LDS_LAUNCH(H, true, false); else KH_LDS_LAUNCH(H, false, false);           \
+    }                                                                                          \
+  }
       if (hm <= 16)
-        { if (multi) KH_LDS_LAUNCH(16, true); else KH_LDS_LAUNCH(16, false); }
+        KH_LDS_PICK(16)
       else if (hm <= 32)
-        { if (multi) KH_LDS_LAUNCH(32, true); else KH_LDS_LAUNCH(32, false); }
+        KH_LDS_PICK(32)
       else if (hm <= 48)
-        { if (multi) KH_LDS_LAUNCH(48, true); else KH_LDS_LAUNCH(48, false); }
+        KH_LDS_PICK(48)
       else
-        { if (multi) KH_LDS_LAUNCH(64, true); else KH_LDS_LAUNCH(64, false); }
+        KH_LDS_PICK(64)
+#undef KH_LDS_PICK
 #undef KH_LDS_LAUNCH
     }
     hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl, out, 0);
