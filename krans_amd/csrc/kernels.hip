@@ -73,6 +73,18 @@ inline int grid_for(int64_t len, int per_thread = 1) {
 }
 
 // ----------------------------------------------------------------- sparse --
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2. The gather kernels renumber blocks so that
+// XCD x takes the contiguous block range [x q, (x + 1) q), q = nb / 8 (the
+// last nb % 8 blocks keep their numbers): an XCD's rows then touch the x
+// entries of one contiguous eighth of the operator, which its L2 keeps,
+// instead of every XCD fetching all of x. A bijection on [0, nb); it changes
+// only which workgroup computes a row, never the row's arithmetic.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t b = blockIdx.x, nb = gridDim.x, q = nb >> 3;
+  return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
+}
+
 // sum of val[p] * x(idx[p]) over p in [p0, p1), ascending from 0 with no
 // contraction (the rounding of a sequential loop). Entries go in groups of
 // four, then pairs, then a last single one; the index, value and x loads of
@@ -118,7 +130,7 @@ __global__ void __launch_bounds__(kBlock)
                 const T *__restrict__ x, T *__restrict__ y,
                 const T *__restrict__ z, const double *__restrict__ beta) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
   if (i >= rows)
     return;
   const T b = z ? (T)(*beta) : (T)0;
@@ -182,7 +194,7 @@ __global__ void __launch_bounds__(kBlock)
                const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ z,
                const double *__restrict__ beta) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
   if (i >= rows)
     return;
   const T b = z ? (T)(*beta) : (T)0;
@@ -209,7 +221,7 @@ __global__ void __launch_bounds__(kBlock)
                const T *__restrict__ x, int64_t ldx, T *__restrict__ y, int64_t ldy,
                const int *__restrict__ active) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
   if (i >= rows)
     return;
   int32_t j[R];
@@ -269,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
                 const int32_t *__restrict__ idx, const T *__restrict__ val,
                 int s, const T *__restrict__ x, int64_t ldx, T *__restrict__ y,
                 int64_t ldy, const int *__restrict__ active) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
   if (i >= rows)
     return;
   const int32_t p0 = ptr[i], p1 = ptr[i + 1];

@@ -4,6 +4,11 @@ reads for the plain copy kernel of known size that bench.py launches first
 (k_axpby<double> copying 256 MiB: 256 MiB read, 256 MiB written).
 
     python -m krans_amd.pmc_traffic <fetch.db> <write.db> <kernel-substring> [--largest]
+    python -m krans_amd.pmc_traffic --avg <db> <counter> <kernel-substring>...
+
+The second form prints, per kernel name containing one of the substrings,
+the dispatch count and the average of one counter per dispatch (any
+counter of a --pmc run, e.g. TCP_TCC_READ_REQ_sum, unscaled).
 
 With --largest only the dispatches whose FETCH_SIZE is at least half the
 largest one count (one launch shape out of several, e.g. the 96-column Gram
@@ -63,5 +68,22 @@ def main(fetch_db, write_db, like, largest=False):
     print(json.dumps(out, indent=1))
 
 
+def averages(db, counter, likes):
+    c = sqlite3.connect(db)
+    out = {}
+    for like in likes:
+        rows = c.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                         "where counter_name = ? and kernel_name like ? "
+                         "group by dispatch_id", (counter, f"%{like}%")).fetchall()
+        for name, _, v in rows:
+            n, tot = out.get(name, (0, 0.0))
+            out[name] = (n + 1, tot + v)
+    res = {name: {"dispatches": n, f"avg_{counter}": tot / n} for name, (n, tot) in out.items()}
+    print(json.dumps(res, indent=1))
+
+
 if __name__ == "__main__":
-    main(sys.argv[1], sys.argv[2], sys.argv[3], "--largest" in sys.argv[4:])
+    if sys.argv[1] == "--avg":
+        averages(sys.argv[2], sys.argv[3], sys.argv[4:])
+    else:
+        main(sys.argv[1], sys.argv[2], sys.argv[3], "--largest" in sys.argv[4:])
