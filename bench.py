@@ -52,11 +52,18 @@ def parse():
     ap.add_argument("--max-iters", type=int, default=3000, help="c4: LOBPCG max_iters")
     ap.add_argument("--matrix-free", action="store_true",
                     help="c3a: apply the shifted difference as a stencil (no CSR in HBM)")
-    return ap.parse_args()
+    ap.add_argument("--hip-graphs", default="auto", choices=["auto", "on", "off"],
+                    help="replay Golub-Kahan cycles from captured HIP graphs (on), launch "
+                         "every kernel directly (off), or decide by problem size (auto)")
+    args = ap.parse_args()
+    global HIPGRAPHS
+    HIPGRAPHS = {"auto": 0, "on": 1, "off": -1}[args.hip_graphs]
+    return args
 
 
 PRESCREEN = 1e-4
 MAX_RESTARTS = 400
+HIPGRAPHS = 0  # CyclicLanczosOptions.graphs: 1 captured cycles, 0 by problem size, -1 never
 
 
 def spmv_bytes(A, elt, transpose=False):
@@ -93,7 +100,7 @@ def main():
     k = args.graphs
     setup_s = time.time() - t0
     opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
-                                 max_restarts=MAX_RESTARTS)
+                                 max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS)
     S = A.to_scipy()
 
     def check(U):
@@ -220,7 +227,9 @@ def main():
         "accuracy": {"max_res_rel": chk_m[0], "sine_vs_exact": chk_m[1],
                      "vectors": found_m[0], "restarts": info_m.restarts,
                      "cg_solves": info_m.cg_solves, "phase_s": info_m.times,
-                     "screened": info_m.screened},
+                     "screened": info_m.screened,
+                     "hip_graphs": {"captures": info_m.graph_captures,
+                                    "replays": info_m.graph_replays}},
         "roofline": {
             "kernel": "k_vtw4_partial (reorthogonalisation V^T w, FP32 in with 16-byte loads, "
                       "FP64 accumulation)",
@@ -310,10 +319,10 @@ def variant(args):
         if cfg == "c3a":
             opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0,
                                          prescreen_factor=0.05 / fro,
-                                         max_restarts=MAX_RESTARTS, cg_maxit=2000)
+                                         max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS, cg_maxit=2000)
         else:
             opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
-                                         max_restarts=MAX_RESTARTS, cg_maxit=4000,
+                                         max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS, cg_maxit=4000,
                                          precond_AAt="jacobi", verbose=True)
         cls = "spmv"
 
@@ -459,7 +468,7 @@ def serve(args):
 
     def opt(k):
         return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
-                                      max_restarts=MAX_RESTARTS, cg_maxit=1000)
+                                      max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS, cg_maxit=1000)
 
     # batched mixed-precision server
     srv = K.NullspaceServer(slots=args.serve_slots)

@@ -42,7 +42,12 @@ typedef struct kh_rng_s kh_rng;
 /* Field for field krans::solvers::CyclicLanczosOptions
  * (include/krans/Solvers/CyclicLanczosSolver.hpp:84-146); the two non-POD
  * members are replaced by `y0` (initial_right_space[0]) in
- * kh_cyclic_lanczos and by `precond_jacobi` (M^{-1} = diag(A A^T)^{-1}). */
+ * kh_cyclic_lanczos and by `precond_jacobi` (M^{-1} = diag(A A^T)^{-1}).
+ * `graphs` is an engine choice with no reference counterpart: 1 replays each
+ * Golub-Kahan cycle from a HIP graph captured once per basis size when the
+ * operator only enqueues kernels (CSR, shifted difference; not callbacks),
+ * -1 launches every kernel directly, 0 (default) decides by problem size.
+ * Results are bitwise the same either way. */
 typedef struct kh_cyclic_options {
   int64_t k, ncv, max_restarts;
   double tol_abs, tol_rel;
@@ -56,7 +61,7 @@ typedef struct kh_cyclic_options {
   double prescreen_factor;
   int64_t stagnation_restarts;
   int32_t return_left_space, verbose;
-  int32_t precond_jacobi, reserved;
+  int32_t precond_jacobi, graphs;
 } kh_cyclic_options;
 
 /* krans::solvers::CyclicLanczosResult (CyclicLanczosSolver.hpp:149-167) plus
@@ -67,6 +72,7 @@ typedef struct kh_cyclic_result {
   int64_t converged_to_target, cg_solves, cg_maxit_hits, spmv, spmtv, cg_iters;
   double t_gk, t_tiny, t_cand, t_polish, t_fallback, t_rng;
   int64_t screened; /* mixed precision: candidates rejected by the FP32 screen */
+  int64_t graph_captures, graph_replays; /* Golub-Kahan cycles as HIP graphs */
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and

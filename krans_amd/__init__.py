@@ -43,7 +43,7 @@ class _CyclicOpts(C.Structure):
         ("prescreen_factor", C.c_double),
         ("stagnation_restarts", C.c_int64),
         ("return_left_space", C.c_int32), ("verbose", C.c_int32),
-        ("precond_jacobi", C.c_int32), ("reserved", C.c_int32),
+        ("precond_jacobi", C.c_int32), ("graphs", C.c_int32),
     ]
 
 
@@ -56,7 +56,7 @@ class _CyclicRes(C.Structure):
         ("cg_iters", C.c_int64),
         ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
-        ("screened", C.c_int64),
+        ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
     ]
 
 
@@ -462,6 +462,10 @@ class CyclicLanczosOptions:
     stagnation_restarts: int = 30
     return_left_space: bool = False
     verbose: bool = False
+    # engine choice (not in the reference struct): Golub-Kahan cycles from
+    # captured HIP graphs where the operator allows it: 1 always, 0 by problem
+    # size, -1 never
+    graphs: int = 0
 
     def _c(self):
         o = _CyclicOpts()
@@ -492,6 +496,8 @@ class CyclicLanczosResult:
     cg_iters: int = 0
     times: dict = field(default_factory=dict)
     screened: int = 0  # mixed precision: candidates rejected by the FP32 screen
+    graph_captures: int = 0  # Golub-Kahan cycles captured as HIP graphs
+    graph_replays: int = 0   # ... and replayed
 
 
 def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
@@ -521,7 +527,8 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         restarts=res.restarts, krylov_steps_total=res.krylov_steps_total,
         frob_norm_est=res.frob_norm_est, converged_to_target=bool(res.converged_to_target),
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
-        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened)
+        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
+        graph_captures=res.graph_captures, graph_replays=res.graph_replays)
 
 
 def find_nullspace_cyclic_lanczos(A, vf, opt, precision="fp64"):

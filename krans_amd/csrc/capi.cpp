@@ -83,6 +83,7 @@ kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
   P.return_left_space = o.return_left_space != 0;
   P.verbose = o.verbose != 0;
   P.jacobi_AAt = o.precond_jacobi != 0;
+  P.graphs = o.graphs;
   P.krylov = prec_of(precision);
   P.polish = P.krylov;
   need(P.k >= 0 && P.ncv >= 0 && P.max_restarts >= 0 && P.cg_maxit >= 0,
@@ -377,6 +378,8 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     res->t_fallback = S.t.fallback + SL.t.fallback;
     res->t_rng = S.t.rng + SL.t.rng;
     res->screened = S.screened + SL.screened;
+    res->graph_captures = S.graph_captures + SL.graph_captures;
+    res->graph_replays = S.graph_replays + SL.graph_replays;
   });
 }
 
@@ -495,6 +498,8 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->t_fallback = r.stats.t.fallback;
     res->t_rng = r.stats.t.rng;
     res->screened = r.stats.screened;
+    res->graph_captures = r.stats.graph_captures;
+    res->graph_replays = r.stats.graph_replays;
     if (latency_s)
       *latency_s = r.latency;
     if (queued_s)

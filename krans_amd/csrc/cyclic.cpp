@@ -83,6 +83,9 @@ struct Group {  // a block of vectors to orthogonalise against
   Prec p;
 };
 
+// Automatic graph replay of the Golub-Kahan cycle up to this m + n.
+constexpr int64_t kGraphAutoLen = 0;
+
 class CyclicDriver {
 public:
   CyclicDriver(Context &ctx, const DeviceOp &op, RefStream &rng, const CyclicParams &P,
@@ -122,6 +125,9 @@ private:
   int screen_miss_ = 0;   // consecutive windows the screen did not thin out
   int wcap_ = 0;
   CgBatch cg_;
+  CapturedWork gk_graph_;  // the queued Golub-Kahan cycle, keyed by the basis size
+  bool graphs_ = false;
+  int64_t gk_nyc_ = 0, gk_nxc_ = 0, gk_nmv_ = 0, gk_nmtv_ = 0;  // counts of the captured cycle
 
   void alloc();
   void ensure_window(int w);
@@ -514,6 +520,8 @@ DeviceBasis CyclicDriver::run() {
   // unit vectors have ||A v|| <= ||A||_2 <= ||A||_F: a screen below a tenth
   // of the norm can reject candidates
   screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1;
+  graphs_ = A_.capturable() && c_.trace_mask == 0 &&
+            (P_.graphs > 0 || (P_.graphs == 0 && m_ + n_ <= kGraphAutoLen));
 
   double *y0 = y0_.as<double>();
   if (P_.y0_host)
@@ -554,56 +562,83 @@ DeviceBasis CyclicDriver::run() {
     // (non-finite or < 1e-14) is replayed on the host afterwards; steps
     // queued after a stop only write columns that are then ignored.
     const int nb = (int)basis_.count;
-    int64_t nyc = 1, nxc = 0;
-    for (int64_t j = 0; j < ncv; ++j) {
-      const bool from_y = nyc > nxc;
-      void *w = from_y ? wx_.p : wy_.p;
-      const int64_t len = from_y ? m_ : n_;
-      if (from_y) {
-        A_.mv(c_, kp, Ycol(nyc - 1), w, j == 0 ? nullptr : Xcol(nxc - 1),
-              j == 0 ? nullptr : bnorm_ + (j - 1));
-        ++S_.spmv;
-      } else {
-        A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
-        ++S_.spmtv;
-      }
-      const int passes = P_.reorth;
-      std::vector<Group> groups;
-      if (from_y)
-        groups.push_back({Xb, ldx_, (int)nxc, kp});
-      else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
-        groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
-      const Group &g0 = groups.front();
-      bool stored = false;  // w / beta_j already written to its basis column
-      if (passes == 2 && g0.count > 0) {
-        // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
-        // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
-        // except that h2's partial sums are grouped per block of the fused
-        // kernel.
-        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
-        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
-        if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
-          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
-                               bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
-          stored = true;
+    int64_t nyc = 1, nxc = 0, nmv = 0, nmtv = 0;
+    // The queued cycle depends on nothing but the basis size: same kernels,
+    // shapes and buffers, with beta_j read on the device. Where the operator
+    // only enqueues kernels it is captured once per basis size and replayed.
+    auto queue_cycle = [&]() {
+      for (int64_t j = 0; j < ncv; ++j) {
+        const bool from_y = nyc > nxc;
+        void *w = from_y ? wx_.p : wy_.p;
+        const int64_t len = from_y ? m_ : n_;
+        if (from_y) {
+          A_.mv(c_, kp, Ycol(nyc - 1), w, j == 0 ? nullptr : Xcol(nxc - 1),
+                j == 0 ? nullptr : bnorm_ + (j - 1));
+          ++nmv;
         } else {
-          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+          A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
+          ++nmtv;
         }
-      } else {
-        for (int pass = 0; pass < passes; ++pass)
-          cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+        const int passes = P_.reorth;
+        std::vector<Group> groups;
+        if (from_y)
+          groups.push_back({Xb, ldx_, (int)nxc, kp});
+        else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
+          groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
+        const Group &g0 = groups.front();
+        bool stored = false;  // w / beta_j already written to its basis column
+        if (passes == 2 && g0.count > 0) {
+          // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
+          // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
+          // except that h2's partial sums are grouped per block of the fused
+          // kernel.
+          gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
+          gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
+          if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
+            gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
+                                 bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
+            stored = true;
+          } else {
+            gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+          }
+        } else {
+          for (int pass = 0; pass < passes; ++pass)
+            cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+        }
+        if (passes <= 0)
+          dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
+        if (nyc + nxc >= ncv)
+          break;
+        if (!stored)
+          scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
+        if (from_y)
+          ++nxc;
+        else
+          ++nyc;
       }
-      if (passes <= 0)
-        dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
-      if (nyc + nxc >= ncv)
-        break;
-      if (!stored)
-        scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
-      if (from_y)
-        ++nxc;
-      else
-        ++nyc;
+    };
+    if (graphs_) {
+      if (!gk_graph_.holds(nb)) {
+        gk_graph_.begin(c_);
+        try {
+          queue_cycle();
+        } catch (...) {
+          gk_graph_.end(c_, nb, false);
+          throw;
+        }
+        gk_graph_.end(c_, nb, true);
+        ++S_.graph_captures;
+        gk_nyc_ = nyc, gk_nxc_ = nxc, gk_nmv_ = nmv, gk_nmtv_ = nmtv;
+      } else {
+        nyc = gk_nyc_, nxc = gk_nxc_, nmv = gk_nmv_, nmtv = gk_nmtv_;
+      }
+      gk_graph_.launch(c_);
+      ++S_.graph_replays;
+    } else {
+      queue_cycle();
     }
+    S_.spmv += nmv;
+    S_.spmtv += nmtv;
     const int64_t queued = nyc + nxc;  // steps queued (the last one stores nothing)
     readback(c_, bnorm_, (int)queued, bh.data());
     double beta_last = 0.0;

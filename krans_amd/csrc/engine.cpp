@@ -71,6 +71,43 @@ Context::~Context() {
 
 void Context::sync() { KH_CHECK(hipStreamSynchronize(stream)); }
 
+void CapturedWork::reset() {
+  if (exec)
+    (void)hipGraphExecDestroy(exec);
+  exec = nullptr;
+  key = -1;
+}
+
+void CapturedWork::begin(Context &ctx) {
+  KH_CHECK(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+}
+
+void CapturedWork::end(Context &ctx, int64_t k, bool ok) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx.stream, &g);
+  if (!ok) {  // an exception left the queued code: drop what was recorded
+    if (g)
+      (void)hipGraphDestroy(g);
+    return;
+  }
+  KH_CHECK(e);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  KH_CHECK(e);
+  reset();
+  exec = x;
+  key = k;
+  ++captures;
+}
+
+void CapturedWork::launch(Context &ctx) {
+  if (!exec)
+    throw std::logic_error("CapturedWork::launch without a captured graph");
+  KH_CHECK(hipGraphLaunch(exec, ctx.stream));
+  ++replays;
+}
+
 void Context::fetch_scalars(int i0, int n) {
   if (i0 < 0 || n < 0 || i0 + n > kScalars)
     throw std::out_of_range("scalar slab index out of range");

@@ -75,6 +75,27 @@ struct Context {
   double *partials(std::size_t need);
 };
 
+// One instantiated HIP graph of work queued on a context's stream, kept
+// under an integer key and captured again when the key changes. Capture
+// records launches without running them (thread-local mode, so other
+// threads' allocations stay legal); the queued code must not synchronise,
+// allocate, or clear memory with hipMemsetAsync.
+struct CapturedWork {
+  hipGraphExec_t exec = nullptr;
+  int64_t key = -1;
+  int64_t captures = 0, replays = 0;
+  CapturedWork() = default;
+  ~CapturedWork() { reset(); }
+  CapturedWork(const CapturedWork &) = delete;
+  CapturedWork &operator=(const CapturedWork &) = delete;
+  void reset();
+  bool holds(int64_t k) const { return exec != nullptr && key == k; }
+  void begin(Context &ctx);
+  // ends the capture begun by begin(); ok = false discards it
+  void end(Context &ctx, int64_t k, bool ok);
+  void launch(Context &ctx);
+};
+
 // Device allocation with an owner; moves only.
 struct DBuf {
   void *p = nullptr;

@@ -30,6 +30,7 @@ struct StencilOp final : DeviceOp {
   void mtm(Context &ctx, Prec p, int s, const void *x, int64_t ldx, void *y, int64_t ldy,
            const int *act) const override;
   bool has_rows() const override { return true; }
+  bool capturable() const override { return true; }
   void mm_rows(Context &ctx, Prec p, int s, const void *X, void *Y, bool trans) const override;
   const double *diag_AAt() const override { return diag.as<double>(); }
   double fro() const override { return fro_; }
@@ -89,6 +90,7 @@ struct TransposedOpView final : DeviceOp {
     A.mm(c, p, s, x, ldx, y, ldy, act);
   }
   bool has_rows() const override { return A.has_rows(); }
+  bool capturable() const override { return A.capturable(); }
   void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const override {
     A.mm_rows(c, p, s, X, Y, !trans);
   }

@@ -33,6 +33,9 @@ struct DeviceOp {
   // Block products on row-major blocks (stride s): Y = op(A) X, op = A or
   // A^T. Operators without them report false and callers use mm/mtm.
   virtual bool has_rows() const { return false; }
+  // True when mv / mtv only enqueue kernels on the context's stream (no
+  // synchronisation, no host callback), so they can be captured in a graph.
+  virtual bool capturable() const { return false; }
   virtual void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const {
     (void)c, (void)p, (void)s, (void)X, (void)Y, (void)trans;
     throw std::logic_error("row-major block products not provided by this operator");
@@ -72,6 +75,7 @@ struct CsrOp final : DeviceOp {
     spmm(c, A, !t, p, s, x, ldx, y, ldy, act);
   }
   bool has_rows() const override { return true; }
+  bool capturable() const override { return true; }
   void mm_rows(Context &c, Prec p, int s, const void *X, void *Y, bool trans) const override {
     spmm_rows(c, A, t != trans, p, s, X, Y);
   }
@@ -141,6 +145,10 @@ struct CyclicParams {
   Prec krylov = Prec::F64;  // Golub-Kahan vectors and products
   Prec polish = Prec::F64;  // CG of the projector polish (F32 = refinement)
   const double *y0_host = nullptr;  // initial_right_space[0], host, length n
+  // Golub-Kahan cycles replayed from a captured HIP graph, for operators
+  // that only enqueue kernels and with tracing off: 1 always, 0 automatic
+  // (vectors short enough for the cycle to be launch-bound), -1 never
+  int graphs = 0;
 };
 
 struct PhaseTimes {
@@ -155,6 +163,7 @@ struct CyclicStats {
   int64_t spmv = 0, spmtv = 0;    // operator applications (column count)
   int64_t cg_iters = 0;           // CG iterations over all columns
   int64_t screened = 0;           // candidates rejected by the FP32 prescreen (mixed)
+  int64_t graph_captures = 0, graph_replays = 0;  // Golub-Kahan cycles as HIP graphs
   PhaseTimes t;
 };
 

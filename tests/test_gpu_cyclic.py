@@ -4,6 +4,8 @@ Checked against the CPU oracle (oracle/, an FP64 restatement of
 src/Solvers/CyclicLanczosSolver.cpp) on the same seeded inputs, and against
 the exact kernels of the generators. Tolerances are stated per test.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -209,3 +211,56 @@ def test_run_twice_bitwise_identical(precision):
     assert np.array_equal(r1.right_null.view(np.uint64), r2.right_null.view(np.uint64))
     assert (r1.restarts, r1.cg_solves, r1.krylov_steps_total) == \
         (r2.restarts, r2.cg_solves, r2.krylov_steps_total)
+
+
+@pytest.mark.parametrize("operand,precision", [("csr", "fp64"), ("csr", "mixed"),
+                                               ("stencil", "fp64"), ("stencil", "mixed")])
+def test_graph_replay_same_as_direct_launches(operand, precision):
+    """Golub-Kahan cycles replayed from captured HIP graphs (graphs=1)
+    against every kernel launched directly (graphs=-1): bitwise the
+    same bases and counters. A graph is captured once per basis size and
+    replayed on the following restarts, with the solve's own eager work and
+    host synchronisations between the replays."""
+    if operand == "csr":
+        A = W.regular_incidence(4, 2048, seed=9)
+        H, n, k = hip(A), A.n, 4
+        kw = dict(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3)
+    else:
+        n, N, c = 1000, 3, 0.5
+        H, k = K.HipShiftedDifferenceOperator(n, N, c), 3
+        kw = dict(k=k)
+    print(f"[case] graph replay {operand} {precision}", flush=True)
+    rg = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(n, 5),
+                                            K.CyclicLanczosOptions(graphs=1, **kw), precision)
+    re = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(n, 5),
+                                            K.CyclicLanczosOptions(graphs=-1, **kw), precision)
+    print(f"captures={rg.graph_captures} replays={rg.graph_replays} restarts={rg.restarts}",
+          flush=True)
+    assert re.graph_captures == 0 and re.graph_replays == 0
+    assert 1 <= rg.graph_captures <= k + 1
+    assert rg.graph_replays >= rg.graph_captures
+    assert rg.right_null.shape == re.right_null.shape == (n, k)
+    assert np.array_equal(rg.right_null.view(np.uint64), re.right_null.view(np.uint64))
+    assert (rg.restarts, rg.cg_solves, rg.krylov_steps_total, rg.spmv, rg.spmtv) == \
+        (re.restarts, re.cg_solves, re.krylov_steps_total, re.spmv, re.spmtv)
+
+
+def test_callback_operator_is_not_captured():
+    """A host callback operator synchronises and calls user code, so its
+    cycles are launched directly even with graphs=1."""
+    A = W.regular_incidence(2, 512, seed=4)
+    H = hip(A)
+
+    def apply(stream, precision, transpose, s, x, ldx, y, ldy):
+        e = 8 if precision == "fp64" else 4
+        p = 0 if precision == "fp64" else 1
+        for j in range(s):
+            K._check(K.lib().kh_spmm_dev(H.h, int(transpose), p, 1,
+                                         C.c_void_p(x + j * ldx * e), C.c_void_p(y + j * ldy * e)))
+    F = K.HipOperatorAdapter(A.m, A.n, apply)
+    print("[case] callback operator without graphs", flush=True)
+    opt = K.CyclicLanczosOptions(k=2, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
+                                 graphs=1)
+    r = K.find_nullspace_cyclic_lanczos_ex(F, K.HipVectorFactory(A.n, 3), opt, "mixed")
+    assert r.graph_captures == 0 and r.graph_replays == 0
+    assert r.right_null.shape == (A.n, 2)
