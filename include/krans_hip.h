@@ -179,6 +179,20 @@ int kh_rng_create(uint32_t seed, kh_rng **out);
 int kh_rng_destroy(kh_rng *r);
 int kh_rng_normal(kh_rng *r, int64_t n, double *out);
 
+/* --- tiny projected problem (host FP64, no device needed) -----------------
+ * The right singular system of the refined Golub-Kahan matrix that
+ * solve_tiny_refined hands to JacobiSVD (src/Solvers/CyclicLanczosSolver.cpp:
+ * 246-262), as the solver computes it: one-sided Jacobi on the column-major
+ * rows x cols matrix `a` (rows >= cols), singular values descending in sv
+ * (cols), right singular vectors in the columns of v (cols x cols, column-
+ * major). mode 0 sweeps the whole matrix; mode 1 sweeps the two parity
+ * blocks of a matrix whose entries with i + j even are zero (bitwise the
+ * same result); mode 2 as 1 but forms only the even block's vectors (the
+ * others are unit vectors on odd rows). Modes 1 and 2 fall back to 0 for
+ * other matrices. */
+int kh_tiny_svd_right(const double *a, int32_t rows, int32_t cols, int32_t mode, double *sv,
+                      double *v);
+
 /* --- solvers --------------------------------------------------------------
  * find_nullspace_cyclic_lanczos_ex (CyclicLanczosSolver.hpp:170-178).
  * Bases are written column-major: right (n x right_cap), left (m x

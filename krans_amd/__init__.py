@@ -109,6 +109,7 @@ SIGNATURES = {
     "kh_rng_create": (I32, [C.c_uint32, C.POINTER(P)]),
     "kh_rng_destroy": (I32, [P]),
     "kh_rng_normal": (I32, [P, I64, P]),
+    "kh_tiny_svd_right": (I32, [P, I32, I32, I32, P, P]),
     "kh_cyclic_lanczos": (I32, [P, P, C.POINTER(_CyclicOpts), I32, P, P, I64, P, I64,
                                 C.POINTER(_CyclicRes)]),
     "kh_lobpcg": (I32, [P, P, C.POINTER(_LobpcgOpts), I32, P, I64, P, I64, P, P,

@@ -331,6 +331,20 @@ int kh_rng_normal(kh_rng *r, int64_t n, double *out) {
   });
 }
 
+int kh_tiny_svd_right(const double *a, int32_t rows, int32_t cols, int32_t mode, double *sv,
+                      double *v) {
+  return guard([&] {
+    need(a && sv && v && cols >= 1 && rows >= cols && mode >= 0 && mode <= 2, "bad argument");
+    kh::dense::Mat A(rows, cols);
+    std::copy(a, a + (std::size_t)rows * cols, A.a.begin());
+    std::vector<double> s;
+    kh::dense::Mat V;
+    kh::dense::svd_right(A, s, V, mode >= 1, mode == 2);
+    std::copy(s.begin(), s.end(), sv);
+    std::copy(V.a.begin(), V.a.end(), v);
+  });
+}
+
 int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
                       int precision, const double *y0, double *right, int64_t right_cap,
                       double *left, int64_t left_cap, kh_cyclic_result *res) {

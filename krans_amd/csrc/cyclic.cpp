@@ -674,7 +674,10 @@ DeviceBasis CyclicDriver::run() {
         }
       }
       Tb((int)l, (int)l - 1) = beta_last;
-      dense::svd_right(Tb, ev, V);
+      // with a zero diagonal (target_tau = 0) T-bar splits into its even- and
+      // odd-column blocks; candidates read only even rows, so only the even
+      // block's vectors are formed (bitwise the same as the full sweeps)
+      dense::svd_right(Tb, ev, V, true, true);
       ord.resize(ev.size());
       std::iota(ord.begin(), ord.end(), 0);
       std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });

@@ -9,13 +9,12 @@ namespace kh::dense {
 
 static constexpr double kEps = std::numeric_limits<double>::epsilon();
 
-void svd_right(const Mat &A, std::vector<double> &sv, Mat &V) {
-  const int m = A.r, n = A.c;
-  std::vector<double> U(A.a);  // columns of A, orthogonalised in place
-  V = Mat(n, n);
-  for (int i = 0; i < n; ++i)
-    V(i, i) = 1.0;
-  std::vector<double> nrm(n);
+// One-sided Jacobi (Hestenes) sweeps over the n columns (length m) of U,
+// column-major; V (n x n, on entry the identity; may be null) accumulates the
+// rotations and nrm receives the squared column norms. The norms and U do not
+// depend on V.
+static void jacobi_sweeps(std::vector<double> &U, int m, int n, Mat *V,
+                          std::vector<double> &nrm) {
   auto col = [&](int j) { return U.data() + (std::size_t)j * m; };
   auto sq = [&](int j) {
     const double *u = col(j);
@@ -47,11 +46,12 @@ void svd_right(const Mat &A, std::vector<double> &sv, Mat &V) {
           wp[i] = c * x - s * y;
           wq[i] = s * x + c * y;
         }
-        for (int i = 0; i < n; ++i) {
-          const double x = V(i, p), y = V(i, q);
-          V(i, p) = c * x - s * y;
-          V(i, q) = s * x + c * y;
-        }
+        if (V)
+          for (int i = 0; i < n; ++i) {
+            const double x = (*V)(i, p), y = (*V)(i, q);
+            (*V)(i, p) = c * x - s * y;
+            (*V)(i, q) = s * x + c * y;
+          }
         // recompute the two norms exactly (keeps tiny singular values accurate)
         nrm[p] = sq(p);
         nrm[q] = sq(q);
@@ -59,6 +59,63 @@ void svd_right(const Mat &A, std::vector<double> &sv, Mat &V) {
     }
     if (rotations == 0)
       break;
+  }
+}
+
+// A matrix whose entries with i + j even are all zero (the refined
+// Golub-Kahan matrix with a zero diagonal) maps even columns to odd rows and
+// odd columns to even rows: two independent blocks. Their columns have
+// disjoint supports, so the sweeps over the whole matrix never rotate a pair
+// across blocks (g == 0 exactly) and never change an entry outside a
+// column's own rows; sweeping each block on its own rows only drops
+// additions and rotations of exact zeros, with the same rotation order
+// inside each block, and gives bitwise the same norms and V. With
+// even_only, the rotations of the odd-column block are not accumulated: its
+// columns of V stay unit vectors on odd rows.
+static bool jacobi_parity_blocks(const Mat &A, Mat &V, std::vector<double> &nrm,
+                                 bool even_only) {
+  const int m = A.r, n = A.c;
+  if (n < 2)
+    return false;
+  for (int j = 0; j < n; ++j)
+    for (int i = j % 2; i < m; i += 2)
+      if (A(i, j) != 0.0)
+        return false;
+  for (int b = 0; b < 2; ++b) {
+    std::vector<int> cols, rows;
+    for (int j = b; j < n; j += 2)
+      cols.push_back(j);
+    for (int i = 1 - b; i < m; i += 2)
+      rows.push_back(i);
+    const int nb = (int)cols.size(), mb = (int)rows.size();
+    std::vector<double> U((std::size_t)mb * nb);
+    for (int t = 0; t < nb; ++t)
+      for (int r = 0; r < mb; ++r)
+        U[(std::size_t)t * mb + r] = A(rows[r], cols[t]);
+    Mat Vb(nb, nb);
+    for (int t = 0; t < nb; ++t)
+      Vb(t, t) = 1.0;
+    std::vector<double> nb2(nb);
+    const bool vec = b == 0 || !even_only;
+    jacobi_sweeps(U, mb, nb, vec ? &Vb : nullptr, nb2);
+    for (int t = 0; t < nb; ++t) {
+      nrm[cols[t]] = nb2[t];
+      for (int r = 0; r < nb; ++r)
+        V(cols[r], cols[t]) = Vb(r, t);
+    }
+  }
+  return true;
+}
+
+void svd_right(const Mat &A, std::vector<double> &sv, Mat &V, bool split, bool even_only) {
+  const int m = A.r, n = A.c;
+  V = Mat(n, n);
+  std::vector<double> nrm(n);
+  if (!(split && jacobi_parity_blocks(A, V, nrm, even_only))) {
+    std::vector<double> U(A.a);  // columns of A, orthogonalised in place
+    for (int i = 0; i < n; ++i)
+      V(i, i) = 1.0;
+    jacobi_sweeps(U, m, n, &V, nrm);
   }
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);

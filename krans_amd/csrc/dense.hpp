@@ -18,8 +18,14 @@ struct Mat {
 };
 
 // Right singular system of A (rows >= cols): singular values descending and
-// V (cols x cols). One-sided Jacobi (Hestenes) with column-norm updates.
-void svd_right(const Mat &A, std::vector<double> &sv, Mat &V);
+// V (cols x cols). One-sided Jacobi (Hestenes) with column-norm updates; with
+// `split`, a matrix whose entries with i + j even are zero is swept as its
+// two independent blocks (bitwise the same result, half the work). With
+// even_only (and such a matrix) only singular vectors supported on even rows
+// are formed; the other columns of V are unit vectors on odd rows, and all
+// singular values are unchanged.
+void svd_right(const Mat &A, std::vector<double> &sv, Mat &V, bool split = true,
+               bool even_only = false);
 
 // Symmetric eigen-decomposition, eigenvalues ascending, eigenvectors in the
 // columns of Q: Householder tridiagonalisation + implicit QR with Wilkinson
