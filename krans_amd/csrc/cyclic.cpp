@@ -53,7 +53,7 @@ void readback(Context &ctx, const double *d, int n, double *out) {
     throw std::length_error("readback too large");
   KH_CHECK(hipMemcpyAsync(ctx.hscal, d, sizeof(double) * (std::size_t)n,
                           hipMemcpyDeviceToHost, ctx.stream));
-  KH_CHECK(hipStreamSynchronize(ctx.stream));
+  ctx.sync();
   std::copy(ctx.hscal, ctx.hscal + n, out);
 }
 
@@ -62,11 +62,7 @@ void upload_doubles(Context &ctx, const double *h, int n, double *d) {
     return;
   if (n > Context::kScalars)
     throw std::length_error("upload too large");
-  // stage through pinned memory; the sync keeps the staging buffer stable
-  std::copy(h, h + n, ctx.hscal);
-  KH_CHECK(hipMemcpyAsync(d, ctx.hscal, sizeof(double) * (std::size_t)n,
-                          hipMemcpyHostToDevice, ctx.stream));
-  KH_CHECK(hipStreamSynchronize(ctx.stream));
+  ctx.upload(d, h, sizeof(double) * (std::size_t)n);  // queued, no wait
 }
 
 // Upload a host vector of FP64 (any length) via pageable memory.
@@ -305,9 +301,7 @@ void CyclicDriver::push_basis(const double *v) {
 void CyclicDriver::upload_mask(const std::vector<int> &mask) {
   if (mask_.bytes < mask.size() * sizeof(int))
     mask_ = DBuf(std::max<std::size_t>(mask.size(), 64) * sizeof(int));
-  KH_CHECK(hipMemcpyAsync(mask_.p, mask.data(), mask.size() * sizeof(int),
-                          hipMemcpyHostToDevice, c_.stream));
-  KH_CHECK(hipStreamSynchronize(c_.stream));
+  c_.upload(mask_.p, mask.data(), mask.size() * sizeof(int));
 }
 
 void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,

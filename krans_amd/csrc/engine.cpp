@@ -27,6 +27,7 @@ Context::Context(int dev) : device(dev) {
   KH_CHECK(hipMalloc((void **)&dscal, sizeof(double) * kScalars));
   KH_CHECK(hipMemset(dscal, 0, sizeof(double) * kScalars));
   KH_CHECK(hipHostMalloc((void **)&hscal, sizeof(double) * kScalars, hipHostMallocDefault));
+  KH_CHECK(hipHostMalloc((void **)&hstage, kStage, hipHostMallocDefault));
   part_cap = (std::size_t)1 << 24;  // 16 Mi doubles = 128 MiB of reduction partials
   KH_CHECK(hipMalloc((void **)&dpart, sizeof(double) * part_cap));
   hipDeviceProp_t prop;
@@ -65,11 +66,29 @@ Context::~Context() {
     (void)hipFree(dscal);
   if (hscal)
     (void)hipHostFree(hscal);
+  if (hstage)
+    (void)hipHostFree(hstage);
   if (stream)
     (void)hipStreamDestroy(stream);
 }
 
-void Context::sync() { KH_CHECK(hipStreamSynchronize(stream)); }
+void Context::sync() {
+  KH_CHECK(hipStreamSynchronize(stream));
+  stage_off = 0;
+}
+
+void Context::upload(void *dst, const void *src, std::size_t bytes) {
+  if (bytes == 0)
+    return;
+  if (bytes > kStage)
+    throw std::length_error("upload larger than the staging area");
+  if (stage_off + bytes > kStage)
+    sync();  // every staged slice has been consumed
+  char *h = hstage + stage_off;
+  std::memcpy(h, src, bytes);
+  KH_CHECK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, stream));
+  stage_off += (bytes + 255) & ~std::size_t(255);
+}
 
 void CapturedWork::reset() {
   if (exec)

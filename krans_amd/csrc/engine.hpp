@@ -61,6 +61,12 @@ struct Context {
   hipStream_t stream = nullptr;
   double *dscal = nullptr;     // device scalar slab (kScalars doubles)
   double *hscal = nullptr;     // pinned host mirror
+  // Pinned staging for host-to-device uploads: each upload takes the next
+  // slice and is queued without waiting; slices are reused only after a
+  // wait on the stream (sync()), which rewinds the area.
+  char *hstage = nullptr;
+  std::size_t stage_off = 0;
+  static constexpr std::size_t kStage = std::size_t(4) << 20;
   double *dpart = nullptr;     // reduction partials
   std::size_t part_cap = 0;    // doubles
   int cu_count = 0;
@@ -69,7 +75,10 @@ struct Context {
   ~Context();
   Context(const Context &) = delete;
   Context &operator=(const Context &) = delete;
-  void sync();
+  void sync();  // waits for the stream and rewinds the upload staging
+  // queues a copy of `bytes` host bytes to device `dst`; the host source may
+  // change as soon as this returns
+  void upload(void *dst, const void *src, std::size_t bytes);
   // copy `n` device scalars starting at slab index `i0` into hscal[i0..]
   void fetch_scalars(int i0, int n);
   double *partials(std::size_t need);

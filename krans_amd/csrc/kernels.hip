@@ -2773,7 +2773,9 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     return;
   }
   // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
-  // unless the partials would not fit the context's reduction buffer
+  // unless the partials would not fit the context's reduction buffer (2048
+  // blocks of a quarter of the rows were no faster for the narrow products:
+  // their 32-row slabs touch every column 128 bytes at a time)
   const int64_t maxblk = std::max<int64_t>(
       1, (int64_t)(ctx.part_cap * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
   int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);

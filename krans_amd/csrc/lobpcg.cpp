@@ -34,7 +34,7 @@ void rb(Context &ctx, const double *d, int n, double *out) {
     throw std::length_error("readback too large");
   KH_CHECK(hipMemcpyAsync(ctx.hscal, d, sizeof(double) * (std::size_t)n,
                           hipMemcpyDeviceToHost, ctx.stream));
-  KH_CHECK(hipStreamSynchronize(ctx.stream));
+  ctx.sync();
   std::copy(ctx.hscal, ctx.hscal + n, out);
 }
 
@@ -43,10 +43,7 @@ void up(Context &ctx, const double *h, int n, double *d) {
     return;
   if (n > Context::kScalars)
     throw std::length_error("upload too large");
-  std::copy(h, h + n, ctx.hscal);
-  KH_CHECK(hipMemcpyAsync(d, ctx.hscal, sizeof(double) * (std::size_t)n,
-                          hipMemcpyHostToDevice, ctx.stream));
-  KH_CHECK(hipStreamSynchronize(ctx.stream));
+  ctx.upload(d, h, sizeof(double) * (std::size_t)n);  // queued, no wait
 }
 
 // A block of `cap` columns of length n in precision p.

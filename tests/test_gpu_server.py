@@ -24,14 +24,18 @@ def mix():
     return reqs
 
 
-def opts(k):
+def opts(k, graphs=0):
     return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
-                                  max_restarts=200, cg_maxit=400)
+                                  max_restarts=200, cg_maxit=400, graphs=graphs)
 
 
-def test_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_same_alone_and_in_the_mix(graphs):
+    """Solo solves with direct launches against the server's four slots;
+    with graphs=1 the slots capture and replay their Golub-Kahan cycles
+    concurrently (thread-local capture while other slots allocate)."""
     reqs = mix()
-    print("[case] server alone", flush=True)
+    print(f"[case] server alone (mix graphs={graphs})", flush=True)
     alone = []
     for i, (A, k, prec) in enumerate(reqs):
         r = K.find_nullspace_cyclic_lanczos_ex(K.HipCsrMatrixAdapter.from_csr(A),
@@ -40,9 +44,11 @@ def test_same_alone_and_in_the_mix():
         assert r.right_null.shape[1] == k, (i, r.right_null.shape)
     print("[case] server mix", flush=True)
     srv = K.NullspaceServer(slots=4)
-    ids = [srv.submit(A, opts(k), prec, seed=100 + i) for i, (A, k, prec) in enumerate(reqs)]
+    ids = [srv.submit(A, opts(k, graphs), prec, seed=100 + i)
+           for i, (A, k, prec) in enumerate(reqs)]
     for i, rid in enumerate(ids):
         U, info, lat, q = srv.wait(rid)
+        assert (info["graph_replays"] > 0) == (graphs == 1), (i, info["graph_replays"])
         assert U.shape == alone[i].shape
         assert np.array_equal(U.view(np.uint64), alone[i].view(np.uint64)), i
         assert lat >= q >= 0
