@@ -173,6 +173,8 @@ private:
   bool mgs(Blk &X, int i0, int cnt, double tol, bool reorth);
   bool orthonormalize(Blk &X, int i0, int cnt, double tol, bool reorth, Blk &scratch);
   void mgs_dev(Blk &X, int cnt, bool reorth, double *dn);
+  bool cholqr2(Blk &X, int i0, int cnt, Blk &scratch);
+  int64_t cholqr_calls_ = 0, cholqr_fallbacks_ = 0;
   void apply_normal(const Blk &Z, int q, Blk &AZ);
   // Y = op(A) X for q FP64 columns (ld = input / output length), through
   // row-major copies for two or more columns
@@ -280,6 +282,55 @@ void LobpcgDriver::mgs_dev(Blk &X, int cnt, bool reorth, double *dn) {
   }
 }
 
+// Phase 2 of orthonormalize for FP32 blocks of 4..128 columns as Cholesky QR
+// applied twice instead of column-by-column MGS with re-orthogonalisation:
+// G = Z^T Z accumulated in FP64 (products of FP32 values are exact in FP64),
+// G = L L^T on the host, Z <- Z L^-T on the matrix cores, twice. Two passes
+// over the block and two block products replace about 3 cnt^2 / 2 column
+// sweeps. The first factor's diagonal is, in exact arithmetic, the norms the
+// MGS pass would test against gs_tol (:150-157). A pivot below
+// kCholQrMinPivot marks nearly dependent columns, for which a Gram-based
+// factor loses the accuracy MGS keeps: then nothing has been written and the
+// caller runs MGS. On success X(:, i0:i0+cnt) holds the orthonormal block.
+constexpr double kCholQrMinPivot = 0.05;
+
+bool LobpcgDriver::cholqr2(Blk &X, int i0, int cnt, Blk &scratch) {
+  ++cholqr_calls_;
+  double *G = sc_ + 20000, *Ri = sc_ + 40000;  // cnt^2 <= 16384 each
+  std::vector<double> g((std::size_t)cnt * cnt);
+  for (int pass = 0; pass < 2; ++pass) {
+    // pass 0: scratch -> X(:, i0..); pass 1: X(:, i0..) -> scratch
+    const void *in = pass == 0 ? scratch.col(0) : X.col(i0);
+    void *out = pass == 0 ? X.col(i0) : scratch.col(0);
+    gemm_tn(c_, Prec::F32, in, n_, cnt, Prec::F32, in, n_, cnt, n_, G);
+    rb(c_, G, cnt * cnt, g.data());
+    dense::Mat S(cnt, cnt), L;
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < cnt; ++i)  // symmetrise (both triangles are summed alike)
+        S(i, j) = 0.5 * (g[(std::size_t)j * cnt + i] + g[(std::size_t)i * cnt + j]);
+    bool ok = dense::cholesky(S, L);
+    for (int i = 0; ok && i < cnt; ++i)
+      ok = std::isfinite(L(i, i)) && L(i, i) >= (pass == 0 ? kCholQrMinPivot : 0.5);
+    if (!ok) {
+      ++cholqr_fallbacks_;
+      if (pass == 1)  // MGS continues from the first pass's block
+        KH_CHECK(hipMemcpyAsync(scratch.col(0), X.col(i0), (std::size_t)cnt * n_ * 4,
+                                hipMemcpyDeviceToDevice, c_.stream));
+      return false;
+    }
+    const dense::Mat Li = dense::lower_inverse(L);
+    std::vector<double> r((std::size_t)cnt * cnt);  // L^-T, column-major
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < cnt; ++i)
+        r[(std::size_t)j * cnt + i] = Li(j, i);
+    up(c_, r.data(), cnt * cnt, Ri);
+    gemm_nn(c_, Prec::F32, in, n_, cnt, Ri, cnt, Prec::F32, out, n_, n_);
+  }
+  KH_CHECK(hipMemcpyAsync(X.col(i0), scratch.col(0), (std::size_t)cnt * n_ * 4,
+                          hipMemcpyDeviceToDevice, c_.stream));
+  return true;
+}
+
 // :137-160. Phase 1 (each column against the locked basis, then normalised)
 // runs on all columns at once in the scratch block; phase 2 (MGS with
 // re-orthogonalisation on the normalised copy, as the reference's clones)
@@ -326,6 +377,8 @@ bool LobpcgDriver::orthonormalize(Blk &X, int i0, int cnt, double tol, bool reor
                           hipMemcpyDeviceToDevice, c_.stream));
   if (fail >= 0)
     return false;
+  if (X.p == Prec::F32 && reorth && cnt >= 4 && cnt <= 128 && cholqr2(X, i0, cnt, scratch))
+    return true;
   mgs_dev(scratch, cnt, reorth, dn);
   std::vector<double> n12((std::size_t)2 * cnt);
   rb(c_, dn, 2 * cnt, n12.data());
@@ -1114,9 +1167,11 @@ DeviceBasis LobpcgDriver::run() {
   if (P_.verbose)
     std::fprintf(stderr,
                  "[LOBPCG-HIP] done: it=%lld locked=%lld random columns=%lld (%.3f s) "
-                 "tiny GEPs %.3f s host total %.3f s P drops %lld deferred refinements %lld\n",
+                 "tiny GEPs %.3f s host total %.3f s P drops %lld deferred refinements %lld "
+                 "Cholesky-QR blocks %lld (MGS fallbacks %lld)\n",
                  (long long)its, (long long)S_.locked, (long long)S_.random_cols, S_.t_rng,
-                 S_.t_gep, S_.t_host, (long long)S_.p_drops, (long long)S_.deferred);
+                 S_.t_gep, S_.t_host, (long long)S_.p_drops, (long long)S_.deferred,
+                 (long long)cholqr_calls_, (long long)cholqr_fallbacks_);
   if (locked_.count > 0) {  // final polish (:806-831)
     const double desired = accept_thr;
     const double target = std::max(1e-14, std::min(1e-10, 0.01 * desired));
