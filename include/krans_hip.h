@@ -92,6 +92,8 @@ typedef struct kh_lobpcg_result {
   double frob_norm_est, accept_threshold, best_unaccepted_residual;
   int64_t block_size, converged_to_target, spmv, spmtv;
   double t_gram, t_sparse, t_host;
+  /* FP32 blocks orthonormalised by Cholesky QR, and those sent on to MGS */
+  int64_t cholqr_blocks, cholqr_fallbacks;
 } kh_lobpcg_result;
 
 const char *kh_last_error(void);

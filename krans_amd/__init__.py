@@ -79,6 +79,7 @@ class _LobpcgRes(C.Structure):
         ("best_unaccepted_residual", C.c_double), ("block_size", C.c_int64),
         ("converged_to_target", C.c_int64), ("spmv", C.c_int64), ("spmtv", C.c_int64),
         ("t_gram", C.c_double), ("t_sparse", C.c_double), ("t_host", C.c_double),
+        ("cholqr_blocks", C.c_int64), ("cholqr_fallbacks", C.c_int64),
     ]
 
 
@@ -586,6 +587,8 @@ class LOBPCGResult:
     spmv: int = 0
     spmtv: int = 0
     times: dict = field(default_factory=dict)
+    cholqr_blocks: int = 0     # FP32 blocks orthonormalised by Cholesky QR
+    cholqr_fallbacks: int = 0  # ... of which nearly dependent ones went on to MGS
 
 
 def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
@@ -616,7 +619,8 @@ def find_nullspace_lobpcg_ex(A, vf, opt, precision="fp64"):
         accept_threshold=res.accept_threshold,
         best_unaccepted_residual=res.best_unaccepted_residual, block_size=res.block_size,
         converged_to_target=bool(res.converged_to_target), spmv=res.spmv, spmtv=res.spmtv,
-        times={"gram": res.t_gram, "sparse": res.t_sparse, "host": res.t_host})
+        times={"gram": res.t_gram, "sparse": res.t_sparse, "host": res.t_host},
+        cholqr_blocks=res.cholqr_blocks, cholqr_fallbacks=res.cholqr_fallbacks)
 
 
 def find_nullspace_lobpcg(A, vf, opt, precision="fp64"):

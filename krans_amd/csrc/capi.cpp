@@ -446,6 +446,8 @@ int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt, int preci
     res->t_gram = S.t_gram;
     res->t_sparse = S.t_sparse;
     res->t_host = S.t_host;
+    res->cholqr_blocks = S.cholqr_blocks;
+    res->cholqr_fallbacks = S.cholqr_fallbacks;
   });
 }
 

@@ -174,7 +174,6 @@ private:
   bool orthonormalize(Blk &X, int i0, int cnt, double tol, bool reorth, Blk &scratch);
   void mgs_dev(Blk &X, int cnt, bool reorth, double *dn);
   bool cholqr2(Blk &X, int i0, int cnt, Blk &scratch);
-  int64_t cholqr_calls_ = 0, cholqr_fallbacks_ = 0;
   void apply_normal(const Blk &Z, int q, Blk &AZ);
   // Y = op(A) X for q FP64 columns (ld = input / output length), through
   // row-major copies for two or more columns
@@ -295,7 +294,7 @@ void LobpcgDriver::mgs_dev(Blk &X, int cnt, bool reorth, double *dn) {
 constexpr double kCholQrMinPivot = 0.05;
 
 bool LobpcgDriver::cholqr2(Blk &X, int i0, int cnt, Blk &scratch) {
-  ++cholqr_calls_;
+  ++S_.cholqr_blocks;
   double *G = sc_ + 20000, *Ri = sc_ + 40000;  // cnt^2 <= 16384 each
   std::vector<double> g((std::size_t)cnt * cnt);
   for (int pass = 0; pass < 2; ++pass) {
@@ -312,7 +311,7 @@ bool LobpcgDriver::cholqr2(Blk &X, int i0, int cnt, Blk &scratch) {
     for (int i = 0; ok && i < cnt; ++i)
       ok = std::isfinite(L(i, i)) && L(i, i) >= (pass == 0 ? kCholQrMinPivot : 0.5);
     if (!ok) {
-      ++cholqr_fallbacks_;
+      ++S_.cholqr_fallbacks;
       if (pass == 1)  // MGS continues from the first pass's block
         KH_CHECK(hipMemcpyAsync(scratch.col(0), X.col(i0), (std::size_t)cnt * n_ * 4,
                                 hipMemcpyDeviceToDevice, c_.stream));
@@ -1171,7 +1170,7 @@ DeviceBasis LobpcgDriver::run() {
                  "Cholesky-QR blocks %lld (MGS fallbacks %lld)\n",
                  (long long)its, (long long)S_.locked, (long long)S_.random_cols, S_.t_rng,
                  S_.t_gep, S_.t_host, (long long)S_.p_drops, (long long)S_.deferred,
-                 (long long)cholqr_calls_, (long long)cholqr_fallbacks_);
+                 (long long)S_.cholqr_blocks, (long long)S_.cholqr_fallbacks);
   if (locked_.count > 0) {  // final polish (:806-831)
     const double desired = accept_thr;
     const double target = std::max(1e-14, std::min(1e-10, 0.01 * desired));

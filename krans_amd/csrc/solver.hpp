@@ -226,6 +226,7 @@ struct LobpcgStats {
   int64_t p_drops = 0;  // FP32 steps restarted without P (dependent [X W P])
   int64_t deferred = 0;  // FP32 iterations whose candidates waited for a batch
   int64_t random_cols = 0;  // columns drawn from the random stream
+  int64_t cholqr_blocks = 0, cholqr_fallbacks = 0;  // FP32 blocks: Cholesky QR / MGS after it
   double t_rng = 0, t_gep = 0;  // host time: random columns, tiny GEPs
   double t_gram = 0, t_sparse = 0, t_host = 0;
 };

@@ -56,6 +56,7 @@ def test_rg_block_kernel_fp64():
     assert residuals(A, U).max() <= r.accept_threshold
     assert principal_sine(U, Kx) <= 1e-8
     assert np.abs(U.T @ U - np.eye(p)).max() <= 1e-10
+    assert r.cholqr_blocks == 0  # FP64 blocks keep the reference's MGS
 
 
 def test_lobpcg_small_analytic_and_identity():
@@ -115,4 +116,7 @@ def test_lobpcg_mixed_precision(case):
     assert residuals(A, U).max() <= r.accept_threshold
     assert principal_sine(U, Kx) <= 1e-8
     assert np.abs(U.T @ U - np.eye(nev)).max() <= 1e-10
-    print("mixed iters", r.iters)
+    print("mixed iters", r.iters, "cholesky-qr blocks", r.cholqr_blocks, "fallbacks",
+          r.cholqr_fallbacks)
+    if nev >= 4:  # FP32 blocks of 4 or more columns take Cholesky QR
+        assert r.cholqr_blocks > 0 and r.cholqr_fallbacks <= r.cholqr_blocks
