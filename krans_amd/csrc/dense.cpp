@@ -2,8 +2,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <limits>
 #include <numeric>
+#include <thread>
 
 namespace kh::dense {
 
@@ -81,7 +83,9 @@ static bool jacobi_parity_blocks(const Mat &A, Mat &V, std::vector<double> &nrm,
     for (int i = j % 2; i < m; i += 2)
       if (A(i, j) != 0.0)
         return false;
-  for (int b = 0; b < 2; ++b) {
+  // the two blocks write disjoint columns of V and entries of nrm; from 16
+  // columns each, the odd block runs on a second thread
+  auto sweep_block = [&](int b) {
     std::vector<int> cols, rows;
     for (int j = b; j < n; j += 2)
       cols.push_back(j);
@@ -103,6 +107,28 @@ static bool jacobi_parity_blocks(const Mat &A, Mat &V, std::vector<double> &nrm,
       for (int r = 0; r < nb; ++r)
         V(cols[r], cols[t]) = Vb(r, t);
     }
+  };
+  if (n >= 32) {
+    std::exception_ptr err;
+    std::thread odd([&] {
+      try {
+        sweep_block(1);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    try {
+      sweep_block(0);
+    } catch (...) {
+      odd.join();
+      throw;
+    }
+    odd.join();
+    if (err)
+      std::rethrow_exception(err);
+  } else {
+    sweep_block(0);
+    sweep_block(1);
   }
   return true;
 }
