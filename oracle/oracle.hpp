@@ -214,7 +214,7 @@ double orthogonalize_against_basis(Vec &v, const std::vector<Vec> &basis,
                                    double tol, bool reorth);
 double max_pairwise_abs_dot(const std::vector<Vec> &basis);
 
-// ---- judge-side metrics -------------------------------------------------------
+// ---- basis comparison metrics ------------------------------------------------
 // SubspaceCompare.hpp:17-110 (rows = basis vectors).
 void compute_rref(std::vector<Vec> &rows, double tol);
 double subspace_diff(const std::vector<Vec> &b1, const std::vector<Vec> &b2,
