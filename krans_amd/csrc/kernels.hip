@@ -1455,14 +1455,28 @@ __device__ inline int g2_ntiles(int mode, int s, int q) {
   return mode == 0 ? nq * (nq + 1) / 2 + nq * nq : ns * nq;
 }
 
-template <class T>
+// SLAB rows per LDS slab (a multiple of 4; SLAB * MAXQ a multiple of 512),
+// at most MAXQ columns per operand (a multiple of 16). The Gram pair uses 32 x
+// 128; the tall-skinny products of at most 64 columns use 96-row (FP32) or
+// 48-row (FP64) slabs, so every column is read 384 bytes at a time instead of
+// 128 (the short runs held those products near 1 TB/s).
+template <class T, int SLAB = kG2Slab, int MAXQ = kG2MaxQ>
 __global__ void __launch_bounds__(kG2Threads)
     k_gram2_partial(int64_t len, int64_t rows_per_block, const T *__restrict__ Z,
                     int64_t ldz, int s, const T *__restrict__ AZ, int64_t ldaz, int q,
                     int mode, T *__restrict__ part) {
   using V4 = typename Mfma16<T>::V;
-  __shared__ T Zs[kG2MaxQ * kG2Ld];
-  __shared__ T As[kG2MaxQ * kG2Ld];
+  constexpr int LD = SLAB + 2;
+  constexpr int MAXT = MAXQ / 16;
+  // tiles per wave: the full-width kernel also serves the Gram pair (mode 0,
+  // triangle of S plus all of G); the narrower ones only V^T W (mode 1)
+  constexpr int MAX_TILES =
+      MAXQ == kG2MaxQ ? MAXT * (MAXT + 1) / 2 + MAXT * MAXT : MAXT * MAXT;
+  constexpr int PER_WAVE = (MAX_TILES + kG2Waves - 1) / kG2Waves;
+  static_assert(SLAB % 4 == 0 && (SLAB * MAXQ) % kG2Threads == 0 && MAXQ % 16 == 0,
+                "slab shape");
+  __shared__ T Zs[MAXQ * LD];
+  __shared__ T As[MAXQ * LD];
   const int nt = (q + 15) / 16;
   const int SP = ((s + 15) / 16) * 16, QP = nt * 16;
   const int ntiles = g2_ntiles(mode, s, q);
@@ -1470,10 +1484,10 @@ __global__ void __launch_bounds__(kG2Threads)
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < len ? r0 + rows_per_block : len;
   // this wave's tiles: operand offsets in LDS (floats)
-  int aoff[kG2PerWave], boff[kG2PerWave];
-  bool bs[kG2PerWave];
+  int aoff[PER_WAVE], boff[PER_WAVE];
+  bool bs[PER_WAVE];
 #pragma unroll
-  for (int u = 0; u < kG2PerWave; ++u) {
+  for (int u = 0; u < PER_WAVE; ++u) {
     const int t = wid + u * kG2Waves;
     int ti = 0, tj = 0;
     bool is_s = true;
@@ -1486,23 +1500,23 @@ __global__ void __launch_bounds__(kG2Threads)
         is_s = false;
       }
     }
-    aoff[u] = (16 * ti + (lane & 15)) * kG2Ld + (lane >> 4);
-    boff[u] = (16 * tj + (lane & 15)) * kG2Ld + (lane >> 4);
+    aoff[u] = (16 * ti + (lane & 15)) * LD + (lane >> 4);
+    boff[u] = (16 * tj + (lane & 15)) * LD + (lane >> 4);
     bs[u] = is_s;
   }
-  V4 acc[kG2PerWave];
+  V4 acc[PER_WAVE];
 #pragma unroll
-  for (int u = 0; u < kG2PerWave; ++u)
+  for (int u = 0; u < PER_WAVE; ++u)
     acc[u] = V4{(T)0, (T)0, (T)0, (T)0};
-  // staging: element e = tid + u*512 of the (32 rows x QP cols) slab of Z and
-  // of AZ: row e % 32, column e / 32
-  constexpr int kLd = kG2Slab * kG2MaxQ / kG2Threads;  // 8 per operand
+  // staging: element e = tid + u*512 of the (SLAB rows x MAXQ cols) slab of Z
+  // and of AZ: row e % SLAB, column e / SLAB
+  constexpr int kLd = SLAB * MAXQ / kG2Threads;  // loads per operand and thread
   T za[kLd], aa[kLd];
   auto load = [&](int64_t rb) {
 #pragma unroll
     for (int u = 0; u < kLd; ++u) {
       const int e = threadIdx.x + u * kG2Threads;
-      const int r = e % kG2Slab, c = e / kG2Slab;
+      const int r = e % SLAB, c = e / SLAB;
       const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
       za[u] = c < s ? Z[row + (int64_t)c * ldz] : (T)0;
       aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : (T)0;
@@ -1510,25 +1524,25 @@ __global__ void __launch_bounds__(kG2Threads)
   };
   if (r0 < r1)
     load(r0);
-  for (int64_t rb = r0; rb < r1; rb += kG2Slab) {
+  for (int64_t rb = r0; rb < r1; rb += SLAB) {
     __syncthreads();  // previous slab consumed
 #pragma unroll
     for (int u = 0; u < kLd; ++u) {
       const int e = threadIdx.x + u * kG2Threads;
-      const int r = e % kG2Slab, c = e / kG2Slab;
+      const int r = e % SLAB, c = e / SLAB;
       const bool inr = (rb + r) < r1;
       if (c < SP)
-        Zs[c * kG2Ld + r] = (inr && c < s) ? za[u] : (T)0;
+        Zs[c * LD + r] = (inr && c < s) ? za[u] : (T)0;
       if (c < QP)
-        As[c * kG2Ld + r] = (inr && c < q) ? aa[u] : (T)0;
+        As[c * LD + r] = (inr && c < q) ? aa[u] : (T)0;
     }
     __syncthreads();
-    if (rb + kG2Slab < r1)
-      load(rb + kG2Slab);
+    if (rb + SLAB < r1)
+      load(rb + SLAB);
+#pragma unroll 8
+    for (int k = 0; k < SLAB; k += 4) {
 #pragma unroll
-    for (int k = 0; k < kG2Slab; k += 4) {
-#pragma unroll
-      for (int u = 0; u < kG2PerWave; ++u) {
+      for (int u = 0; u < PER_WAVE; ++u) {
         if (wid + u * kG2Waves < ntiles) {
           const T a = Zs[aoff[u] + k];
           const T b = bs[u] ? Zs[boff[u] + k] : As[boff[u] + k];
@@ -1539,7 +1553,7 @@ __global__ void __launch_bounds__(kG2Threads)
   }
   T *out = part + (int64_t)blockIdx.x * ntiles * 256;
 #pragma unroll
-  for (int u = 0; u < kG2PerWave; ++u) {
+  for (int u = 0; u < PER_WAVE; ++u) {
     const int t = wid + u * kG2Waves;
     if (t < ntiles)
 #pragma unroll
@@ -2772,24 +2786,33 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     KH_CHECK(hipGetLastError());
     return;
   }
-  // about 512 blocks, at most 4096 rows (FP32 accumulation length) each,
-  // unless the partials would not fit the context's reduction buffer (2048
-  // blocks of a quarter of the rows were no faster for the narrow products:
-  // their 32-row slabs touch every column 128 bytes at a time)
+  // Tall-skinny products of at most 64 columns per operand read each column
+  // in 384-byte runs (96-row FP32 / 48-row FP64 slabs); the Gram pair and
+  // wider products keep 32-row slabs of up to 128 columns.
+  constexpr int kWideSlab = sizeof(T) == 4 ? 96 : 48;
+  const bool wide = mode == 1 && s <= 64 && q <= 64;
+  const int slab = wide ? kWideSlab : kG2Slab;
+  // about 512 blocks, at most about 4096 rows (FP32 accumulation length)
+  // each, unless the partials would not fit the context's reduction buffer
   const int64_t maxblk = std::max<int64_t>(
       1, (int64_t)(ctx.part_cap * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
   int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);
   rpb = std::min<int64_t>(rpb, 4096);
   rpb = std::max<int64_t>(rpb, (len + maxblk - 1) / maxblk);
-  rpb = (rpb + kG2Slab - 1) / kG2Slab * kG2Slab;
+  rpb = (rpb + slab - 1) / slab * slab;
   const int64_t nblk64 = (len + rpb - 1) / rpb;
   const std::size_t need = (std::size_t)nblk64 * ntiles * 256;  // elements
   T *part = reinterpret_cast<T *>(ctx.partials((need * sizeof(T) + 7) / 8));
   {
     TraceScope tr(ctx, (mode == 0 && sizeof(T) == 4) ? kTraceGram : kTraceNone,
                   2.0 * (double)len * s * q * (mode == 0 ? 2.0 : 1.0));
-    hipLaunchKernelGGL(k_gram2_partial<T>, dim3((unsigned)nblk64), dim3(kG2Threads), 0,
-                       ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, mode, part);
+    if (wide)
+      hipLaunchKernelGGL((k_gram2_partial<T, kWideSlab, 64>), dim3((unsigned)nblk64),
+                         dim3(kG2Threads), 0, ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q,
+                         mode, part);
+    else
+      hipLaunchKernelGGL(k_gram2_partial<T>, dim3((unsigned)nblk64), dim3(kG2Threads), 0,
+                         ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, mode, part);
   }
   if (mode == 0) {
     const int64_t outs = 2 * (int64_t)q * q * 64;  // one wave per output

@@ -141,6 +141,26 @@ def test_incidence_small_matches_exact_kernel(precision):
           ro.cg_solves)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_incidence_k24_wide_window_products(precision):
+    """k = 24: the window-against-basis products grow past 16 columns and
+    take the 64-column MFMA tiles (96-row FP32 / 48-row FP64 slabs), in the
+    FP32 screen and in the FP64 evaluation."""
+    g, v, k = 24, 512, 24
+    A = W.regular_incidence(g, v, seed=3)
+    Kx = W.component_indicators(g, v)
+    print(f"[case] incidence k=24 {precision}", flush=True)
+    opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
+                                 max_restarts=400)
+    r = K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(A.n, 42), opt, precision)
+    U = r.right_null
+    assert U.shape[1] == k and r.converged_to_target
+    assert residuals(A, U).max() <= 1e-12 * r.frob_norm_est
+    assert principal_sine(U, Kx) <= 5e-9
+    assert np.abs(U.T @ U - np.eye(k)).max() <= 1e-10
+    print(precision, "restarts", r.restarts, "screened", r.screened)
+
+
 def test_small_2x3_analytic():
     # tests/test_cyclic_lanczos.cpp:144-190
     A = csr_small_2x3()

@@ -216,7 +216,8 @@ def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
 
 
 @pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (16, 16, 1_000_003, 0), (11, 15, 257, 0),
-                                        (64, 64, 300_001, 0), (97, 33, 70_000, 0),
+                                        (64, 64, 300_001, 0), (17, 33, 70_001, 0),
+                                        (64, 5, 1_000_003, 0), (97, 33, 70_000, 0),
                                         (128, 128, 40_000, 0), (130, 9, 5000, 0),
                                         (40, 24, 100_000, 1)])
 def test_gemm_tn_blocks(ctx, s, q, n, prec):
