@@ -79,7 +79,10 @@ struct Group {  // a block of vectors to orthogonalise against
   Prec p;
 };
 
-// Automatic graph replay of the Golub-Kahan cycle up to this m + n.
+// Automatic graph replay of the Golub-Kahan cycle up to this m + n. Zero:
+// on MI355X replay was slower than direct launches at every size measured
+// (n = 8K to 1M; the cycle waits on dependent-dispatch latency, not on host
+// launch cost), so the automatic setting launches directly.
 constexpr int64_t kGraphAutoLen = 0;
 
 class CyclicDriver {
