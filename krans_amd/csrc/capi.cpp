@@ -486,6 +486,7 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
                    double *latency_s, double *queued_s) {
   return guard([&] {
     need(s && res, "null argument");
+    need(cap >= 0, "negative basis capacity");
     kh::Server::Result r = s->srv->wait(id);
     std::memset(res, 0, sizeof(*res));
     if (r.status == 1)

@@ -140,9 +140,9 @@ private:
   void push_basis(const double *v);
   // candidate window processing
   struct ColOut {
-    bool valid = false, hit = false;
+    bool valid = false, hit = false;  // hit: some pass stopped at cg_maxit
     double nv = 0, res_raw = 0, res = 0;
-    int passes = 0;
+    int passes = 0, hits = 0;         // CG solves, and those that hit cg_maxit
   };
   void polish_batch(const std::vector<int> &cols, std::vector<ColOut> &out);
   bool fallback_one();
@@ -311,9 +311,13 @@ void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const doub
                             int64_t maxit, double rel_tol, std::vector<CgState> &st) {
   cg_solve_ops(c_, A_, cg_, q, rhs, w, pc, diag, maxit, rel_tol, P_.cg_breakdown_tol, st);
   for (int c = 0; c < q; ++c) {
-    S_.cg_iters += st[(std::size_t)c].k;
-    S_.spmv += st[(std::size_t)c].k;
-    S_.spmtv += st[(std::size_t)c].k;
+    const CgState &s = st[(std::size_t)c];
+    // an iteration that stops on breakdown (reason 2) or a non-finite r.z
+    // (3) has applied A A^T before stopping (CyclicLanczosSolver.cpp:79-95)
+    const int64_t prods = s.k + ((s.reason == 2 || s.reason == 3) ? 1 : 0);
+    S_.cg_iters += s.k;
+    S_.spmv += prods;
+    S_.spmtv += prods;
   }
 }
 
@@ -379,8 +383,10 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
     for (int a = 0; a < qa; ++a) {
       ColOut &o = out[(std::size_t)cols[(std::size_t)act[(std::size_t)a]]];
       ++o.passes;
-      if (st[(std::size_t)a].k >= P_.cg_maxit)
+      if (st[(std::size_t)a].k >= P_.cg_maxit) {  // :522-526, counted per pass
+        ++o.hits;
         o.hit = true;
+      }
     }
     // v' = v - A^T w; GS vs basis; normalise; ||A v'||
     block_product(c_, A_, Prec::F64, qa, Wm64, m_, Tn, n_, true, nullptr, rin_, rout_);
@@ -896,8 +902,7 @@ DeviceBasis CyclicDriver::run() {
         const int slot = slot_of_[(std::size_t)t];
         const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)t * n_;
         S_.cg_solves += o.passes;
-        if (o.hit)
-          S_.cg_maxit_hits += 1;  // one counted hit per column and pass set
+        S_.cg_maxit_hits += o.hits;
         if (P_.verbose)
           std::fprintf(stderr, "[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
                       (long long)restart, o.res_raw, o.res, accept_thr_);

@@ -44,6 +44,15 @@ struct Server::Req {
   CyclicStats stats;
 };
 
+// Requests return the right null space only: the left space of
+// find_nullspace_cyclic_lanczos_ex (CyclicLanczosSolver.cpp:618-660) is a
+// second solve on A^T, submitted as its own request by the caller.
+void Server::check_params(const CyclicParams &P) {
+  if (P.return_left_space)
+    throw std::invalid_argument(
+        "server: return_left_space is not served; submit A^T as its own request");
+}
+
 Server::Server(int device, int slots, double hbm_budget)
     : device_(device), budget_(hbm_budget) {
   if (slots < 1 || slots > 64)
@@ -73,10 +82,15 @@ int64_t Server::submit(int64_t m, int64_t n, const int32_t *rowptr, const int32_
                        const double *val, const CyclicParams &P, unsigned seed) {
   if (m < 0 || n < 0 || !rowptr)
     throw std::invalid_argument("server: bad matrix");
+  check_params(P);
+  const int64_t nnz = rowptr[m];
+  if (nnz < 0 || rowptr[0] != 0)
+    throw std::invalid_argument("server: bad row pointers");
+  if (nnz > 0 && (!col || !val))
+    throw std::invalid_argument("server: null column indices or values");
   auto r = std::make_shared<Req>();
   r->m = m;
   r->n = n;
-  const int64_t nnz = rowptr[m];
   r->rp.assign(rowptr, rowptr + m + 1);
   r->ci.assign(col, col + nnz);
   r->va.assign(val, val + nnz);
@@ -89,6 +103,9 @@ int64_t Server::submit(int64_t m, int64_t n, const int32_t *rowptr, const int32_
 }
 
 int64_t Server::submit_resident(const DeviceCsr *A, const CyclicParams &P, unsigned seed) {
+  if (!A)
+    throw std::invalid_argument("server: null matrix");
+  check_params(P);
   auto r = std::make_shared<Req>();
   r->m = A->m;
   r->n = A->n;
@@ -117,8 +134,12 @@ void Server::work(int slot) {
   try {
     ctx = std::make_unique<Context>(device_);
   } catch (const std::exception &e) {
-    std::lock_guard<std::mutex> lk(mu_);
-    slot_errors_.push_back(e.what());
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      slot_errors_.push_back(e.what());
+    }
+    // a client may already wait for a request that no slot can now run
+    cv_.notify_all();
     return;
   }
   (void)slot;

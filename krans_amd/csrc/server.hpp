@@ -48,6 +48,7 @@ public:
 
 private:
   struct Req;
+  static void check_params(const CyclicParams &P);
   int64_t enqueue(std::shared_ptr<Req> r);
   void work(int slot);
   int workers_alive() const;

@@ -44,3 +44,30 @@ def random_sparse(m, n, per_row, seed):
     np.add.at(rp, r + 1, 1)
     rp = np.cumsum(rp).astype(np.int32)
     return W.CsrMatrix(m, n, rp, c.astype(np.int32), v)
+
+
+# Two projector passes at cg_maxit 2 on the incidence of two 64-vertex
+# graphs: 5 CG solves, all at the limit (the count per pass of
+# CyclicLanczosSolver.cpp:522-526; one per candidate would give 4).
+TWO_PASS_OPTS = dict(k=2, projector_passes=2, cg_maxit=2, stagnation_restarts=4,
+                     max_restarts=4, tol_rel=1e-12, tol_abs=0.0, grad_max_iters=10)
+
+
+def two_pass_case():
+    return W.regular_incidence(2, 64, seed=3)
+
+
+def gradient_steps_closed_form(B, s, v0, thr, max_steps=100000):
+    """Steps of v <- (v - tau A^T A v)/||.|| until ||A v|| <= thr, with
+    A = s B and tau = 0.9 / max(1e-30, ||A||_F~^2) = 0.9e30 (the estimate is
+    below 1e-30 here): v_t is proportional to (I - tau A^T A)^t v0, evaluated
+    in the eigenbasis of B^T B."""
+    lam, Q = np.linalg.eigh(B.T @ B)
+    c = Q.T @ v0
+    f = 1.0 - 0.9e30 * s * s * lam
+    for t in range(max_steps + 1):
+        ct = c * f ** t
+        v = Q @ (ct / np.linalg.norm(ct))
+        if s * np.linalg.norm(B @ v) <= thr:
+            return t
+    return None
