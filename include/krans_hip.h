@@ -73,6 +73,9 @@ typedef struct kh_cyclic_result {
   double t_gk, t_tiny, t_cand, t_polish, t_fallback, t_rng;
   int64_t screened; /* mixed precision: candidates rejected by the FP32 screen */
   int64_t graph_captures, graph_replays; /* Golub-Kahan cycles as HIP graphs */
+  /* mixed precision: locked vectors replaced by the final batched FP64
+   * projector pass (those above max(1e-14, min(1e-10, 0.01 thr))) */
+  int64_t refined;
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and

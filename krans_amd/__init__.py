@@ -57,6 +57,7 @@ class _CyclicRes(C.Structure):
         ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
         ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
+        ("refined", C.c_int64),
     ]
 
 
@@ -500,6 +501,7 @@ class CyclicLanczosResult:
     screened: int = 0  # mixed precision: candidates rejected by the FP32 screen
     graph_captures: int = 0  # Golub-Kahan cycles captured as HIP graphs
     graph_replays: int = 0   # ... and replayed
+    refined: int = 0  # mixed: locked vectors replaced by the final FP64 projector pass
 
 
 def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
@@ -530,7 +532,8 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         frob_norm_est=res.frob_norm_est, converged_to_target=bool(res.converged_to_target),
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
-        graph_captures=res.graph_captures, graph_replays=res.graph_replays)
+        graph_captures=res.graph_captures, graph_replays=res.graph_replays,
+        refined=res.refined)
 
 
 def find_nullspace_cyclic_lanczos(A, vf, opt, precision="fp64"):

@@ -392,6 +392,7 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     res->t_fallback = S.t.fallback + SL.t.fallback;
     res->t_rng = S.t.rng + SL.t.rng;
     res->screened = S.screened + SL.screened;
+    res->refined = S.refined + SL.refined;
     res->graph_captures = S.graph_captures + SL.graph_captures;
     res->graph_replays = S.graph_replays + SL.graph_replays;
   });
@@ -515,6 +516,7 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->t_fallback = r.stats.t.fallback;
     res->t_rng = r.stats.t.rng;
     res->screened = r.stats.screened;
+    res->refined = r.stats.refined;
     res->graph_captures = r.stats.graph_captures;
     res->graph_replays = r.stats.graph_replays;
     if (latency_s)

@@ -146,6 +146,8 @@ private:
   };
   void polish_batch(const std::vector<int> &cols, std::vector<ColOut> &out);
   bool fallback_one();
+  void refine_basis();
+  DeviceBasis finish();
   void cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
                 int64_t maxit, double rel_tol, std::vector<CgState> &st);
   void upload_mask(const std::vector<int> &mask);
@@ -932,10 +934,8 @@ DeviceBasis CyclicDriver::run() {
       single_next = cut;
     }
     S_.t.cand += since(t0);
-    if (done_all) {
-      c_.sync();
-      return std::move(basis_);
-    }
+    if (done_all)
+      return finish();
     // ---- stagnation guard (:569-581)
     if (!accepted_cycle) {
       ++restarts_since_accept;
@@ -968,8 +968,111 @@ DeviceBasis CyclicDriver::run() {
     S_.converged_to_target = basis_.count >= k_target_;
     S_.t.fallback += since(t0);
   }
+  return finish();
+}
+
+DeviceBasis CyclicDriver::finish() {
+  if (P_.polish == Prec::F32) {
+    auto t0 = Clock::now();
+    refine_basis();
+    S_.t.polish += since(t0);
+  }
   c_.sync();
   return std::move(basis_);
+}
+
+// Mixed precision only: the FP32 projector polish accepts a vector once its
+// FP64 residual is below the threshold, which can leave it just below it.
+// One batched FP64 projector pass v <- v - A^T (A A^T)^{-1} A v (FP64 CG at
+// cg_rel_tol, the Jacobi preconditioner when set) refines every locked vector
+// whose residual is above the final-polish target of LOBPCGSolver.cpp:806-811,
+// max(1e-14, min(1e-10, 0.01 thr)); a refined vector replaces the locked one
+// only if its normalised residual dropped (the pass rule of
+// CyclicLanczosSolver.cpp:527-539), and the basis is then orthonormalised
+// again in place (CGS2, column by column). A basis already below the target
+// costs one block product and is left bitwise unchanged.
+void CyclicDriver::refine_basis() {
+  const int k = (int)basis_.count;
+  if (k == 0)
+    return;
+  const double target = std::max(1e-14, std::min(1e-10, 0.01 * accept_thr_));
+  constexpr int kChunk = 64;  // window blocks and scalar slots hold 64 columns
+  ensure_window(std::min(k, kChunk));
+  double *AV = AV_.as<double>(), *Vs = Vs_.as<double>(), *Rs = Rs_.as<double>();
+  double *Vp = Vp_.as<double>(), *AVp = AVp_.as<double>(), *Tn = Tn_.as<double>();
+  double *Wm64 = Wm64_.as<double>();
+  auto dcopy = [&](double *dst, const double *src, int64_t len) {
+    KH_CHECK(hipMemcpyAsync(dst, src, (std::size_t)len * 8, hipMemcpyDeviceToDevice,
+                            c_.stream));
+  };
+  const double *diag = P_.jacobi_AAt ? A_.diag_AAt() : nullptr;
+  int replaced = 0;
+  for (int j0 = 0; j0 < k; j0 += kChunk) {
+    const int kc = std::min(kChunk, k - j0);
+    block_product(c_, A_, Prec::F64, kc, basis_.col(j0), n_, AV, m_, false, nullptr, rin_,
+                  rout_);
+    S_.spmv += kc;
+    col_norms(c_, Prec::F64, AV, m_, kc, m_, wsc_);
+    std::vector<double> res((std::size_t)kc);
+    readback(c_, wsc_, kc, res.data());
+    std::vector<int> sel;  // chunk columns above the target
+    for (int j = 0; j < kc; ++j)
+      if (std::isfinite(res[(std::size_t)j]) && res[(std::size_t)j] > target)
+        sel.push_back(j);
+    if (sel.empty())
+      continue;
+    const int q = (int)sel.size();
+    for (int a = 0; a < q; ++a) {
+      dcopy(Vs + (int64_t)a * n_, basis_.col(j0 + sel[(std::size_t)a]), n_);
+      dcopy(Rs + (int64_t)a * m_, AV + (int64_t)sel[(std::size_t)a] * m_, m_);
+    }
+    std::vector<CgState> st;
+    cg_solve(q, Rs, Wm64, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
+    block_product(c_, A_, Prec::F64, q, Wm64, m_, Tn, n_, true, nullptr, rin_, rout_);
+    S_.spmtv += q;
+    sub_cols(c_, Prec::F64, n_, q, Vs, n_, Tn, n_, Vp, n_, nullptr);
+    col_norms(c_, Prec::F64, Vp, n_, q, n_, wsc_);
+    std::vector<double> np((std::size_t)q), sc((std::size_t)q, 0.0);
+    readback(c_, wsc_, q, np.data());
+    std::vector<int> ok((std::size_t)q, 0);
+    // a vector that loses half its length to the projection was not close
+    // to the null space: keep the locked one
+    for (int a = 0; a < q; ++a)
+      if (std::isfinite(np[(std::size_t)a]) && np[(std::size_t)a] >= 0.5) {
+        ok[(std::size_t)a] = 1;
+        sc[(std::size_t)a] = 1.0 / np[(std::size_t)a];
+      }
+    upload_doubles(c_, sc.data(), q, wsc_ + 1024);
+    scale_cols(c_, Prec::F64, n_, q, Vp, n_, wsc_ + 1024, true);
+    upload_mask(ok);
+    block_product(c_, A_, Prec::F64, q, Vp, n_, AVp, m_, false, mask_.as<int>(), rin_, rout_);
+    S_.spmv += q;
+    col_norms(c_, Prec::F64, AVp, m_, q, m_, wsc_ + 2048);
+    std::vector<double> rp((std::size_t)q);
+    readback(c_, wsc_ + 2048, q, rp.data());
+    for (int a = 0; a < q; ++a) {
+      const int j = sel[(std::size_t)a];
+      if (!ok[(std::size_t)a] || !(rp[(std::size_t)a] < res[(std::size_t)j]))
+        continue;
+      dcopy(basis_.col(j0 + j), Vp + (int64_t)a * n_, n_);
+      ++replaced;
+    }
+  }
+  S_.refined += replaced;
+  if (replaced == 0)
+    return;
+  // orthonormalise again: column j against columns 0..j-1, two passes
+  for (int j = 0; j < k; ++j) {
+    double *u = basis_.col(j);
+    if (j > 0)
+      for (int pass = 0; pass < 2; ++pass) {
+        gemv_t(c_, Prec::F64, basis_.col(0), n_, j, Prec::F64, u, n_, h_);
+        gemv_n_sub(c_, Prec::F64, basis_.col(0), n_, j, h_, Prec::F64, u, n_);
+      }
+    const double nu = norm_dev(Prec::F64, u, n_, wsc_);
+    if (std::isfinite(nu) && nu > 0.0)
+      axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nu, u);
+  }
 }
 
 }  // namespace

@@ -163,6 +163,7 @@ struct CyclicStats {
   int64_t spmv = 0, spmtv = 0;    // operator applications (column count)
   int64_t cg_iters = 0;           // CG iterations over all columns
   int64_t screened = 0;           // candidates rejected by the FP32 prescreen (mixed)
+  int64_t refined = 0;            // locked vectors replaced by the final FP64 pass (mixed)
   int64_t graph_captures = 0, graph_replays = 0;  // Golub-Kahan cycles as HIP graphs
   PhaseTimes t;
 };
