@@ -287,6 +287,20 @@ int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s,
  * use) or as four separate sweeps (fused == 0); the results are identical. */
 int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w,
                    int64_t len, int fused, double *h_out, double *norm_out);
+/* kh_cgs2_dev without allocating. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same kernels as
+ * kh_cgs2_dev (bitwise the same w, h and norm), with their reduction partials
+ * and second-pass coefficients in a static 32 MiB device array per GPU
+ * instead of allocated buffers; h_out (s doubles) and norm_out (1 double)
+ * are DEVICE pointers, and the call only enqueues work on the context's
+ * stream (wait with kh_context_sync). Limits: 0 <= s <= 1024 and the call's
+ * partials within the array (len x s about 4 M doubles; larger calls fail
+ * with KH_EINVAL before any work); kernel tracing must be off. All calls on
+ * one GPU share its array: calls that overlap on one GPU (other contexts,
+ * other threads) race and must be serialised by the caller. */
+int kh_cgs2_dev_noalloc(kh_context *ctx, int precision, const void *V, int s, void *w,
+                        int64_t len, double *h_out, double *norm_out);
 /* G = Z^T [Z AZ]: the S and G Gram blocks of LOBPCGSolver.cpp:67-80 on
  * FP32 MFMA (precision 1) or FP64 (precision 0); Z, AZ len x k column-major,
  * S and G k x k FP64 out (host). */

@@ -139,6 +139,7 @@ SIGNATURES = {
     "kh_gemm_nn_dev": (I32, [P, I32, P, I32, P, I32, P, I64, I32]),
     "kh_dot_dev": (I32, [P, I32, P, P, I64, C.POINTER(D)]),
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
+    "kh_cgs2_dev_noalloc": (I32, [P, I32, P, I32, P, I64, P, P]),
     "kh_cgs2_dev_ex": (I32, [P, I32, P, I32, P, I64, I32, P, P]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
     "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),

@@ -119,6 +119,7 @@ int kh_context_create(int device, kh_context **out) {
     need(out, "null out");
     auto c = std::make_unique<kh_context_s>();
     c->ctx = std::make_unique<kh::Context>(device);
+    kh::noalloc_prepare(device);  // the scratch of kh_cgs2_dev_noalloc on this device
     *out = c.release();
   });
 }
@@ -711,6 +712,50 @@ int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w
 int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
                 double *h_out, double *norm_out) {
   return kh_cgs2_dev_ex(ctx, precision, V, s, w, len, s <= 1024 ? 1 : 0, h_out, norm_out);
+}
+
+// kh_cgs2_dev's kernels (fused form) with their reduction partials and the
+// second-pass coefficients in the per-device static scratch of noalloc.hip
+// instead of the context's buffers, h and ||w|| written to device memory and
+// no wait: no allocation, no host copy, no synchronisation.
+int kh_cgs2_dev_noalloc(kh_context *ctx, int precision, const void *V, int s, void *w,
+                        int64_t len, double *h_out, double *norm_out) {
+  return guard([&] {
+    need(ctx && V && w && h_out && norm_out, "null argument");
+    need(s >= 0 && s <= 1024 && len >= 0, "bad argument (0 <= s <= 1024, len >= 0)");
+    auto &c = *ctx->ctx;
+    need(c.trace_mask == 0, "kernel tracing is on for this context (it records events)");
+    need(c.part_ext == nullptr, "a no-alloc call is already running on this context");
+    const kh::Prec p = prec_of(precision);
+    std::size_t cap = 0;
+    double *scr = kh::noalloc_scratch(c.device, &cap);
+    // layout: h2 (s doubles, padded to 256 bytes) | reduction partials
+    const std::size_t hpad = ((std::size_t)s + 31) / 32 * 32;
+    const std::size_t part = kh::cgs2_partials_bound(len, s);
+    if (hpad + part > cap)
+      throw std::invalid_argument(
+          "kh_cgs2_dev_noalloc: len x s too large for the 32 MiB device scratch (" +
+          std::to_string(hpad + part) + " doubles needed, " + std::to_string(cap) +
+          " available)");
+    struct Redirect {  // partials go to the scratch for this call only
+      kh::Context &c;
+      Redirect(kh::Context &cc, double *pp, std::size_t n) : c(cc) {
+        c.part_ext = pp;
+        c.part_ext_cap = n;
+      }
+      ~Redirect() {
+        c.part_ext = nullptr;
+        c.part_ext_cap = 0;
+      }
+    } redirect(c, scr + hpad, cap - hpad);
+    double *h2 = scr;
+    kh::gemv_t(c, p, V, len, s, p, w, len, h_out);  // h1
+    if (s > 0)
+      kh::gemv_n_sub_t(c, p, V, len, s, h_out, p, w, len, h2);
+    kh::gemv_n_sub(c, p, V, len, s, h2, p, w, len, norm_out);
+    if (s > 0)
+      kh::axpby(c, kh::Prec::F64, s, 1.0, h2, 1.0, h_out);  // h1 + h2
+  });
 }
 
 int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, int k,

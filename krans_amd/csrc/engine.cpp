@@ -136,9 +136,9 @@ void Context::fetch_scalars(int i0, int n) {
 }
 
 double *Context::partials(std::size_t need) {
-  if (need > part_cap)
+  if (need > part_capacity())
     throw std::length_error("reduction partial buffer too small for this batch");
-  return dpart;
+  return part_ext ? part_ext : dpart;
 }
 
 DBuf::DBuf(std::size_t b) : bytes(b) {

@@ -69,6 +69,11 @@ struct Context {
   static constexpr std::size_t kStage = std::size_t(4) << 20;
   double *dpart = nullptr;     // reduction partials
   std::size_t part_cap = 0;    // doubles
+  // A caller-supplied partial buffer that replaces dpart for the duration of
+  // one call (set and cleared by that call; nullptr otherwise).
+  double *part_ext = nullptr;
+  std::size_t part_ext_cap = 0;  // doubles
+  std::size_t part_capacity() const { return part_ext ? part_ext_cap : part_cap; }
   int cu_count = 0;
   static constexpr int kScalars = 1 << 16;
   explicit Context(int dev);
@@ -198,6 +203,10 @@ void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
 double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps);
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
             const void *w, int64_t len, double *out);
+// Reduction partials (doubles) that a CGS2 of w (length len) against s
+// columns can take in the launches of gemv_t, gemv_n_sub_t and gemv_n_sub:
+// an upper bound over every kernel those launchers may choose.
+std::size_t cgs2_partials_bound(int64_t len, int s);
 // w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
 // w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
 // R(:, j) = AX(:, j) - theta_j X(:, j), j < q (neg_theta: q device doubles,
@@ -304,6 +313,12 @@ void cg_init(Context &ctx, CgBatch &B, const void *rhs, void *w,
 // updates, rho, beta and the new direction. No-ops for stopped columns.
 void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
                        int64_t maxit, double breakdown_tol);
+
+// The device scratch of kh_cgs2_dev_noalloc (noalloc.hip): noalloc_prepare
+// looks up its address on `device` (once per device, that device current);
+// noalloc_scratch returns it and its capacity in doubles.
+void noalloc_prepare(int device);
+double *noalloc_scratch(int device, std::size_t *cap_doubles);
 
 // Partial-buffer size needed for the reductions of `len` elements.
 int reduce_blocks(int64_t len);

@@ -2386,6 +2386,16 @@ double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps) {
                         : time_vtw_t<float>(ctx, len, s, reps);
 }
 
+std::size_t cgs2_partials_bound(int64_t len, int s) {
+  const std::size_t nb = (std::size_t)vtw_blocks(len), sc = (std::size_t)std::max(s, 0);
+  std::size_t need = nb * std::min<std::size_t>(sc, 512);  // V^T w (launch_gemv_t)
+  need = std::max(need, nb * sc);                           // two-sweep middle (k_vh_sub_vtw)
+  need = std::max(need, nb);                                // norm of the last update
+  const std::size_t ntl = (std::size_t)((std::max<int64_t>(len, 0) + kLdsRows - 1) / kLdsRows);
+  need = std::max(need, std::min<std::size_t>(ntl, 16384) * (sc + 1));  // LDS-tiled middle
+  return need;
+}
+
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
             const void *w, int64_t len, double *out) {
   if (s <= 0)
@@ -2472,7 +2482,7 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
   // dword-load pipelined form needs many more registers). Both bound the
   // partials per column that k_finish sums.
   const int64_t nbl = std::min<int64_t>(ntl, vec ? 4096 : 16384);
-  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * (s + 1) <= ctx.part_cap) {
+  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * (s + 1) <= ctx.part_capacity()) {
     double *part = ctx.partials((std::size_t)nbl * (s + 1));
     double *npart = nrm1 ? part + (std::size_t)nbl * s : nullptr;
     {
@@ -2795,7 +2805,7 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
   // about 512 blocks, at most about 4096 rows (FP32 accumulation length)
   // each, unless the partials would not fit the context's reduction buffer
   const int64_t maxblk = std::max<int64_t>(
-      1, (int64_t)(ctx.part_cap * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
+      1, (int64_t)(ctx.part_capacity() * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
   int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);
   rpb = std::min<int64_t>(rpb, 4096);
   rpb = std::max<int64_t>(rpb, (len + maxblk - 1) / maxblk);
