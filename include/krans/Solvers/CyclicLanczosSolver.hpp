@@ -1,9 +1,12 @@
 // Cyclic Lanczos (Golub-Kahan) null-space solver: the options and result
 // structs and entry points of the reference, with the same names, fields and
 // defaults (reference: include/krans/Solvers/CyclicLanczosSolver.hpp:73-178).
-// On HIP operands (krans::hip::HipCsrMatrixAdapter + HipVectorFactory) the
-// call runs the device-resident driver of libkrans_hip.so; other backends
-// raise std::invalid_argument ("backend mismatch").
+// These krans::solvers entry points are defined by libkrans_solvers_hip.so,
+// which forwards every call to krans::hip::find_nullspace_cyclic_lanczos[_ex]
+// (libkrans_hip.so, krans/adapters/hip/HipSolvers.hpp): a HipVectorFactory
+// with any MatrixInterface runs the device-resident driver; other factories
+// raise std::invalid_argument ("backend mismatch"). Beside the reference's
+// own library the forwarding is instead a dispatch in its entry points.
 #pragma once
 #include <cstddef>
 #include <functional>
@@ -33,7 +36,9 @@ struct CyclicLanczosOptions {
   int nullity_known = -1;
   int rank_known = -1;
   std::vector<std::unique_ptr<VectorInterface>> initial_right_space;
-  /** nullptr, or a krans::hip::JacobiAAtPreconditioner on the HIP backend */
+  /** On the HIP backend the callback receives HipVector views of device
+   * columns; krans::hip::JacobiAAtPreconditioner on the solve's own matrix
+   * runs inside the CG kernels. */
   AAtPreconditioner precond_AAt = nullptr;
   std::size_t cg_maxit = 80;
   double cg_rel_tol = 1e-10;

@@ -1,7 +1,9 @@
 // LOBPCG null-space solver on A^T A: options, result and entry points of the
 // reference with the same names, fields and defaults (reference:
-// include/krans/Solvers/LOBPCGSolver.hpp:68-149). HIP operands run the
-// device-resident driver; M_inv must be nullptr on the HIP backend.
+// include/krans/Solvers/LOBPCGSolver.hpp:68-149). Defined by
+// libkrans_solvers_hip.so as forwards to krans::hip::find_nullspace_lobpcg[_ex]
+// (HIP operands run the device-resident driver; M_inv receives HipVector
+// views of device columns).
 #pragma once
 #include <cstddef>
 #include <functional>

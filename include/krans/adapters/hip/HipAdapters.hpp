@@ -66,8 +66,14 @@ public:
   /** Adopt a device allocation made with kh_dev_alloc. */
   struct Adopt {};
   HipVector(kh_context *ctx, std::size_t n, double *dev, Adopt) : ctx_(ctx), n_(n), d_(dev) {}
+  /** A non-owning view of n device doubles (valid while the memory is):
+   * how the solvers hand their device columns to user operators and
+   * preconditioners. */
+  struct View {};
+  HipVector(kh_context *ctx, std::size_t n, double *dev, View)
+      : ctx_(ctx), n_(n), d_(dev), own_(false) {}
   ~HipVector() override {
-    if (d_)
+    if (d_ && own_)
       kh_dev_free(ctx_, d_);
   }
   HipVector(const HipVector &) = delete;
@@ -132,6 +138,7 @@ private:
   kh_context *ctx_;
   std::size_t n_;
   double *d_ = nullptr;
+  bool own_ = true;
 };
 
 /** VectorFactory with the reference's seeded N(0,1) stream; `precision`
@@ -219,7 +226,17 @@ private:
     const std::size_t nin = t ? m_ : n_, nout = t ? n_ : m_;
     if (hx->size() != nin || hy->size() != nout)
       throw std::invalid_argument(std::string("HipMatrix::") + what + " size mismatch");
+    // vectors of another context live on another stream: order the product
+    // after their pending work and finish it before returning
+    const bool other = hx->context() != ctx_ || hy->context() != ctx_;
+    if (other) {
+      check(kh_context_sync(hx->context()));
+      if (hy->context() != hx->context())
+        check(kh_context_sync(hy->context()));
+    }
     check(kh_spmm_dev(A_, t, KH_PREC_FP64, 1, hx->device_ptr(), hy->device_ptr()));
+    if (other)
+      check(kh_context_sync(ctx_));
   }
   kh_context *ctx_;
   kh_matrix *A_ = nullptr;
@@ -239,6 +256,34 @@ public:
     adopt(A);
   }
 };
+
+/** A matrix file loaded into HBM: the IJA triplet format of the reference's
+ * load_ija_triplet_matrix (include/krans/IO/MatrixLoader.hpp; rows/cols < 0
+ * take the sizes from the file) or Matrix Market (load_matrix_market), with
+ * repeated entries summed in file order. Throws std::runtime_error when the
+ * file cannot be read, as the reference's loaders do. */
+inline std::unique_ptr<HipCsrMatrixAdapter> load_matrix(kh_context *ctx, const std::string &path,
+                                                        long rows = -1, long cols = -1) {
+  kh_host_csr *h = nullptr;
+  const bool mtx = path.size() > 4 && path.compare(path.size() - 4, 4, ".mtx") == 0;
+  const int rc = mtx ? kh_load_matrix_market(path.c_str(), &h)
+                     : kh_load_ija(path.c_str(), rows, cols, &h);
+  if (rc != KH_OK)
+    throw std::runtime_error(kh_last_error());
+  int64_t m = 0, n = 0, nnz = 0;
+  const int32_t *rp = nullptr, *ci = nullptr;
+  const double *va = nullptr;
+  kh_host_csr_info(h, &m, &n, &nnz, nullptr, nullptr);
+  kh_host_csr_data(h, &rp, &ci, &va);
+  try {
+    auto A = std::make_unique<HipCsrMatrixAdapter>(ctx, (std::size_t)m, (std::size_t)n, rp, ci, va);
+    kh_host_csr_free(h);
+    return A;
+  } catch (...) {
+    kh_host_csr_free(h);
+    throw;
+  }
+}
 
 /** The quick start's matrix-free operator (examples/quickstart.cpp:25-37,
  * a StdMatrixAdapter of two lambdas): (A x)_i = x_{i+N} - c x_i, A is

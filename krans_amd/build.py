@@ -47,14 +47,34 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_solvers_entry()
     build_examples()
     if verbose:
         print("built", LIB)
     return LIB
 
 
+ENTRY_SRC = os.path.join(SRC, "solvers_entry", "krans_solvers_hip.cpp")
+ENTRY_LIB = os.path.join(OUT, "libkrans_solvers_hip.so")
+
+
+def build_solvers_entry():
+    """libkrans_solvers_hip.so: the reference's krans::solvers entry points
+    forwarding to libkrans_hip.so (plain host C++, g++)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = ENTRY_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", inc, ENTRY_SRC, "-o",
+           tmp, "-L", OUT, "-lkrans_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"entry library build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ENTRY_LIB)
+
+
 EXAMPLES = {"quickstart_hip": os.path.join(os.path.dirname(HERE), "tests", "cxx",
                                            "quickstart_hip.cpp"),
+            "reference_scenarios_hip": os.path.join(os.path.dirname(HERE), "tests", "cxx",
+                                                    "reference_scenarios_hip.cpp"),
             "krans_nullspace_hip": os.path.join(HERE, "apps", "krans_nullspace_hip.cpp")}
 
 
@@ -67,8 +87,8 @@ def build_examples():
         if not os.path.exists(src):
             continue
         out = os.path.join(OUT, name)
-        cmd = ["g++", "-std=c++17", "-O2", "-I", inc, src, "-o", out, "-L", OUT, "-lkrans_hip",
-               f"-Wl,-rpath,$ORIGIN"]
+        cmd = ["g++", "-std=c++17", "-O2", "-I", inc, src, "-o", out, "-L", OUT,
+               "-lkrans_solvers_hip", "-lkrans_hip", "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

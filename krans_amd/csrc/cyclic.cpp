@@ -150,6 +150,10 @@ private:
   DeviceBasis finish();
   void cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
                 int64_t maxit, double rel_tol, std::vector<CgState> &st);
+  CgState cg_callback(const double *rhs, double *w, int64_t maxit, double rel_tol);
+  // the projector CGs of the polish passes on q columns (rhs, w: m x q)
+  void polish_cg(int q, const double *rhs, double *w, Prec pc, const double *diag,
+                 std::vector<CgState> &st);
   void upload_mask(const std::vector<int> &mask);
   DBuf mask_;
   DBuf rin_, rout_;  // row-major copies of block_product
@@ -323,6 +327,114 @@ void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const doub
   }
 }
 
+// cg_normal_eq_prec (CyclicLanczosSolver.cpp:48-113) with the caller's
+// preconditioner P_.precond (or none), for one column in FP64: the same
+// recurrence, stop tests and returned (iterations, residual) as the batched
+// kernels, with the scalars read back every iteration and the callback
+// given the device residual r and the device vector z = M^{-1} r.
+CgState CyclicDriver::cg_callback(const double *rhs, double *w, int64_t maxit,
+                                  double rel_tol) {
+  cg_.ensure(1, m_, n_, Prec::F64);
+  double *r = cg_.r.as<double>(), *z = cg_.z.as<double>(), *p = cg_.pv.as<double>();
+  double *Ap = cg_.Ap.as<double>(), *tn = cg_.tmp_n.as<double>();
+  double *slot = wsc_ + 3072;
+  auto dotp = [&](const double *a, const double *b) {
+    dot(c_, Prec::F64, a, Prec::F64, b, m_, slot);
+    double v = 0.0;
+    readback(c_, slot, 1, &v);
+    return v;
+  };
+  auto precondition = [&]() {
+    if (P_.precond) {
+      c_.sync();
+      P_.precond->apply(c_, r, z, m_);
+    } else {
+      copy(c_, Prec::F64, m_, r, z);
+    }
+  };
+  CgState S{};
+  S.done = 1;
+  axpby(c_, Prec::F64, m_, 0.0, nullptr, 0.0, w);  // w = 0
+  copy(c_, Prec::F64, m_, rhs, r);
+  const double rhsn = std::sqrt(std::max(0.0, dotp(r, r)));
+  S.rhsn = S.rnorm = rhsn;
+  if (!(rhsn > 0.0) || !std::isfinite(rhsn)) {  // zero or non-finite rhs: w = 0
+    S.reason = 1;
+    return S;
+  }
+  precondition();
+  copy(c_, Prec::F64, m_, z, p);
+  double rz = dotp(r, z);
+  if (!(rz > 0.0) || !std::isfinite(rz)) {  // preconditioner not SPD on r
+    S.reason = 1;
+    return S;
+  }
+  double rnorm = std::sqrt(rz);
+  const double tol = std::max(rel_tol * rnorm, 1e-30);
+  S.rnorm = rnorm;
+  S.tol = tol;
+  if (rnorm <= tol)
+    return S;
+  int64_t k = 0;
+  int reason = 4;  // stopped at maxit unless another test ends the loop
+  for (; k < maxit; ++k) {
+    A_.mtv(c_, Prec::F64, p, tn);
+    A_.mv(c_, Prec::F64, tn, Ap);
+    ++S_.spmv;
+    ++S_.spmtv;
+    const double pAp = dotp(p, Ap);
+    if (!std::isfinite(pAp) || std::fabs(pAp) <= P_.cg_breakdown_tol) {
+      reason = 2;
+      break;
+    }
+    const double alpha = rz / pAp;
+    axpby(c_, Prec::F64, m_, alpha, p, 1.0, w);
+    axpby(c_, Prec::F64, m_, -alpha, Ap, 1.0, r);
+    precondition();
+    const double rz_new = dotp(r, z);
+    if (!std::isfinite(rz_new)) {
+      reason = 3;
+      break;
+    }
+    rnorm = std::sqrt(std::max(0.0, rz_new));
+    if (rnorm <= tol) {
+      ++k;
+      reason = 0;
+      break;
+    }
+    const double beta = rz_new / std::max(1e-300, rz);
+    rz = rz_new;
+    axpby(c_, Prec::F64, m_, 1.0, z, beta, p);  // p = beta p + z
+  }
+  S.k = (int32_t)k;
+  S.rnorm = rnorm;
+  S.reason = reason;
+  S_.cg_iters += k;
+  return S;
+}
+
+void CyclicDriver::polish_cg(int q, const double *rhs, double *w, Prec pc, const double *diag,
+                             std::vector<CgState> &st) {
+  if (P_.precond) {  // the caller's preconditioner: FP64, column by column
+    st.resize((std::size_t)q);
+    for (int a = 0; a < q; ++a)
+      st[(std::size_t)a] = cg_callback(rhs + (int64_t)a * m_, w + (int64_t)a * m_,
+                                       P_.cg_maxit, P_.cg_rel_tol);
+    return;
+  }
+  if (pc == Prec::F64) {
+    cg_solve(q, rhs, w, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
+    return;
+  }
+  // FP32 CG on the rounded FP64 residual; the FP64 residual of the updated
+  // v decides whether the pass is kept
+  float *W32 = Wm_.as<float>();
+  float *R32 = W32 + (int64_t)wcap_ * m_;
+  convert(c_, Prec::F64, rhs, Prec::F32, R32, (int64_t)q * m_);
+  cg_solve(q, R32, W32, Prec::F32, diag, P_.cg_maxit, std::max(P_.cg_rel_tol, 1e-6), st);
+  convert(c_, Prec::F32, W32, Prec::F64, w, (int64_t)q * m_);
+}
+
 // Batched projector polish of the window columns `cols` (indices into the
 // window block Wv/AV): per column the pass loop of
 // CyclicLanczosSolver.cpp:509-542. Slot i holds the current v of cols[i] in
@@ -370,18 +482,7 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
         dcopy(Rw + (int64_t)a * m_, AVc + (int64_t)i * m_, m_);
       }
     }
-    if (P_.polish == Prec::F64) {
-      cg_solve(qa, Rw, Wm64, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
-    } else {
-      // FP32 CG on the rounded FP64 residual; the FP64 residual of the
-      // updated v below decides whether the pass is kept.
-      float *W32 = Wm_.as<float>();
-      float *R32 = W32 + (int64_t)wcap_ * m_;
-      convert(c_, Prec::F64, Rw, Prec::F32, R32, (int64_t)qa * m_);
-      cg_solve(qa, R32, W32, Prec::F32, diag, P_.cg_maxit, std::max(P_.cg_rel_tol, 1e-6),
-               st);
-      convert(c_, Prec::F32, W32, Prec::F64, Wm64, (int64_t)qa * m_);
-    }
+    polish_cg(qa, Rw, Wm64, P_.polish, diag, st);
     for (int a = 0; a < qa; ++a) {
       ColOut &o = out[(std::size_t)cols[(std::size_t)act[(std::size_t)a]]];
       ++o.passes;
@@ -1027,7 +1128,7 @@ void CyclicDriver::refine_basis() {
       dcopy(Rs + (int64_t)a * m_, AV + (int64_t)sel[(std::size_t)a] * m_, m_);
     }
     std::vector<CgState> st;
-    cg_solve(q, Rs, Wm64, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
+    polish_cg(q, Rs, Wm64, Prec::F64, diag, st);
     block_product(c_, A_, Prec::F64, q, Wm64, m_, Tn, n_, true, nullptr, rin_, rout_);
     S_.spmtv += q;
     sub_cols(c_, Prec::F64, n_, q, Vs, n_, Tn, n_, Vp, n_, nullptr);

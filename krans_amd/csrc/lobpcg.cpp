@@ -182,6 +182,15 @@ private:
   void combine(const Blk &Z, int z0, int q, const dense::Mat &Y, int row0, int p,
                Blk &out, int o0);
   void to64(const void *x, Prec p, double *d) { convert(c_, p, x, Prec::F64, d, n_); }
+  // x <- M^{-1} x for an FP64 device vector of length n (P_.M_inv)
+  void precondition64(double *x) {
+    if (minv_.bytes < (std::size_t)n_ * 8)
+      minv_ = DBuf((std::size_t)n_ * 8);
+    c_.sync();
+    P_.M_inv->apply(c_, x, minv_.as<double>(), n_);
+    copy(c_, Prec::F64, n_, minv_.p, x);
+  }
+  DBuf minv_;
   double estimate_fro();
   bool orthonormalize_vs(Blk &B, int k, int nd, Blk &scratch);
   void block_rr_polish(double target, int max_sweeps);
@@ -537,7 +546,12 @@ bool LobpcgDriver::two_vec_sharpen(int idx, double target, int max_iters) {
       return true;
     A_.mtv(c_, Prec::F64, Av, t);
     ++S_.spmtv;
-    copy(c_, Prec::F64, n_, t, d);
+    if (P_.M_inv) {  // d = M^{-1} t (:232-237)
+      c_.sync();
+      P_.M_inv->apply(c_, t, d, n_);
+    } else {
+      copy(c_, Prec::F64, n_, t, d);
+    }
     proj(d);
     const double nd = nrm(Prec::F64, d, n_);
     if (nd <= 1e-14)
@@ -692,6 +706,8 @@ void LobpcgDriver::block_rr_polish(double target, int max_sweeps) {
       A_.mtv(c_, Prec::F64, r, g);
       ++S_.spmv;
       ++S_.spmtv;
+      if (P_.M_inv)  // g = M^{-1} N v_j (:327-331)
+        precondition64(g);
       // project vs V (one pass, :334)
       gemv_t(c_, Prec::F64, Vb.col(0), n_, k, Prec::F64, g, n_, sc_ + 16);
       gemv_n_sub(c_, Prec::F64, Vb.col(0), n_, k, sc_ + 16, Prec::F64, g, n_, sc_);
@@ -1105,7 +1121,18 @@ DeviceBasis LobpcgDriver::run() {
     }
 
     const int na = B - ini;
-    // W = R on the active slice (no M_inv), orthonormalised in place (:653-680):
+    // W = M^{-1} R on the active slice (:654-659) when the caller gives M_inv:
+    // its FP64 device views, through FP64 copies for FP32 blocks
+    if (P_.M_inv) {
+      double *r64 = v64_.as<double>(), *z64 = v64b_.as<double>();
+      for (int j = ini; j < B; ++j) {
+        to64(W.col(j), bp_, r64);
+        c_.sync();
+        P_.M_inv->apply(c_, r64, z64, n_);
+        convert(c_, Prec::F64, z64, bp_, W.col(j), n_);
+      }
+    }
+    // W = R (or M^{-1} R) on the active slice, orthonormalised in place (:653-680):
     // on a failure W is replaced by random columns anyway, so the partially
     // processed state orthonormalize leaves does not matter
     if (!orthonormalize(W, ini, na, P_.gs_tol, P_.gs_reorth, scratch)) {

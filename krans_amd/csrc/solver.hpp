@@ -124,6 +124,16 @@ private:
   std::thread th_;
 };
 
+// A caller's preconditioner on FP64 device vectors of length len: z = M^{-1} r
+// (the reference's AAtPreconditioner / NormalPreconditioner std::functions,
+// CyclicLanczosSolver.hpp:77-78, LOBPCGSolver.hpp:68-69). Called with all
+// earlier work on the context's stream complete; its work is complete, or
+// enqueued on that stream, when it returns; it must not write r.
+struct DevicePreconditioner {
+  virtual ~DevicePreconditioner() = default;
+  virtual void apply(Context &c, const double *r, double *z, int64_t len) const = 0;
+};
+
 // ---------------------------------------------------------------------------
 // Options: field for field CyclicLanczosOptions (CyclicLanczosSolver.hpp:84-146)
 // minus the two non-POD members, which are passed separately.
@@ -142,6 +152,9 @@ struct CyclicParams {
   bool return_left_space = false, verbose = false;
   // engine choices (not in the reference struct)
   bool jacobi_AAt = false;  // M^{-1} = diag(A A^T)^{-1} in the polish CG
+  // precond_AAt as a callback (CyclicLanczosSolver.hpp:115): the polish CGs
+  // then run in FP64, one column at a time, calling it once per iteration
+  const DevicePreconditioner *precond = nullptr;
   Prec krylov = Prec::F64;  // Golub-Kahan vectors and products
   Prec polish = Prec::F64;  // CG of the projector polish (F32 = refinement)
   const double *y0_host = nullptr;  // initial_right_space[0], host, length n
@@ -214,6 +227,7 @@ struct LobpcgParams {
   bool verbose = false;
   int64_t progress_every = 100;
   Prec block = Prec::F64;  // X/W/P blocks and their products
+  const DevicePreconditioner *M_inv = nullptr;  // W = M^{-1} R (LOBPCGSolver.cpp:654-659)
   const double *initial_block_host = nullptr;  // n x n_initial, column-major
   int64_t n_initial = 0;
 };

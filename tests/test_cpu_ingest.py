@@ -39,8 +39,9 @@ def canonical(A):
         assert np.all(np.diff(c) > 0)
 
 
-@pytest.mark.parametrize("name,shape,nnz", [("mat1", (140, 105), 380), ("mat3", (200, 200), None),
-                                            ("mat6", (1000, 500), None)])
+@pytest.mark.parametrize("name,shape,nnz", [("mat1", (140, 105), 380), ("mat2", (9520, 5440), 27440),
+                                            ("mat3", (200, 200), None), ("mat4", (300, 300), None),
+                                            ("mat5", (500, 500), None), ("mat6", (1000, 500), None)])
 def test_reference_data_fixtures(name, shape, nnz):
     path = os.path.join(GOLDEN, name + ".ija")
     A = K.load_ija(path)

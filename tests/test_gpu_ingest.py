@@ -49,7 +49,7 @@ def test_mat1_cyclic_reference_options(precision):
     assert oracle.subspace_diff(U, Uo, 1e-7) <= 1e-5  # the reference test's bound
 
 
-@pytest.mark.parametrize("name,k", [("mat3", 10), ("mat6", 25)])
+@pytest.mark.parametrize("name,k", [("mat3", 10), ("mat4", 15), ("mat5", 20), ("mat6", 25)])
 @pytest.mark.parametrize("solver", ["cyclic", "lobpcg"])
 def test_all_solvers_matrices(name, k, solver):
     A = K.load_ija(os.path.join(GOLDEN, name + ".ija"))
