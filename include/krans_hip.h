@@ -309,7 +309,10 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ,
 /* Kernel timing with HIP events on the context's stream: between begin and
  * end every launch of the class (1 = reorthogonalisation V^T w products,
  * 2 = w -= V h updates, 3 = sparse products, 4 = FP32-MFMA Gram tiles) is
- * bracketed by an event pair; end returns the launch count, their summed
+ * bracketed by an event pair (class 1, the FP32 V^T w products: the start
+ * and end its dispatch stamps, i.e. the kernel's own duration; the other
+ * classes: events recorded on the stream around the launch, i.e. including
+ * its dispatch); end returns the launch count, their summed
  * algorithmic bytes (each array touched once; for class 4 the summed flops,
  * 2 len s q per product) and summed device time in ms. */
 int kh_trace_begin(kh_context *ctx, int kernel_class);
@@ -333,7 +336,8 @@ int kh_trace_shapes(kh_context *ctx, int kernel_class, int max, int64_t *len, in
 /* Average device time (ms) of one V^T w partial-product launch (the kernel of
  * class 1 without its finishing sum) on len x s zero blocks of precision
  * 0 (FP64) or 1 (FP32), over `reps` back-to-back launches between two HIP
- * events on the context's stream; 0 < s <= 512. Observability of this build;
+ * events on the context's stream, rotating over copies of the blocks that
+ * total at least 1 GiB (none is served from the Infinity Cache); 0 < s <= 512. Observability of this build;
  * the reference has no counterpart. */
 int kh_time_vtw(kh_context *ctx, int precision, int64_t len, int s, int reps, double *avg_ms);
 

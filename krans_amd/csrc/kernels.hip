@@ -13,6 +13,7 @@
 //    (EigenSparseMatrixAdapter.hpp:34-57).
 //  * Scalars (alpha, beta, norms, CG state) stay on the device; kernels read
 //    them through pointers so the host can queue work without syncing.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2128,27 +2129,37 @@ __global__ void k_act_normalise(int s, int *__restrict__ act) {
 
 // =============================================================== launchers ==
 // Records an event pair around the launches of a traced class.
+// Per-launch timing of a traced class. By default an event is recorded on
+// the stream before and after the launch (the pair then also spans the
+// dispatch of the launch); with `stamped` the scope only provides the events
+// and the launch passes them to hipExtLaunchKernelGGL, which stamps the
+// kernel's own start and end (the duration a kernel-trace profiler reports).
 struct TraceScope {
   Context &c;
-  bool on;
-  hipEvent_t a = nullptr;
+  bool on, stamped;
+  hipEvent_t a = nullptr, b = nullptr;
   double bytes;
   int cls;
   int64_t len = 0;
   int s = 0;
-  TraceScope(Context &ctx, int cls_, double b, int64_t len_ = 0, int s_ = 0)
-      : c(ctx), on(cls_ > 0 && ((ctx.trace_mask >> cls_) & 1u)), bytes(b), cls(cls_), len(len_),
-        s(s_) {
+  TraceScope(Context &ctx, int cls_, double by, int64_t len_ = 0, int s_ = 0,
+             bool stamped_ = false)
+      : c(ctx), on(cls_ > 0 && ((ctx.trace_mask >> cls_) & 1u)), stamped(stamped_), bytes(by),
+        cls(cls_), len(len_), s(s_) {
     if (on) {
       a = c.take_event();
-      KH_CHECK(hipEventRecord(a, c.stream));
+      b = c.take_event();
+      if (!stamped)
+        KH_CHECK(hipEventRecord(a, c.stream));
     }
   }
+  hipEvent_t start() const { return on ? a : nullptr; }
+  hipEvent_t stop() const { return on ? b : nullptr; }
   ~TraceScope() {
     if (on) {
-      hipEvent_t b2 = c.take_event();
-      (void)hipEventRecord(b2, c.stream);
-      c.trace.push_back({bytes, a, b2, cls, len, s});
+      if (!stamped)
+        (void)hipEventRecord(b, c.stream);
+      c.trace.push_back({bytes, a, b, cls, len, s});
     }
   }
 };
@@ -2318,20 +2329,22 @@ inline int vtw_blocks(int64_t len) {
 }
 
 // The partial kernel of V^T w: 16-byte loads for aligned FP32 blocks.
+// With events ea / eb (tracing) the launch stamps the kernel's start and end.
 template <class TV, class TW>
 static void launch_vtw_partial(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
-                               int64_t len, double *part, int nb) {
+                               int64_t len, double *part, int nb, hipEvent_t ea = nullptr,
+                               hipEvent_t eb = nullptr) {
   const std::size_t lds = (std::size_t)s * kWaves * sizeof(double);
   if constexpr (std::is_same<TV, float>::value && std::is_same<TW, float>::value) {
     if (len % 4 == 0 && ld % 4 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
         reinterpret_cast<uintptr_t>(w) % 16 == 0) {
-      hipLaunchKernelGGL(k_vtw4_partial, dim3(nb), dim3(kBlock), lds, ctx.stream, len, V, ld, s,
-                         w, part, nb);
+      hipExtLaunchKernelGGL(k_vtw4_partial, dim3(nb), dim3(kBlock), lds, ctx.stream, ea, eb, 0,
+                            len, V, ld, s, w, part, nb);
       return;
     }
   }
-  hipLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock), lds, ctx.stream, len, V, ld,
-                     s, w, part, nb);
+  hipExtLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock), lds, ctx.stream, ea, eb,
+                        0, len, V, ld, s, w, part, nb);
 }
 
 template <class TV, class TW>
@@ -2342,10 +2355,14 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
     const int sc = std::min(512, s - c0);
     double *part = ctx.partials((std::size_t)nb * sc);
     {
-      TraceScope tr(ctx, kTraceVtW,
+      // class 1 is the FP32 reorthogonalisation product (k_vtw4_partial for
+      // aligned blocks), the kernel the roofline reports
+      constexpr bool kF32 = std::is_same<TV, float>::value && std::is_same<TW, float>::value;
+      TraceScope tr(ctx, kF32 ? kTraceVtW : kTraceNone,
                     (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc,
-                    len, sc);
-      launch_vtw_partial(ctx, V + (int64_t)c0 * ld, ld, sc, w, len, part, nb);
+                    len, sc, /*stamped=*/true);
+      launch_vtw_partial(ctx, V + (int64_t)c0 * ld, ld, sc, w, len, part, nb, tr.start(),
+                         tr.stop());
     }
     hipLaunchKernelGGL(k_finish, dim3(sc), dim3(kBlock), 0, ctx.stream, part, nb, out + c0, 0);
   }
@@ -2358,13 +2375,27 @@ static double time_vtw_t(Context &ctx, int64_t len, int s, int reps) {
     throw std::invalid_argument("time_vtw: 0 < s <= 512, len > 0, reps >= 1");
   const int nb = vtw_blocks(len);
   double *part = ctx.partials((std::size_t)nb * s);
-  DBuf V((std::size_t)len * s * sizeof(T)), w((std::size_t)len * sizeof(T));
-  KH_CHECK(hipMemsetAsync(V.p, 0, V.bytes, ctx.stream));
-  KH_CHECK(hipMemsetAsync(w.p, 0, w.bytes, ctx.stream));
+  // rotate over copies of V and w totalling at least 1 GiB, so that no
+  // launch finds its block in the 256 MB Infinity Cache (as in a solve,
+  // where the block is read once per pass between other sweeps)
+  const std::size_t vb = (std::size_t)len * s * sizeof(T), wb = (std::size_t)len * sizeof(T);
+  const int copies = (int)std::min<std::size_t>(16, std::max<std::size_t>(
+                                                        2, ((std::size_t)1 << 30) / (vb + wb) + 1));
+  std::vector<DBuf> V, w;
+  for (int i = 0; i < copies; ++i) {
+    V.emplace_back(vb);
+    w.emplace_back(wb);
+    KH_CHECK(hipMemsetAsync(V.back().p, 0, vb, ctx.stream));
+    KH_CHECK(hipMemsetAsync(w.back().p, 0, wb, ctx.stream));
+  }
+  int turn = 0;
   auto launch = [&] {
-    launch_vtw_partial(ctx, (const T *)V.p, len, s, (const T *)w.p, len, part, nb);
+    const int i = turn++ % copies;
+    launch_vtw_partial(ctx, (const T *)V[(std::size_t)i].p, len, s, (const T *)w[(std::size_t)i].p,
+                       len, part, nb);
   };
-  launch();
+  for (int i = 0; i < copies; ++i)
+    launch();
   hipEvent_t e0, e1;
   KH_CHECK(hipEventCreate(&e0));
   KH_CHECK(hipEventCreate(&e1));
