@@ -76,6 +76,9 @@ typedef struct kh_cyclic_result {
   /* mixed precision: locked vectors replaced by the final batched FP64
    * projector pass (those above max(1e-14, min(1e-10, 0.01 thr))) */
   int64_t refined;
+  /* serving only: the number of requests in the pack whose shared launches
+   * ran this request's Golub-Kahan cycles (0: solved on its own) */
+  int64_t packed;
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and
@@ -240,6 +243,13 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
 int kh_server_stats(kh_server *s, int64_t *done, int64_t *vectors, double *p50, double *p99,
                     double *wall_s);
 int kh_server_reset_clock(kh_server *s);
+/* Batched launches for small requests: a request with m + n <= pack_len
+ * (host CSR, reorth 2) is solved in a pack with up to pack_max - 1
+ * compatible queued requests (same precision and ncv), whose Golub-Kahan
+ * cycles run in shared segmented launches (defaults 16 and 2^19); alone it
+ * runs in a pack of one, so its result is bitwise the same alone and in any
+ * mix. pack_max = 1 solves every request on its own. */
+int kh_server_set_packing(kh_server *s, int pack_max, int64_t pack_len);
 
 /* --- device buffers and kernel entry points (tests, benchmarks) ---------- */
 int kh_dev_alloc(kh_context *ctx, size_t bytes, void **out);

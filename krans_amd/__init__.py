@@ -57,7 +57,7 @@ class _CyclicRes(C.Structure):
         ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
         ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
-        ("refined", C.c_int64),
+        ("refined", C.c_int64), ("packed", C.c_int64),
     ]
 
 
@@ -127,6 +127,7 @@ SIGNATURES = {
     "kh_server_stats": (I32, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(D), C.POINTER(D),
                               C.POINTER(D)]),
     "kh_server_reset_clock": (I32, [P]),
+    "kh_server_set_packing": (I32, [P, I32, I64]),
     "kh_dev_alloc": (I32, [P, SZ, C.POINTER(P)]),
     "kh_dev_free": (I32, [P, P]),
     "kh_h2d": (I32, [P, P, P, SZ]),
@@ -691,11 +692,18 @@ class NullspaceServer:
     admitted in arrival order when one of `slots` solver slots (one device
     stream each) is free and their HBM footprint fits the budget."""
 
-    def __init__(self, slots=4, hbm_budget_bytes=0.0, device=0):
+    def __init__(self, slots=4, hbm_budget_bytes=0.0, device=0, pack_max=16, pack_len=1 << 19):
         h = P()
         _check(lib().kh_server_create(device, slots, float(hbm_budget_bytes), C.byref(h)))
         self.h = h
         self._pending = {}
+        self.set_packing(pack_max, pack_len)
+
+    def set_packing(self, pack_max, pack_len=1 << 19):
+        """Small requests (m + n <= pack_len) run in packs of up to pack_max
+        whose Golub-Kahan cycles share segmented launches (kh_server_set_packing);
+        pack_max=1 solves every request on its own."""
+        _check(lib().kh_server_set_packing(self.h, int(pack_max), int(pack_len)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -518,12 +518,20 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->t_rng = r.stats.t.rng;
     res->screened = r.stats.screened;
     res->refined = r.stats.refined;
+    res->packed = r.packed;
     res->graph_captures = r.stats.graph_captures;
     res->graph_replays = r.stats.graph_replays;
     if (latency_s)
       *latency_s = r.latency;
     if (queued_s)
       *queued_s = r.queued;
+  });
+}
+
+int kh_server_set_packing(kh_server *s, int pack_max, int64_t pack_len) {
+  return guard([&] {
+    need(s, "null server");
+    s->srv->set_packing(pack_max, pack_len);
   });
 }
 

@@ -25,8 +25,8 @@
 #include <limits>
 #include <numeric>
 
+#include "cyclic_driver.hpp"
 #include "dense.hpp"
-#include "solver.hpp"
 
 namespace kh {
 namespace {
@@ -72,12 +72,6 @@ void upload_vec(Context &ctx, const double *h, int64_t n, double *d) {
   KH_CHECK(hipStreamSynchronize(ctx.stream));
 }
 
-struct Group {  // a block of vectors to orthogonalise against
-  const void *V;
-  int64_t ld;
-  int count;
-  Prec p;
-};
 
 // Automatic graph replay of the Golub-Kahan cycle up to this m + n. Zero:
 // on MI355X replay was slower than direct launches at every size measured
@@ -85,80 +79,8 @@ struct Group {  // a block of vectors to orthogonalise against
 // launch cost), so the automatic setting launches directly.
 constexpr int64_t kGraphAutoLen = 0;
 
-class CyclicDriver {
-public:
-  CyclicDriver(Context &ctx, const DeviceOp &op, RefStream &rng, const CyclicParams &P,
-               CyclicStats &S)
-      : c_(ctx), A_(op), rng_(rng), P_(P), S_(S), m_(op.rows()), n_(op.cols()) {}
 
-  DeviceBasis run();
-
-private:
-  Context &c_;
-  const DeviceOp &A_;
-  RefStream &rng_;
-  const CyclicParams &P_;
-  CyclicStats &S_;
-  int64_t m_, n_;
-  Prec kp_ = Prec::F64;  // Krylov precision
-
-  // device state
-  DeviceBasis basis_;   // FP64 locked basis (n x k_target)
-  // [Y reversed | basis] in Krylov precision: Golub-Kahan vector y_j sits in
-  // column ycap-1-j and locked vector b_i in column ycap+i, so the y-side
-  // reorthogonalisation set [basis, Y] is one contiguous block.
-  DBuf VB_;
-  DBuf X_, wx_, wy_;
-  DBuf y0_, best_v_, tmp_m_, tmp_n_;
-  DBuf small_;          // device scalars of this solve
-  double *bnorm_ = nullptr, *h_ = nullptr, *wsc_ = nullptr, *C_ = nullptr, *H_ = nullptr;
-  int64_t ycap_ = 0, xcap_ = 0;
-  int64_t ldy_ = 0, ldx_ = 0;  // padded leading dimensions of VB_ and X_
-  int64_t k_target_ = 0;
-  double accept_thr_ = 0, prescreen_thr_ = 0;
-  std::vector<double> hbuf_;
-  // window blocks
-  DBuf Wv_, AV_, Vc_, AVc_, Vs_, Rs_, Vp_, AVp_, Wm_, Tn_, Wm64_;
-  DBuf Wv32_, AV32_;      // FP32 screen of a window (mixed precision)
-  bool screen_ = false;   // screen candidate windows in FP32 first
-  int screen_miss_ = 0;   // consecutive windows the screen did not thin out
-  int wcap_ = 0;
-  CgBatch cg_;
-  CapturedWork gk_graph_;  // the queued Golub-Kahan cycle, keyed by the basis size
-  bool graphs_ = false;
-  int64_t gk_nyc_ = 0, gk_nxc_ = 0, gk_nmv_ = 0, gk_nmtv_ = 0;  // counts of the captured cycle
-
-  void alloc();
-  void ensure_window(int w);
-  void random_vec(double *d);
-  double norm_dev(Prec p, const void *x, int64_t len, double *slot);
-  void gs_basis_vec(double *v);           // 2 CGS passes vs basis (FP64)
-  void gs_basis_block(double *W, int q);  // same on q columns
-  void cgs_pass(void *w, Prec pw, int64_t len, const std::vector<Group> &groups,
-                double *nrm_out);
-  double estimate_fro();
-  void push_basis(const double *v);
-  // candidate window processing
-  struct ColOut {
-    bool valid = false, hit = false;  // hit: some pass stopped at cg_maxit
-    double nv = 0, res_raw = 0, res = 0;
-    int passes = 0, hits = 0;         // CG solves, and those that hit cg_maxit
-  };
-  void polish_batch(const std::vector<int> &cols, std::vector<ColOut> &out);
-  bool fallback_one();
-  void refine_basis();
-  DeviceBasis finish();
-  void cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
-                int64_t maxit, double rel_tol, std::vector<CgState> &st);
-  CgState cg_callback(const double *rhs, double *w, int64_t maxit, double rel_tol);
-  // the projector CGs of the polish passes on q columns (rhs, w: m x q)
-  void polish_cg(int q, const double *rhs, double *w, Prec pc, const double *diag,
-                 std::vector<CgState> &st);
-  void upload_mask(const std::vector<int> &mask);
-  DBuf mask_;
-  DBuf rin_, rout_;  // row-major copies of block_product
-  std::vector<int> slot_of_;
-};
+}  // namespace
 
 void CyclicDriver::alloc() {
   kp_ = P_.krylov;
@@ -171,12 +93,15 @@ void CyclicDriver::alloc() {
   basis_.buf = DBuf((std::size_t)std::max<int64_t>(1, k_target_) * n_ * 8);
   // Column strides are padded off powers of two (by 17 x 256 bytes) so that
   // the same row of consecutive columns does not map to one HBM channel.
-  ldy_ = pad_ld(n_, esize(kp_));
-  ldx_ = pad_ld(m_, esize(kp_));
-  VB_ = DBuf((std::size_t)(ycap_ + std::max<int64_t>(1, k_target_)) * ldy_ * esize(kp_));
-  X_ = DBuf((std::size_t)xcap_ * ldx_ * esize(kp_));
-  wx_ = DBuf((std::size_t)m_ * esize(kp_));
-  wy_ = DBuf((std::size_t)n_ * esize(kp_));
+  if (!ext_) {
+    ldy_ = pad_ld(n_, esize(kp_));
+    ldx_ = pad_ld(m_, esize(kp_));
+    VB_ = DBuf((std::size_t)(ycap_ + std::max<int64_t>(1, k_target_)) * ldy_ * esize(kp_));
+    X_ = DBuf((std::size_t)xcap_ * ldx_ * esize(kp_));
+    wx_ = DBuf((std::size_t)m_ * esize(kp_));
+    wy_ = DBuf((std::size_t)n_ * esize(kp_));
+    vbp_ = VB_.p, xp_ = X_.p, wxp_ = wx_.p, wyp_ = wy_.p;
+  }
   y0_ = DBuf((std::size_t)n_ * 8);
   best_v_ = DBuf((std::size_t)n_ * 8);
   tmp_m_ = DBuf((std::size_t)m_ * 8);
@@ -303,7 +228,7 @@ void CyclicDriver::push_basis(const double *v) {
   KH_CHECK(hipMemcpyAsync(basis_.col(j), v, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
                           c_.stream));
   convert(c_, Prec::F64, v, kp_,
-          static_cast<char *>(VB_.p) + (std::size_t)(ycap_ + j) * ldy_ * esize(kp_), n_);
+          static_cast<char *>(vbp_) + (std::size_t)(ycap_ + j) * ldy_ * esize(kp_), n_);
   basis_.count = j + 1;
 }
 
@@ -605,20 +530,35 @@ bool CyclicDriver::fallback_one() {
 }
 
 DeviceBasis CyclicDriver::run() {
+  if (prepare())
+    return std::move(basis_);
+  while (cycle_begin()) {
+    const CycleQueued q = queue_cycle();
+    const int st = cycle_end(q.queued, q.nmv, q.nmtv);
+    if (st == kDone)
+      return finish();
+    if (st == kStop)
+      break;
+  }
+  return after_loop();
+}
+
+// The phases of the restart loop of find_right_nullspace
+// (CyclicLanczosSolver.cpp:291-616). prepare: the norm estimate, thresholds
+// and start vector (:306-342); 1 when there is nothing to compute.
+int CyclicDriver::prepare() {
   basis_.len = n_;
   if (n_ == 0 || m_ == 0)
-    return std::move(basis_);
+    return 1;
   k_target_ = (P_.nullity_known >= 0) ? P_.nullity_known : P_.k;
   if (k_target_ == 0) {
     S_.converged_to_target = true;
-    return std::move(basis_);
+    return 1;
   }
   if (P_.rank_known >= 0 && n_ <= P_.rank_known)
-    return std::move(basis_);
-  const int64_t ncv = std::max<int64_t>(std::max<int64_t>(2, P_.ncv), k_target_ + 2);
+    return 1;
+  ncv_ = std::max<int64_t>(std::max<int64_t>(2, P_.ncv), k_target_ + 2);
   alloc();
-  const Prec kp = kp_;
-  const std::size_t ke = esize(kp);
 
   S_.frob_norm_est = estimate_fro();
   accept_thr_ = std::max(P_.tol_abs, P_.tol_rel * std::max(1.0, S_.frob_norm_est));
@@ -626,7 +566,7 @@ DeviceBasis CyclicDriver::run() {
   // unit vectors have ||A v|| <= ||A||_2 <= ||A||_F: a screen below a tenth
   // of the norm can reject candidates
   screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1;
-  graphs_ = A_.capturable() && c_.trace_mask == 0 &&
+  graphs_ = !ext_ && A_.capturable() && c_.trace_mask == 0 &&
             (P_.graphs > 0 || (P_.graphs == 0 && m_ + n_ <= kGraphAutoLen));
 
   double *y0 = y0_.as<double>();
@@ -634,428 +574,467 @@ DeviceBasis CyclicDriver::run() {
     upload_vec(c_, P_.y0_host, n_, y0);
   else
     random_vec(y0);
+  bh_.assign((std::size_t)ncv_, 0.0);
+  restart_ = 0;
+  restarts_since_accept_ = 0;
+  return 0;
+}
 
-  auto random_right = [&]() {
-    random_vec(y0);
-    gs_basis_vec(y0);
+void CyclicDriver::random_right() {  // :332-336
+  double *y0 = y0_.as<double>();
+  random_vec(y0);
+  gs_basis_vec(y0);
+}
+
+void *CyclicDriver::Ycol(int64_t j) const {
+  return static_cast<char *>(vbp_) + (std::size_t)(ycap_ - 1 - j) * ldy_ * esize(kp_);
+}
+
+void *CyclicDriver::Xcol(int64_t j) const {
+  return static_cast<char *>(xp_) + (std::size_t)j * ldx_ * esize(kp_);
+}
+
+// The start vector z0 = [0; y0] of the next cycle in Y's first column
+// (:360-372); false when the restart loop ends.
+bool CyclicDriver::cycle_begin() {
+  if (restart_ >= P_.max_restarts)
+    return false;
+  S_.restarts = restart_;
+  alpha_.clear();
+  beta_.clear();
+  tcyc_ = Clock::now();
+  const Prec kp = kp_;
+  double *y0 = y0_.as<double>();
+  // ---- start vector z0 = [0; y0] orthogonal to the locked vectors
+  gs_basis_vec(y0);
+  double ny = norm_dev(Prec::F64, y0, n_, wsc_);
+  if (!std::isfinite(ny) || ny < 1e-14) {
+    random_right();
+    ny = norm_dev(Prec::F64, y0, n_, wsc_);
+    if (!(ny >= 1e-14))
+      return false;  // the basis already spans R^n
+  }
+  axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / ny, y0);
+  convert(c_, Prec::F64, y0, kp, Ycol(0), n_);
+  return true;
+}
+
+// The Golub-Kahan build of one cycle (:374-421) on this solve's own storage.
+CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
+  const Prec kp = kp_;
+  char *Xb = static_cast<char *>(xp_);
+  const int64_t ncv = ncv_;
+  // ---- Golub-Kahan build (:387-421), queued without host round trips:
+  // beta_j stays on the device (bnorm_[j]) where the next step's -beta z
+  // term and the w / beta store read it. The reference's stop test
+  // (non-finite or < 1e-14) is replayed on the host afterwards; steps
+  // queued after a stop only write columns that are then ignored.
+  const int nb = (int)basis_.count;
+  int64_t nyc = 1, nxc = 0, nmv = 0, nmtv = 0;
+  // The queued cycle depends on nothing but the basis size: same kernels,
+  // shapes and buffers, with beta_j read on the device. Where the operator
+  // only enqueues kernels it is captured once per basis size and replayed.
+  auto build = [&]() {
+    for (int64_t j = 0; j < ncv; ++j) {
+      const bool from_y = nyc > nxc;
+      void *w = from_y ? wxp_ : wyp_;
+      const int64_t len = from_y ? m_ : n_;
+      if (from_y) {
+        A_.mv(c_, kp, Ycol(nyc - 1), w, j == 0 ? nullptr : Xcol(nxc - 1),
+              j == 0 ? nullptr : bnorm_ + (j - 1));
+        ++nmv;
+      } else {
+        A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
+        ++nmtv;
+      }
+      const int passes = P_.reorth;
+      std::vector<Group> groups;
+      if (from_y)
+        groups.push_back({Xb, ldx_, (int)nxc, kp});
+      else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
+        groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
+      const Group &g0 = groups.front();
+      bool stored = false;  // w / beta_j already written to its basis column
+      if (passes == 2 && g0.count > 0) {
+        // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
+        // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
+        // except that h2's partial sums are grouped per block of the fused
+        // kernel.
+        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
+        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
+        if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
+          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
+                               bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
+          stored = true;
+        } else {
+          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+        }
+      } else {
+        for (int pass = 0; pass < passes; ++pass)
+          cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
+      }
+      if (passes <= 0)
+        dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
+      if (nyc + nxc >= ncv)
+        break;
+      if (!stored)
+        scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
+      if (from_y)
+        ++nxc;
+      else
+        ++nyc;
+    }
   };
-
-  char *VBb = static_cast<char *>(VB_.p), *Xb = static_cast<char *>(X_.p);
-  auto Ycol = [&](int64_t j) { return (void *)(VBb + (std::size_t)(ycap_ - 1 - j) * ldy_ * ke); };
-  auto Xcol = [&](int64_t j) { return (void *)(Xb + (std::size_t)j * ldx_ * ke); };
-
-  std::vector<double> alpha, beta, bh((std::size_t)ncv);
-  int64_t restarts_since_accept = 0;
-  for (int64_t restart = 0; restart < P_.max_restarts; ++restart) {
-    S_.restarts = restart;
-    alpha.clear();
-    beta.clear();
-    auto t0 = Clock::now();
-    // ---- start vector z0 = [0; y0] orthogonal to the locked vectors
-    gs_basis_vec(y0);
-    double ny = norm_dev(Prec::F64, y0, n_, wsc_);
-    if (!std::isfinite(ny) || ny < 1e-14) {
-      random_right();
-      ny = norm_dev(Prec::F64, y0, n_, wsc_);
-      if (!(ny >= 1e-14))
-        break;
-    }
-    axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / ny, y0);
-    convert(c_, Prec::F64, y0, kp, Ycol(0), n_);
-    // ---- Golub-Kahan build (:387-421), queued without host round trips:
-    // beta_j stays on the device (bnorm_[j]) where the next step's -beta z
-    // term and the w / beta store read it. The reference's stop test
-    // (non-finite or < 1e-14) is replayed on the host afterwards; steps
-    // queued after a stop only write columns that are then ignored.
-    const int nb = (int)basis_.count;
-    int64_t nyc = 1, nxc = 0, nmv = 0, nmtv = 0;
-    // The queued cycle depends on nothing but the basis size: same kernels,
-    // shapes and buffers, with beta_j read on the device. Where the operator
-    // only enqueues kernels it is captured once per basis size and replayed.
-    auto queue_cycle = [&]() {
-      for (int64_t j = 0; j < ncv; ++j) {
-        const bool from_y = nyc > nxc;
-        void *w = from_y ? wx_.p : wy_.p;
-        const int64_t len = from_y ? m_ : n_;
-        if (from_y) {
-          A_.mv(c_, kp, Ycol(nyc - 1), w, j == 0 ? nullptr : Xcol(nxc - 1),
-                j == 0 ? nullptr : bnorm_ + (j - 1));
-          ++nmv;
-        } else {
-          A_.mtv(c_, kp, Xcol(nxc - 1), w, Ycol(nyc - 1), bnorm_ + (j - 1));
-          ++nmtv;
-        }
-        const int passes = P_.reorth;
-        std::vector<Group> groups;
-        if (from_y)
-          groups.push_back({Xb, ldx_, (int)nxc, kp});
-        else  // [Y_{nyc-1} .. Y_0 | b_0 .. b_{nb-1}], contiguous
-          groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
-        const Group &g0 = groups.front();
-        bool stored = false;  // w / beta_j already written to its basis column
-        if (passes == 2 && g0.count > 0) {
-          // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
-          // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
-          // except that h2's partial sums are grouped per block of the fused
-          // kernel.
-          gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
-          gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
-          if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
-            gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
-                                 bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
-            stored = true;
-          } else {
-            gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
-          }
-        } else {
-          for (int pass = 0; pass < passes; ++pass)
-            cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
-        }
-        if (passes <= 0)
-          dot(c_, kp, w, kp, w, len, bnorm_ + j, true);
-        if (nyc + nxc >= ncv)
-          break;
-        if (!stored)
-          scale_copy_dev(c_, kp, len, bnorm_ + j, w, from_y ? Xcol(nxc) : Ycol(nyc));
-        if (from_y)
-          ++nxc;
-        else
-          ++nyc;
+  if (graphs_) {
+    if (!gk_graph_.holds(nb)) {
+      gk_graph_.begin(c_);
+      try {
+        build();
+      } catch (...) {
+        gk_graph_.end(c_, nb, false);
+        throw;
       }
-    };
-    if (graphs_) {
-      if (!gk_graph_.holds(nb)) {
-        gk_graph_.begin(c_);
-        try {
-          queue_cycle();
-        } catch (...) {
-          gk_graph_.end(c_, nb, false);
-          throw;
-        }
-        gk_graph_.end(c_, nb, true);
-        ++S_.graph_captures;
-        gk_nyc_ = nyc, gk_nxc_ = nxc, gk_nmv_ = nmv, gk_nmtv_ = nmtv;
-      } else {
-        nyc = gk_nyc_, nxc = gk_nxc_, nmv = gk_nmv_, nmtv = gk_nmtv_;
-      }
-      gk_graph_.launch(c_);
-      ++S_.graph_replays;
+      gk_graph_.end(c_, nb, true);
+      ++S_.graph_captures;
+      gk_nyc_ = nyc, gk_nxc_ = nxc, gk_nmv_ = nmv, gk_nmtv_ = nmtv;
     } else {
-      queue_cycle();
+      nyc = gk_nyc_, nxc = gk_nxc_, nmv = gk_nmv_, nmtv = gk_nmtv_;
     }
-    S_.spmv += nmv;
-    S_.spmtv += nmtv;
-    const int64_t queued = nyc + nxc;  // steps queued (the last one stores nothing)
-    readback(c_, bnorm_, (int)queued, bh.data());
-    double beta_last = 0.0;
-    int64_t local_steps = 0;
-    for (int64_t j = 0; j < queued; ++j) {
-      const double bj = bh[(std::size_t)j];
-      alpha.push_back(0.0);
-      beta_last = bj;
-      if (!std::isfinite(bj) || bj < 1e-14 || j + 1 >= ncv)
-        break;
-      beta.push_back(bj);
-      ++local_steps;
-    }
-    S_.krylov_steps_total += local_steps;
-    S_.t.gk += since(t0);
-    const int64_t l = (int64_t)alpha.size();
-    if (l == 0 || !std::isfinite(beta_last))
+    gk_graph_.launch(c_);
+    ++S_.graph_replays;
+  } else {
+    build();
+  }
+  return {nyc + nxc, nmv, nmtv};  // steps queued (the last one stores nothing)
+}
+
+// Everything after the build of a cycle (:423-585): the replay of the beta_j
+// stop test, the projected problem, the candidates with their polish, the
+// stagnation guard and the restart vector.
+int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
+  S_.spmv += nmv;
+  S_.spmtv += nmtv;
+  auto t0 = tcyc_;
+  const Prec kp = kp_;
+  const std::size_t ke = esize(kp);
+  char *VBb = static_cast<char *>(vbp_);
+  const int64_t ncv = ncv_;
+  double *y0 = y0_.as<double>();
+  readback(c_, bnorm_, (int)queued, bh_.data());
+  double beta_last = 0.0;
+  int64_t local_steps = 0;
+  for (int64_t j = 0; j < queued; ++j) {
+    const double bj = bh_[(std::size_t)j];
+    alpha_.push_back(0.0);
+    beta_last = bj;
+    if (!std::isfinite(bj) || bj < 1e-14 || j + 1 >= ncv)
       break;
+    beta_.push_back(bj);
+    ++local_steps;
+  }
+  S_.krylov_steps_total += local_steps;
+  S_.t.gk += since(t0);
+  const int64_t l = (int64_t)alpha_.size();
+  if (l == 0 || !std::isfinite(beta_last))
+    return kStop;
 
-    // ---- tiny problem & interior selection (:428-453)
-    t0 = Clock::now();
-    std::vector<double> ev;
-    dense::Mat V;
-    std::vector<int> ord;
-    bool finite = true;
-    if (P_.use_harmonic) {
-      dense::Mat Tb((int)l + 1, (int)l);
-      for (int64_t i = 0; i < l; ++i) {
-        Tb((int)i, (int)i) = alpha[(std::size_t)i] - P_.target_tau;
-        if (i + 1 < l) {
-          Tb((int)i, (int)i + 1) = beta[(std::size_t)i];
-          Tb((int)i + 1, (int)i) = beta[(std::size_t)i];
-        }
+  // ---- tiny problem & interior selection (:428-453)
+  t0 = Clock::now();
+  std::vector<double> ev;
+  dense::Mat V;
+  std::vector<int> ord;
+  bool finite = true;
+  if (P_.use_harmonic) {
+    dense::Mat Tb((int)l + 1, (int)l);
+    for (int64_t i = 0; i < l; ++i) {
+      Tb((int)i, (int)i) = alpha_[(std::size_t)i] - P_.target_tau;
+      if (i + 1 < l) {
+        Tb((int)i, (int)i + 1) = beta_[(std::size_t)i];
+        Tb((int)i + 1, (int)i) = beta_[(std::size_t)i];
       }
-      Tb((int)l, (int)l - 1) = beta_last;
-      // with a zero diagonal (target_tau = 0) T-bar splits into its even- and
-      // odd-column blocks; candidates read only even rows, so only the even
-      // block's vectors are formed (bitwise the same as the full sweeps)
-      dense::svd_right(Tb, ev, V, true, true);
-      ord.resize(ev.size());
-      std::iota(ord.begin(), ord.end(), 0);
-      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
-    } else {
-      dense::Mat T((int)l, (int)l);
-      for (int64_t i = 0; i < l; ++i) {
-        T((int)i, (int)i) = alpha[(std::size_t)i];
-        if (i + 1 < l) {
-          T((int)i, (int)i + 1) = beta[(std::size_t)i];
-          T((int)i + 1, (int)i) = beta[(std::size_t)i];
-        }
-      }
-      finite = dense::sym_eig(T, ev, V);
-      ord.resize(ev.size());
-      std::iota(ord.begin(), ord.end(), 0);
-      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-        return std::abs(ev[a] - P_.target_tau) < std::abs(ev[b] - P_.target_tau);
-      });
     }
-    double smax = 0.0;  // largest singular value of the projected problem
-    for (double x : ev) {
-      finite = finite && std::isfinite(x);
-      smax = std::max(smax, std::fabs(x));
-    }
-    for (double x : V.a)
-      finite = finite && std::isfinite(x);
-    S_.t.tiny += since(t0);
-    if (!finite) {
-      if (P_.verbose)
-        std::fprintf(stderr, "[Cyclic-HIP] non-finite projected problem, random restart\n");
-      random_right();
-      continue;
-    }
-
-    // ---- candidates (:455-567), processed in windows
-    t0 = Clock::now();
-    const int64_t test_cap = std::min<int64_t>(l, std::max<int64_t>(k_target_, 8));
-    const int ny_used = (int)((l + 1) / 2);
-    std::vector<int> cl;
-    for (int jbest : ord) {
-      double ce = 0.0;
-      for (int64_t i = 0; i < l; i += 2)
-        ce += V((int)i, jbest) * V((int)i, jbest);
-      if (ce >= 1e-20)
-        cl.push_back(jbest);
-    }
-    double best_res = std::numeric_limits<double>::infinity();
-    bool have_best_v = false, accepted_cycle = false, polish_stalled = false;
-    int64_t tested = 0;
-    std::size_t pos = 0;
-    bool single_next = true;
-    bool done_all = false;
-    while (pos < cl.size() && tested < test_cap) {
-      int wn = single_next ? 1 : (int)std::min<int64_t>((int64_t)(cl.size() - pos), test_cap - tested);
-      wn = std::min(wn, 64);
-      ensure_window(wn);
-      // coefficient block C (ny_used x wn); Y_{ny_used-1} .. Y_0 are
-      // contiguous (reversed), so the coefficients are stored in reverse order
-      std::vector<double> Ch((std::size_t)ny_used * wn, 0.0);
-      for (int c = 0; c < wn; ++c) {
-        const int jb = cl[pos + (std::size_t)c];
-        for (int64_t i = 0; i < l; i += 2)
-          Ch[(std::size_t)c * ny_used + (std::size_t)(ny_used - 1 - i / 2)] = V((int)i, jb);
+    Tb((int)l, (int)l - 1) = beta_last;
+    // with a zero diagonal (target_tau = 0) T-bar splits into its even- and
+    // odd-column blocks; candidates read only even rows, so only the even
+    // block's vectors are formed (bitwise the same as the full sweeps)
+    dense::svd_right(Tb, ev, V, true, true);
+    ord.resize(ev.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+  } else {
+    dense::Mat T((int)l, (int)l);
+    for (int64_t i = 0; i < l; ++i) {
+      T((int)i, (int)i) = alpha_[(std::size_t)i];
+      if (i + 1 < l) {
+        T((int)i, (int)i + 1) = beta_[(std::size_t)i];
+        T((int)i + 1, (int)i) = beta_[(std::size_t)i];
       }
-      upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
-      double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
-      std::vector<ColOut> out((std::size_t)wn);
-      // Mixed precision: the window is screened in FP32 first (Y C, two
-      // Gram-Schmidt passes against the FP32 copy of the basis, ||A v||).
-      // A column whose FP32 residual clears the prescreen threshold by more
-      // than the FP32 error margin is rejected as the FP64 test would reject
-      // it (its FP32 vector becomes the best-so-far restart candidate when
-      // it is the best); every other column (possible null vectors, columns
-      // near the basis span) is evaluated in FP64 exactly as without the
-      // screen. Screening switches off for the solve when it keeps failing to
-      // reject columns (thresholds above the spectrum).
-      std::vector<int> sub;             // window columns evaluated in FP64
-      std::vector<char> screened((std::size_t)wn, 0);
-      float *W32 = nullptr;
-      if (screen_) {
-        W32 = Wv32_.as<float>();
-        float *AV32 = AV32_.as<float>();
-        gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F32, W32, n_, n_);
-        const int nb = (int)basis_.count;
-        if (nb > 0) {
-          const void *B32 = VBb + (std::size_t)ycap_ * ldy_ * ke;
-          for (int pass = 0; pass < 2; ++pass) {
-            gemm_tn_f32_mfma(c_, static_cast<const float *>(B32), ldy_, nb, W32, n_, wn, n_, H_);
-            gemm_nn_sub(c_, Prec::F32, B32, ldy_, nb, H_, wn, Prec::F32, W32, n_, n_);
-          }
-        }
-        col_norms(c_, Prec::F32, W32, n_, wn, n_, wsc_);
-        std::vector<double> nv32(wn);
-        readback(c_, wsc_, wn, nv32.data());
-        std::vector<double> sc32(wn, 0.0);
-        std::vector<int> ok32(wn, 0);
-        for (int c = 0; c < wn; ++c)
-          if (std::isfinite(nv32[(std::size_t)c]) && nv32[(std::size_t)c] >= 1e-5) {
-            ok32[(std::size_t)c] = 1;
-            sc32[(std::size_t)c] = 1.0 / nv32[(std::size_t)c];
-          }
-        upload_doubles(c_, sc32.data(), wn, wsc_ + 1024);
-        scale_cols(c_, Prec::F32, n_, wn, W32, n_, wsc_ + 1024, true);
-        upload_mask(ok32);
-        block_product(c_, A_, Prec::F32, wn, W32, n_, AV32, m_, false, mask_.as<int>(), rin_,
-                      rout_);
-        col_norms(c_, Prec::F32, AV32, m_, wn, m_, wsc_ + 2048);
-        std::vector<double> r32(wn);
-        readback(c_, wsc_ + 2048, wn, r32.data());
-        // FP32 error of ||A v|| for a unit v: a few units of 6e-8 times
-        // ||A||_2, estimated (from below) by the largest singular value of
-        // the projected problem; the margin allows 1e-5 of twice that
-        const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
-        for (int c = 0; c < wn; ++c) {
-          const double r = r32[(std::size_t)c];
-          if (ok32[(std::size_t)c] && std::isfinite(r) && r > prescreen_thr_ + margin) {
-            screened[(std::size_t)c] = 1;
-            ColOut &o = out[(std::size_t)c];
-            o.valid = true;
-            o.nv = nv32[(std::size_t)c];
-            o.res_raw = o.res = r;
-          } else {
-            sub.push_back(c);
-          }
-        }
-        S_.screened += (int64_t)(wn - (int)sub.size());
-        if (wn >= 4) {
-          screen_miss_ = (int)sub.size() * 4 > wn * 3 ? screen_miss_ + 1 : 0;
-          if (screen_miss_ >= 3)
-            screen_ = false;
-        }
-        if (!sub.empty() && (int)sub.size() < wn) {  // compact the FP64 coefficients
-          std::vector<double> Cs((std::size_t)ny_used * sub.size());
-          for (std::size_t t = 0; t < sub.size(); ++t)
-            std::copy(Ch.begin() + (std::ptrdiff_t)((std::size_t)sub[t] * ny_used),
-                      Ch.begin() + (std::ptrdiff_t)((std::size_t)(sub[t] + 1) * ny_used),
-                      Cs.begin() + (std::ptrdiff_t)(t * ny_used));
-          upload_doubles(c_, Cs.data(), (int)Cs.size(), C_);
-        }
-      } else {
-        for (int c = 0; c < wn; ++c)
-          sub.push_back(c);
-      }
-      // FP64 evaluation of the columns in `sub` (window column sub[t] sits in
-      // column t of Wv / AV)
-      const int ws = (int)sub.size();
-      std::vector<int> t64((std::size_t)wn, -1);
-      for (int t = 0; t < ws; ++t)
-        t64[(std::size_t)sub[(std::size_t)t]] = t;
-      slot_of_.assign((std::size_t)std::max(ws, 1), -1);
-      if (ws > 0) {
-        gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, ws, Prec::F64, Wv, n_, n_);
-        gs_basis_block(Wv, ws);
-        col_norms(c_, Prec::F64, Wv, n_, ws, n_, wsc_);
-        std::vector<double> nv(ws);
-        readback(c_, wsc_, ws, nv.data());
-        std::vector<ColOut> o64((std::size_t)ws);
-        std::vector<double> sc(ws, 0.0);
-        std::vector<int> valid(ws, 0);
-        for (int t = 0; t < ws; ++t) {
-          o64[(std::size_t)t].nv = nv[(std::size_t)t];
-          if (std::isfinite(nv[(std::size_t)t]) && nv[(std::size_t)t] >= 1e-8) {
-            o64[(std::size_t)t].valid = true;
-            valid[(std::size_t)t] = 1;
-            sc[(std::size_t)t] = 1.0 / nv[(std::size_t)t];
-          }
-        }
-        upload_doubles(c_, sc.data(), ws, wsc_ + 1024);
-        scale_cols(c_, Prec::F64, n_, ws, Wv, n_, wsc_ + 1024, true);
-        upload_mask(valid);
-        block_product(c_, A_, Prec::F64, ws, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
-        col_norms(c_, Prec::F64, AV, m_, ws, m_, wsc_ + 2048);
-        std::vector<double> rr(ws);
-        readback(c_, wsc_ + 2048, ws, rr.data());
-        std::vector<int> pol;
-        for (int t = 0; t < ws; ++t) {
-          ColOut &o = o64[(std::size_t)t];
-          if (!o.valid)
-            continue;
-          ++S_.spmv;
-          o.res_raw = o.res = rr[(std::size_t)t];
-          if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
-              o.res_raw > accept_thr_)
-            pol.push_back(t);
-        }
-        for (std::size_t i = 0; i < pol.size(); ++i)
-          slot_of_[(std::size_t)pol[i]] = (int)i;
-        if (!pol.empty()) {
-          auto tp = Clock::now();
-          polish_batch(pol, o64);
-          S_.t.polish += since(tp);
-        }
-        for (int t = 0; t < ws; ++t)
-          out[(std::size_t)sub[(std::size_t)t]] = o64[(std::size_t)t];
-      }
-      // replay the sequential accept / stall / best logic over the window
-      std::size_t consumed = (std::size_t)wn;
-      bool cut = false;
-      for (int c = 0; c < wn; ++c) {
-        ColOut &o = out[(std::size_t)c];
-        if (!o.valid)
-          continue;
-        ++tested;
-        if (!std::isfinite(o.res_raw))
-          continue;
-        if (o.res_raw > prescreen_thr_ || polish_stalled) {
-          if (o.res_raw < best_res) {
-            best_res = o.res_raw;
-            if (screened[(std::size_t)c])
-              convert(c_, Prec::F32, W32 + (int64_t)c * n_, Prec::F64, best_v_.p, n_);
-            else
-              KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)t64[(std::size_t)c] * n_,
-                                      (std::size_t)n_ * 8, hipMemcpyDeviceToDevice, c_.stream));
-            have_best_v = true;
-          }
-          continue;
-        }
-        const int t = t64[(std::size_t)c];
-        const int slot = slot_of_[(std::size_t)t];
-        const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)t * n_;
-        S_.cg_solves += o.passes;
-        S_.cg_maxit_hits += o.hits;
-        if (P_.verbose)
-          std::fprintf(stderr, "[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
-                      (long long)restart, o.res_raw, o.res, accept_thr_);
-        if (o.res <= accept_thr_) {
-          push_basis(vfin);
-          accepted_cycle = true;
-          if (basis_.count >= k_target_) {
-            S_.converged_to_target = true;
-            done_all = true;
-            break;
-          }
-          consumed = (std::size_t)c + 1;
-          cut = true;
-          break;
-        }
-        polish_stalled = polish_stalled || o.hit;
-        if (o.res < best_res) {
-          best_res = o.res;
-          KH_CHECK(hipMemcpyAsync(best_v_.p, vfin, (std::size_t)n_ * 8,
-                                  hipMemcpyDeviceToDevice, c_.stream));
-          have_best_v = true;
-        }
-      }
-      if (done_all)
-        break;
-      pos += consumed;
-      single_next = cut;
     }
-    S_.t.cand += since(t0);
-    if (done_all)
-      return finish();
-    // ---- stagnation guard (:569-581)
-    if (!accepted_cycle) {
-      ++restarts_since_accept;
-      if (P_.stagnation_restarts > 0 && restarts_since_accept >= P_.stagnation_restarts) {
-        if (P_.verbose)
-          std::fprintf(stderr, "[Cyclic-HIP] stopping due to stagnation\n");
-        break;
-      }
-    } else {
-      restarts_since_accept = 0;
-    }
-    // ---- restart vector (:584)
-    if (have_best_v)
-      KH_CHECK(hipMemcpyAsync(y0, best_v_.p, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
-                              c_.stream));
-    else
-      random_right();
+    finite = dense::sym_eig(T, ev, V);
+    ord.resize(ev.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return std::abs(ev[a] - P_.target_tau) < std::abs(ev[b] - P_.target_tau);
+    });
+  }
+  double smax = 0.0;  // largest singular value of the projected problem
+  for (double x : ev) {
+    finite = finite && std::isfinite(x);
+    smax = std::max(smax, std::fabs(x));
+  }
+  for (double x : V.a)
+    finite = finite && std::isfinite(x);
+  S_.t.tiny += since(t0);
+  if (!finite) {
+    if (P_.verbose)
+      std::fprintf(stderr, "[Cyclic-HIP] non-finite projected problem, random restart\n");
+    random_right();
+    ++restart_;
+    return kContinue;
   }
 
+  // ---- candidates (:455-567), processed in windows
+  t0 = Clock::now();
+  const int64_t test_cap = std::min<int64_t>(l, std::max<int64_t>(k_target_, 8));
+  const int ny_used = (int)((l + 1) / 2);
+  std::vector<int> cl;
+  for (int jbest : ord) {
+    double ce = 0.0;
+    for (int64_t i = 0; i < l; i += 2)
+      ce += V((int)i, jbest) * V((int)i, jbest);
+    if (ce >= 1e-20)
+      cl.push_back(jbest);
+  }
+  double best_res = std::numeric_limits<double>::infinity();
+  bool have_best_v = false, accepted_cycle = false, polish_stalled = false;
+  int64_t tested = 0;
+  std::size_t pos = 0;
+  bool single_next = true;
+  bool done_all = false;
+  while (pos < cl.size() && tested < test_cap) {
+    int wn = single_next ? 1 : (int)std::min<int64_t>((int64_t)(cl.size() - pos), test_cap - tested);
+    wn = std::min(wn, 64);
+    ensure_window(wn);
+    // coefficient block C (ny_used x wn); Y_{ny_used-1} .. Y_0 are
+    // contiguous (reversed), so the coefficients are stored in reverse order
+    std::vector<double> Ch((std::size_t)ny_used * wn, 0.0);
+    for (int c = 0; c < wn; ++c) {
+      const int jb = cl[pos + (std::size_t)c];
+      for (int64_t i = 0; i < l; i += 2)
+        Ch[(std::size_t)c * ny_used + (std::size_t)(ny_used - 1 - i / 2)] = V((int)i, jb);
+    }
+    upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
+    double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
+    std::vector<ColOut> out((std::size_t)wn);
+    // Mixed precision: the window is screened in FP32 first (Y C, two
+    // Gram-Schmidt passes against the FP32 copy of the basis, ||A v||).
+    // A column whose FP32 residual clears the prescreen threshold by more
+    // than the FP32 error margin is rejected as the FP64 test would reject
+    // it (its FP32 vector becomes the best-so-far restart candidate when
+    // it is the best); every other column (possible null vectors, columns
+    // near the basis span) is evaluated in FP64 exactly as without the
+    // screen. Screening switches off for the solve when it keeps failing to
+    // reject columns (thresholds above the spectrum).
+    std::vector<int> sub;             // window columns evaluated in FP64
+    std::vector<char> screened((std::size_t)wn, 0);
+    float *W32 = nullptr;
+    if (screen_) {
+      W32 = Wv32_.as<float>();
+      float *AV32 = AV32_.as<float>();
+      gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F32, W32, n_, n_);
+      const int nb = (int)basis_.count;
+      if (nb > 0) {
+        const void *B32 = VBb + (std::size_t)ycap_ * ldy_ * ke;
+        for (int pass = 0; pass < 2; ++pass) {
+          gemm_tn_f32_mfma(c_, static_cast<const float *>(B32), ldy_, nb, W32, n_, wn, n_, H_);
+          gemm_nn_sub(c_, Prec::F32, B32, ldy_, nb, H_, wn, Prec::F32, W32, n_, n_);
+        }
+      }
+      col_norms(c_, Prec::F32, W32, n_, wn, n_, wsc_);
+      std::vector<double> nv32(wn);
+      readback(c_, wsc_, wn, nv32.data());
+      std::vector<double> sc32(wn, 0.0);
+      std::vector<int> ok32(wn, 0);
+      for (int c = 0; c < wn; ++c)
+        if (std::isfinite(nv32[(std::size_t)c]) && nv32[(std::size_t)c] >= 1e-5) {
+          ok32[(std::size_t)c] = 1;
+          sc32[(std::size_t)c] = 1.0 / nv32[(std::size_t)c];
+        }
+      upload_doubles(c_, sc32.data(), wn, wsc_ + 1024);
+      scale_cols(c_, Prec::F32, n_, wn, W32, n_, wsc_ + 1024, true);
+      upload_mask(ok32);
+      block_product(c_, A_, Prec::F32, wn, W32, n_, AV32, m_, false, mask_.as<int>(), rin_,
+                    rout_);
+      col_norms(c_, Prec::F32, AV32, m_, wn, m_, wsc_ + 2048);
+      std::vector<double> r32(wn);
+      readback(c_, wsc_ + 2048, wn, r32.data());
+      // FP32 error of ||A v|| for a unit v: a few units of 6e-8 times
+      // ||A||_2, estimated (from below) by the largest singular value of
+      // the projected problem; the margin allows 1e-5 of twice that
+      const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
+      for (int c = 0; c < wn; ++c) {
+        const double r = r32[(std::size_t)c];
+        if (ok32[(std::size_t)c] && std::isfinite(r) && r > prescreen_thr_ + margin) {
+          screened[(std::size_t)c] = 1;
+          ColOut &o = out[(std::size_t)c];
+          o.valid = true;
+          o.nv = nv32[(std::size_t)c];
+          o.res_raw = o.res = r;
+        } else {
+          sub.push_back(c);
+        }
+      }
+      S_.screened += (int64_t)(wn - (int)sub.size());
+      if (wn >= 4) {
+        screen_miss_ = (int)sub.size() * 4 > wn * 3 ? screen_miss_ + 1 : 0;
+        if (screen_miss_ >= 3)
+          screen_ = false;
+      }
+      if (!sub.empty() && (int)sub.size() < wn) {  // compact the FP64 coefficients
+        std::vector<double> Cs((std::size_t)ny_used * sub.size());
+        for (std::size_t t = 0; t < sub.size(); ++t)
+          std::copy(Ch.begin() + (std::ptrdiff_t)((std::size_t)sub[t] * ny_used),
+                    Ch.begin() + (std::ptrdiff_t)((std::size_t)(sub[t] + 1) * ny_used),
+                    Cs.begin() + (std::ptrdiff_t)(t * ny_used));
+        upload_doubles(c_, Cs.data(), (int)Cs.size(), C_);
+      }
+    } else {
+      for (int c = 0; c < wn; ++c)
+        sub.push_back(c);
+    }
+    // FP64 evaluation of the columns in `sub` (window column sub[t] sits in
+    // column t of Wv / AV)
+    const int ws = (int)sub.size();
+    std::vector<int> t64((std::size_t)wn, -1);
+    for (int t = 0; t < ws; ++t)
+      t64[(std::size_t)sub[(std::size_t)t]] = t;
+    slot_of_.assign((std::size_t)std::max(ws, 1), -1);
+    if (ws > 0) {
+      gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, ws, Prec::F64, Wv, n_, n_);
+      gs_basis_block(Wv, ws);
+      col_norms(c_, Prec::F64, Wv, n_, ws, n_, wsc_);
+      std::vector<double> nv(ws);
+      readback(c_, wsc_, ws, nv.data());
+      std::vector<ColOut> o64((std::size_t)ws);
+      std::vector<double> sc(ws, 0.0);
+      std::vector<int> valid(ws, 0);
+      for (int t = 0; t < ws; ++t) {
+        o64[(std::size_t)t].nv = nv[(std::size_t)t];
+        if (std::isfinite(nv[(std::size_t)t]) && nv[(std::size_t)t] >= 1e-8) {
+          o64[(std::size_t)t].valid = true;
+          valid[(std::size_t)t] = 1;
+          sc[(std::size_t)t] = 1.0 / nv[(std::size_t)t];
+        }
+      }
+      upload_doubles(c_, sc.data(), ws, wsc_ + 1024);
+      scale_cols(c_, Prec::F64, n_, ws, Wv, n_, wsc_ + 1024, true);
+      upload_mask(valid);
+      block_product(c_, A_, Prec::F64, ws, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
+      col_norms(c_, Prec::F64, AV, m_, ws, m_, wsc_ + 2048);
+      std::vector<double> rr(ws);
+      readback(c_, wsc_ + 2048, ws, rr.data());
+      std::vector<int> pol;
+      for (int t = 0; t < ws; ++t) {
+        ColOut &o = o64[(std::size_t)t];
+        if (!o.valid)
+          continue;
+        ++S_.spmv;
+        o.res_raw = o.res = rr[(std::size_t)t];
+        if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
+            o.res_raw > accept_thr_)
+          pol.push_back(t);
+      }
+      for (std::size_t i = 0; i < pol.size(); ++i)
+        slot_of_[(std::size_t)pol[i]] = (int)i;
+      if (!pol.empty()) {
+        auto tp = Clock::now();
+        polish_batch(pol, o64);
+        S_.t.polish += since(tp);
+      }
+      for (int t = 0; t < ws; ++t)
+        out[(std::size_t)sub[(std::size_t)t]] = o64[(std::size_t)t];
+    }
+    // replay the sequential accept / stall / best logic over the window
+    std::size_t consumed = (std::size_t)wn;
+    bool cut = false;
+    for (int c = 0; c < wn; ++c) {
+      ColOut &o = out[(std::size_t)c];
+      if (!o.valid)
+        continue;
+      ++tested;
+      if (!std::isfinite(o.res_raw))
+        continue;
+      if (o.res_raw > prescreen_thr_ || polish_stalled) {
+        if (o.res_raw < best_res) {
+          best_res = o.res_raw;
+          if (screened[(std::size_t)c])
+            convert(c_, Prec::F32, W32 + (int64_t)c * n_, Prec::F64, best_v_.p, n_);
+          else
+            KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)t64[(std::size_t)c] * n_,
+                                    (std::size_t)n_ * 8, hipMemcpyDeviceToDevice, c_.stream));
+          have_best_v = true;
+        }
+        continue;
+      }
+      const int t = t64[(std::size_t)c];
+      const int slot = slot_of_[(std::size_t)t];
+      const double *vfin = slot >= 0 ? Vc_.as<double>() + (int64_t)slot * n_ : Wv + (int64_t)t * n_;
+      S_.cg_solves += o.passes;
+      S_.cg_maxit_hits += o.hits;
+      if (P_.verbose)
+        std::fprintf(stderr, "[Cyclic-HIP] restart=%lld raw=%.4e polished=%.4e thr=%.4e\n",
+                    (long long)restart_, o.res_raw, o.res, accept_thr_);
+      if (o.res <= accept_thr_) {
+        push_basis(vfin);
+        accepted_cycle = true;
+        if (basis_.count >= k_target_) {
+          S_.converged_to_target = true;
+          done_all = true;
+          break;
+        }
+        consumed = (std::size_t)c + 1;
+        cut = true;
+        break;
+      }
+      polish_stalled = polish_stalled || o.hit;
+      if (o.res < best_res) {
+        best_res = o.res;
+        KH_CHECK(hipMemcpyAsync(best_v_.p, vfin, (std::size_t)n_ * 8,
+                                hipMemcpyDeviceToDevice, c_.stream));
+        have_best_v = true;
+      }
+    }
+    if (done_all)
+      break;
+    pos += consumed;
+    single_next = cut;
+  }
+  S_.t.cand += since(t0);
+  if (done_all)
+    return kDone;
+  // ---- stagnation guard (:569-581)
+  if (!accepted_cycle) {
+    ++restarts_since_accept_;
+    if (P_.stagnation_restarts > 0 && restarts_since_accept_ >= P_.stagnation_restarts) {
+      if (P_.verbose)
+        std::fprintf(stderr, "[Cyclic-HIP] stopping due to stagnation\n");
+      return kStop;
+    }
+  } else {
+    restarts_since_accept_ = 0;
+  }
+  // ---- restart vector (:584)
+  if (have_best_v)
+    KH_CHECK(hipMemcpyAsync(y0, best_v_.p, (std::size_t)n_ * 8, hipMemcpyDeviceToDevice,
+                            c_.stream));
+  else
+    random_right();
+  ++restart_;
+  return kContinue;
+}
+
+// :587-602, then the mixed-precision refinement and the final wait.
+DeviceBasis CyclicDriver::after_loop() {
   // ---- fallback (:587-602)
   if (basis_.count < k_target_) {
     auto t0 = Clock::now();
@@ -1175,8 +1154,6 @@ void CyclicDriver::refine_basis() {
       axpby(c_, Prec::F64, n_, 0.0, nullptr, 1.0 / nu, u);
   }
 }
-
-}  // namespace
 
 void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, int64_t ldx,
                    void *Y, int64_t ldy, bool trans, const int *mask, DBuf &rin, DBuf &rout) {

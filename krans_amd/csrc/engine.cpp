@@ -21,14 +21,14 @@ void hip_check(hipError_t e, const char *what) {
   throw DeviceError(msg);
 }
 
-Context::Context(int dev) : device(dev) {
+Context::Context(int dev, std::size_t part_doubles) : device(dev) {
   KH_CHECK(hipSetDevice(dev));
   KH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   KH_CHECK(hipMalloc((void **)&dscal, sizeof(double) * kScalars));
   KH_CHECK(hipMemset(dscal, 0, sizeof(double) * kScalars));
   KH_CHECK(hipHostMalloc((void **)&hscal, sizeof(double) * kScalars, hipHostMallocDefault));
   KH_CHECK(hipHostMalloc((void **)&hstage, kStage, hipHostMallocDefault));
-  part_cap = (std::size_t)1 << 24;  // 16 Mi doubles = 128 MiB of reduction partials
+  part_cap = part_doubles;  // 16 Mi doubles = 128 MiB of reduction partials by default
   KH_CHECK(hipMalloc((void **)&dpart, sizeof(double) * part_cap));
   hipDeviceProp_t prop;
   KH_CHECK(hipGetDeviceProperties(&prop, dev));
@@ -190,8 +190,8 @@ template <class T> static DBuf upload(Context &ctx, const T *h, std::size_t coun
   return d;
 }
 
-DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
-                      const int32_t *col, const double *val) {
+int validate_csr(int64_t m, int64_t n, const int32_t *rowptr, const int32_t *col,
+                 const double *val) {
   if (m < 0 || n < 0)
     throw std::invalid_argument("csr_create: negative dimension");
   if (m > INT32_MAX || n > INT32_MAX)
@@ -208,8 +208,8 @@ DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
   int max_row = 0;
   for (int64_t i = 0; i < m; ++i) {
     const int32_t a = rowptr[i], b = rowptr[i + 1];
-    if (b < a)
-      throw std::invalid_argument("csr_create: rowptr not monotone");
+    if (b < a || b > nnz)
+      throw std::invalid_argument("csr_create: rowptr not monotone within [0, nnz]");
     max_row = std::max(max_row, b - a);
     for (int32_t p = a; p < b; ++p) {
       if (col[p] < 0 || col[p] >= n)
@@ -219,6 +219,13 @@ DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
             "csr_create: columns must be strictly ascending within a row (canonical CSR)");
     }
   }
+  return max_row;
+}
+
+DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
+                      const int32_t *col, const double *val) {
+  const int max_row = validate_csr(m, n, rowptr, col, val);
+  const int64_t nnz = rowptr[m];
   auto *A = new DeviceCsr;
   try {
     A->m = m;

@@ -76,7 +76,8 @@ struct Context {
   std::size_t part_capacity() const { return part_ext ? part_ext_cap : part_cap; }
   int cu_count = 0;
   static constexpr int kScalars = 1 << 16;
-  explicit Context(int dev);
+  // part_doubles: capacity of the reduction partials (default 16 Mi doubles)
+  explicit Context(int dev, std::size_t part_doubles = std::size_t(1) << 24);
   ~Context();
   Context(const Context &) = delete;
   Context &operator=(const Context &) = delete;
@@ -152,6 +153,12 @@ struct DeviceCsr {
   mutable std::atomic<signed char> block_path[2][2][2]{};
 };
 
+// Checks a host canonical CSR (rowptr[0] = 0, monotone within [0, nnz],
+// columns in range and strictly ascending per row) without reading outside
+// rowptr[0 .. m] and col/val[0 .. nnz); returns the longest row. Throws
+// std::invalid_argument on malformed input.
+int validate_csr(int64_t m, int64_t n, const int32_t *rowptr, const int32_t *col,
+                 const double *val);
 // Build from host canonical CSR (validated). Throws std::invalid_argument
 // on malformed input.
 DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n,
@@ -319,6 +326,48 @@ void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
 // noalloc_scratch returns it and its capacity in doubles.
 void noalloc_prepare(int device);
 double *noalloc_scratch(int device, std::size_t *cap_doubles);
+
+// ---------------------------------------------------------------------------
+// Packed Golub-Kahan steps of several cyclic solves (the serving loop's
+// batched launches). The requests' vectors are stacked in one array, each
+// request's rows padded to a multiple of kPackRows (the row block of the
+// V^T w and update kernels), so every such block belongs to one request, its
+// segment. A launch serves every active segment; a segment's arithmetic
+// depends on its own data only, never on the other segments of the pack.
+constexpr int64_t kPackRows = 2048;
+struct PackSeg {
+  double *bnorm = nullptr;    // the request's beta_j slots (device)
+  int32_t nb = 0;             // its locked-basis size (y-side columns beyond are zero)
+  int32_t active = 0;         // 0: every kernel skips the segment
+  int64_t blk0[2] = {0, 0};   // first block on the m side [0] and the n side [1]
+  int64_t nblk[2] = {0, 0};   // block counts
+};
+struct PackLayout {           // device tables of one pack
+  const PackSeg *segs = nullptr;
+  int nseg = 0;
+  const int32_t *seg_of_blk[2] = {nullptr, nullptr};  // block -> segment, per side
+  int64_t nblk[2] = {0, 0};   // total blocks per side (lengths nblk * kPackRows)
+};
+// y = op(A) x - beta_{seg, jb} z (z may be null; one lane per row, the
+// rounding of the solo lane / ELL kernels) on the packed block-diagonal CSR;
+// side of the output = 0 for A x, 1 for A^T x.
+void pack_spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const PackLayout &L,
+               const void *x, void *y, const void *z, int jb);
+// V^T w block partials over the whole packed length of `side` (part[c * nblk + b]).
+void pack_vtw(Context &ctx, Prec p, const PackLayout &L, int side, const void *V, int64_t ld,
+              int s, const void *w, double *part);
+// per segment and column: H[seg * hs + c] = sum of the segment's partials
+void pack_finish(Context &ctx, const PackLayout &L, int side, const double *part, int s,
+                 double *H, int64_t hs);
+// w -= V h_seg in every active segment (h_seg = H + seg * hs); with norm_part,
+// the per-block partials of ||w_new||^2
+void pack_vh_sub(Context &ctx, Prec p, const PackLayout &L, int side, const void *V, int64_t ld,
+                 int s, const double *H, int64_t hs, void *w, double *norm_part);
+// beta_{seg, j} = ||w_seg|| from the norm partials, into the segment's bnorm[j]
+void pack_norm(Context &ctx, const PackLayout &L, int side, const double *norm_part, int j);
+// dst = w * (1 / beta_{seg, j}) in every active segment
+void pack_scale_copy(Context &ctx, Prec p, const PackLayout &L, int side, int j, const void *w,
+                     void *dst);
 
 // Partial-buffer size needed for the reductions of `len` elements.
 int reduce_blocks(int64_t len);

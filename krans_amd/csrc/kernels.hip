@@ -3172,4 +3172,249 @@ void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
     cg_post_t(ctx, B, (float *)w, diag, maxit, breakdown_tol);
 }
 
+// ------------------------------------------------- packed Golub-Kahan steps --
+namespace {
+
+static_assert(kPackRows == kRB, "a pack segment block is a V^T w block");
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pk_spmv(int64_t rows, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+              const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y,
+              const T *__restrict__ z, const PackSeg *__restrict__ segs,
+              const int32_t *__restrict__ seg_of_blk, int jb) {
+#pragma clang fp contract(off)
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= rows)
+    return;
+  const PackSeg &S = segs[seg_of_blk[i / kPackRows]];
+  if (!S.active)
+    return;
+  const T b = z ? (T)S.bnorm[jb] : (T)0;
+  const T zi = z ? z[i] : (T)0;
+  T out = (T)row_dot<T>(ptr[i], ptr[i + 1], idx, val, [=](int32_t j) { return x[j]; });
+  if (z)
+    out = out + (-b) * zi;
+  y[i] = out;
+}
+
+// k_vh_sub with the coefficients of the block's segment
+template <class TV, class TW>
+__global__ void __launch_bounds__(kBlock)
+    k_pk_vh_sub(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
+                const double *__restrict__ H, int64_t hstride, TW *__restrict__ w,
+                double *__restrict__ part, const PackSeg *__restrict__ segs,
+                const int32_t *__restrict__ seg_of_blk) {
+  constexpr int CU = ColUnroll<TV>::value;
+  extern __shared__ double hs[];  // [s]
+  __shared__ double lds[kWaves];
+  const int sg = seg_of_blk[blockIdx.x];
+  if (!segs[sg].active)
+    return;  // the whole block leaves: no barrier is reached
+  const double *h = H + (int64_t)sg * hstride;
+  for (int j = threadIdx.x; j < s; j += kBlock)
+    hs[j] = h[j];
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
+  int64_t rr[kR];
+  double acc[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    rr[k] = r < len ? r : len - 1;
+    acc[k] = 0.0;
+  }
+  const int s_main = s - s % CU;
+  for (int c0 = 0; c0 < s_main; c0 += CU) {
+    TV x[CU][kR];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const TV *vc = V + (int64_t)(c0 + u) * ld;
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        x[u][k] = vc[rr[k]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const double hc = hs[c0 + u];
+#pragma unroll
+      for (int k = 0; k < kR; ++k)
+        acc[k] += (double)x[u][k] * hc;
+    }
+  }
+  for (int c = s_main; c < s; ++c) {
+    const TV *vc = V + (int64_t)c * ld;
+    const double hc = hs[c];
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      acc[k] += (double)vc[rr[k]] * hc;
+  }
+  TW wold[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k)
+    wold[k] = w[rr[k]];
+  double nacc = 0.0;
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    const int64_t r = r0 + (int64_t)k * kBlock;
+    const TW nw = (TW)((double)wold[k] - acc[k]);
+    if (r < len) {
+      w[r] = nw;
+      nacc += (double)nw * (double)nw;
+    }
+  }
+  if (part) {
+    nacc = block_sum(nacc, lds);
+    if (threadIdx.x == 0)
+      part[blockIdx.x] = nacc;
+  }
+}
+
+// k_finish over one segment's blocks: block (c, seg) sums part[c * nb +
+// blk0 .. blk0 + nblk) in k_finish's order. mode 0: H[seg * hs + c];
+// mode 1 (c = 0): beta = sqrt(max(0, sum)) into the segment's bnorm[jb].
+__global__ void __launch_bounds__(kBlock)
+    k_pk_finish(const double *__restrict__ part, int64_t nb, const PackSeg *__restrict__ segs,
+                int side, double *__restrict__ H, int64_t hs, int mode, int jb) {
+  __shared__ double lds[kWaves];
+  const int c = blockIdx.x, sg = blockIdx.y;
+  const PackSeg &S = segs[sg];
+  if (!S.active)
+    return;
+  const double *pc = part + (int64_t)c * nb + S.blk0[side];
+  const int64_t n = S.nblk[side];
+  double acc = 0.0;
+  int64_t i = threadIdx.x;
+  for (; i + 7 * kBlock < n; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = pc[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      acc += v[u];
+  }
+  for (; i < n; i += kBlock)
+    acc += pc[i];
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    if (mode == 0)
+      H[(int64_t)sg * hs + c] = acc;
+    else
+      S.bnorm[jb] = sqrt(fmax(0.0, acc));
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pk_scale_copy(int64_t len, const PackSeg *__restrict__ segs,
+                    const int32_t *__restrict__ seg_of_blk, int j, const T *__restrict__ x,
+                    T *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= len)
+    return;
+  const PackSeg &S = segs[seg_of_blk[i / kPackRows]];
+  if (!S.active)
+    return;
+  const T sc = (T)(1.0 / S.bnorm[j]);
+  y[i] = x[i] * sc;
+}
+
+}  // namespace
+
+void pack_spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const PackLayout &L,
+               const void *x, void *y, const void *z, int jb) {
+  const int side = trans ? 1 : 0;
+  const int64_t rows = trans ? A.n : A.m;
+  if (rows != L.nblk[side] * kPackRows)
+    throw std::invalid_argument("pack_spmv: matrix rows do not match the pack layout");
+  const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
+  const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
+  const double e = p == Prec::F64 ? 8.0 : 4.0;
+  TraceScope tr(ctx, kTraceSpmv,
+                (e + 4.0) * A.nnz + 4.0 * (rows + 1) + e * (trans ? A.m : A.n) +
+                    e * rows * (z ? 2.0 : 1.0));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_pk_spmv<double>, dim3(grid_rows(rows)), dim3(kBlock), 0, ctx.stream,
+                       rows, ptr, idx, (trans ? A.vt : A.va).as<double>(), (const double *)x,
+                       (double *)y, (const double *)z, L.segs, L.seg_of_blk[side], jb);
+  else
+    hipLaunchKernelGGL(k_pk_spmv<float>, dim3(grid_rows(rows)), dim3(kBlock), 0, ctx.stream,
+                       rows, ptr, idx, (trans ? A.vt32 : A.va32).as<float>(), (const float *)x,
+                       (float *)y, (const float *)z, L.segs, L.seg_of_blk[side], jb);
+  KH_CHECK(hipGetLastError());
+}
+
+void pack_vtw(Context &ctx, Prec p, const PackLayout &L, int side, const void *V, int64_t ld,
+              int s, const void *w, double *part) {
+  const int64_t len = L.nblk[side] * kPackRows;
+  const int nb = (int)L.nblk[side];
+  if (s <= 0)
+    return;
+  if (s > 512)
+    throw std::invalid_argument("pack_vtw: at most 512 columns");
+  if ((std::size_t)nb * s > ctx.part_capacity())
+    throw std::length_error("pack_vtw: reduction partial buffer too small for this pack");
+  TraceScope tr(ctx, p == Prec::F32 ? kTraceVtW : kTraceNone,
+                (double)len * s * (p == Prec::F64 ? 8 : 4) + (double)len * (p == Prec::F64 ? 8 : 4) +
+                    8.0 * nb * s,
+                len, s, /*stamped=*/true);
+  if (p == Prec::F64)
+    launch_vtw_partial(ctx, (const double *)V, ld, s, (const double *)w, len, part, nb, tr.start(),
+                       tr.stop());
+  else
+    launch_vtw_partial(ctx, (const float *)V, ld, s, (const float *)w, len, part, nb, tr.start(),
+                       tr.stop());
+}
+
+void pack_finish(Context &ctx, const PackLayout &L, int side, const double *part, int s,
+                 double *H, int64_t hs) {
+  if (s <= 0 || L.nseg <= 0)
+    return;
+  hipLaunchKernelGGL(k_pk_finish, dim3((unsigned)s, (unsigned)L.nseg), dim3(kBlock), 0,
+                     ctx.stream, part, L.nblk[side], L.segs, side, H, hs, 0, 0);
+  KH_CHECK(hipGetLastError());
+}
+
+void pack_vh_sub(Context &ctx, Prec p, const PackLayout &L, int side, const void *V, int64_t ld,
+                 int s, const double *H, int64_t hs, void *w, double *norm_part) {
+  const int64_t len = L.nblk[side] * kPackRows;
+  const std::size_t lds = (std::size_t)std::max(s, 1) * sizeof(double);
+  TraceScope tr(ctx, kTraceVhSub,
+                (double)len * s * (p == Prec::F64 ? 8 : 4) + 2.0 * len * (p == Prec::F64 ? 8 : 4) +
+                    8.0 * s);
+  if (p == Prec::F64)
+    hipLaunchKernelGGL((k_pk_vh_sub<double, double>), dim3((unsigned)L.nblk[side]), dim3(kBlock),
+                       lds, ctx.stream, len, (const double *)V, ld, s, H, hs, (double *)w,
+                       norm_part, L.segs, L.seg_of_blk[side]);
+  else
+    hipLaunchKernelGGL((k_pk_vh_sub<float, float>), dim3((unsigned)L.nblk[side]), dim3(kBlock),
+                       lds, ctx.stream, len, (const float *)V, ld, s, H, hs, (float *)w,
+                       norm_part, L.segs, L.seg_of_blk[side]);
+  KH_CHECK(hipGetLastError());
+}
+
+void pack_norm(Context &ctx, const PackLayout &L, int side, const double *norm_part, int j) {
+  if (L.nseg <= 0)
+    return;
+  hipLaunchKernelGGL(k_pk_finish, dim3(1, (unsigned)L.nseg), dim3(kBlock), 0, ctx.stream,
+                     norm_part, L.nblk[side], L.segs, side, (double *)nullptr, (int64_t)0, 1, j);
+  KH_CHECK(hipGetLastError());
+}
+
+void pack_scale_copy(Context &ctx, Prec p, const PackLayout &L, int side, int j, const void *w,
+                     void *dst) {
+  const int64_t len = L.nblk[side] * kPackRows;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(k_pk_scale_copy<double>, dim3(grid_rows(len)), dim3(kBlock), 0,
+                       ctx.stream, len, L.segs, L.seg_of_blk[side], j, (const double *)w,
+                       (double *)dst);
+  else
+    hipLaunchKernelGGL(k_pk_scale_copy<float>, dim3(grid_rows(len)), dim3(kBlock), 0,
+                       ctx.stream, len, L.segs, L.seg_of_blk[side], j, (const float *)w,
+                       (float *)dst);
+  KH_CHECK(hipGetLastError());
+}
+
 }  // namespace kh

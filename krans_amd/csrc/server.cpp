@@ -42,6 +42,7 @@ struct Server::Req {
   std::vector<double> basis;  // n x count
   int64_t count = 0;
   CyclicStats stats;
+  int packed = 0;  // members of its pack (0: solved alone)
 };
 
 // Requests return the right null space only: the left space of
@@ -117,6 +118,30 @@ int64_t Server::submit_resident(const DeviceCsr *A, const CyclicParams &P, unsig
   return enqueue(std::move(r));
 }
 
+void Server::set_packing(int pack_max, int64_t pack_len) {
+  if (pack_max < 1 || pack_max > 64)
+    throw std::invalid_argument("server: pack_max must be in [1, 64]");
+  if (pack_len < 0)
+    throw std::invalid_argument("server: pack_len must be >= 0");
+  std::lock_guard<std::mutex> lk(mu_);
+  pack_max_ = pack_max;
+  pack_len_ = pack_len;
+}
+
+bool Server::packs(const Req &r) const {
+  return pack_max_ > 1 && !r.shared && r.m + r.n <= pack_len_ && packable(r.P);
+}
+
+void Server::complete(Req &r) {
+  r.t_end = Clock::now();
+  r.done = true;
+  used_ -= r.footprint;
+  --running_;
+  latencies_.push_back(std::chrono::duration<double>(r.t_end - r.t_submit).count());
+  vectors_ += r.count;
+  last_end_ = r.t_end;
+}
+
 int64_t Server::enqueue(std::shared_ptr<Req> r) {
   if (r->footprint > budget_)
     throw std::invalid_argument("server: request footprint exceeds the HBM budget");
@@ -143,8 +168,11 @@ void Server::work(int slot) {
     return;
   }
   (void)slot;
+  // the member contexts of this slot's packs (own stream each), kept
+  std::vector<std::unique_ptr<Context>> members;
   while (true) {
-    std::shared_ptr<Req> r;
+    std::vector<std::shared_ptr<Req>> batch;
+    double pack_extra = 0.0;
     {
       std::unique_lock<std::mutex> lk(mu_);
       // FIFO admission: the head starts when its footprint fits
@@ -154,12 +182,103 @@ void Server::work(int slot) {
       });
       if (queue_.empty())
         return;  // stopping
-      r = queue_.front();
+      auto r = queue_.front();
       queue_.pop_front();
-      used_ += r->footprint;
-      r->t_start = Clock::now();
-      ++running_;
+      batch.push_back(r);
+      if (packs(*r)) {
+        // compatible small requests among the next queued ones join the pack
+        // while the pack (members + packed storage) fits the budget
+        std::vector<const PackRequest *> probe;
+        auto as_pack = [](const Req &q) {
+          PackRequest pr;
+          pr.m = q.m, pr.n = q.n, pr.rowptr = q.rp.data(), pr.P = q.P;
+          return pr;
+        };
+        std::vector<PackRequest> shapes{as_pack(*r)};
+        double members_fp = r->footprint;
+        const int64_t ncv = pack_ncv_eff(r->P);
+        int scanned = 0;
+        for (auto it = queue_.begin();
+             it != queue_.end() && (int)batch.size() < pack_max_ && scanned < 64; ++scanned) {
+          const Req &q = **it;
+          if (!packs(q) || q.P.krylov != r->P.krylov || pack_ncv_eff(q.P) != ncv) {
+            ++it;
+            continue;
+          }
+          shapes.push_back(as_pack(q));
+          probe.clear();
+          for (const auto &sp : shapes)
+            probe.push_back(&sp);
+          const double extra = pack_footprint(probe);
+          if (used_ + members_fp + q.footprint + extra > budget_) {
+            shapes.pop_back();
+            break;
+          }
+          members_fp += q.footprint;
+          batch.push_back(*it);
+          it = queue_.erase(it);
+        }
+        probe.clear();
+        for (const auto &sp : shapes)
+          probe.push_back(&sp);
+        pack_extra = pack_footprint(probe);
+      }
+      const auto now = Clock::now();
+      for (auto &q : batch) {
+        used_ += q->footprint;
+        q->t_start = now;
+        q->packed = packs(*q) ? (int)batch.size() : 0;
+        ++running_;
+      }
+      used_ += pack_extra;
     }
+    if (batch.front()->packed > 0) {
+      // one pack (possibly of one): members report as they finish
+      std::vector<PackRequest> prs(batch.size());
+      std::vector<PackRequest *> pp;
+      std::vector<Context *> mctx;
+      for (std::size_t i = 0; i < batch.size(); ++i) {
+        Req &q = *batch[i];
+        PackRequest &pr = prs[i];
+        pr.m = q.m, pr.n = q.n, pr.rowptr = q.rp.data(), pr.col = q.ci.data();
+        pr.val = q.va.data(), pr.P = q.P, pr.seed = q.seed;
+        pr.on_done = [this, &pr, &q] {
+          std::lock_guard<std::mutex> lk(mu_);
+          q.status = pr.status;
+          q.error = pr.error;
+          q.count = pr.count;
+          q.basis = std::move(pr.basis);
+          q.stats = pr.stats;
+          complete(q);
+          cv_.notify_all();
+        };
+        pp.push_back(&pr);
+      }
+      std::string fail;
+      try {
+        while (members.size() < batch.size())
+          members.push_back(std::make_unique<Context>(device_, std::size_t(1) << 20));
+        for (std::size_t i = 0; i < batch.size(); ++i)
+          mctx.push_back(members[i].get());
+        solve_packed(*ctx, mctx, pp);
+      } catch (const std::exception &e) {
+        fail = e.what();  // the pack could not start: every member fails alone
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!fail.empty())
+          for (auto &q : batch)
+            if (!q->done) {
+              q->status = 2;
+              q->error = "server: pack failed: " + fail;
+              complete(*q);
+            }
+        used_ -= pack_extra;
+      }
+      cv_.notify_all();
+      continue;
+    }
+    auto r = batch.front();
     try {
       std::unique_ptr<DeviceCsr> own;
       const DeviceCsr *A = r->shared;
@@ -189,14 +308,7 @@ void Server::work(int slot) {
     }
     {
       std::lock_guard<std::mutex> lk(mu_);
-      r->t_end = Clock::now();
-      r->done = true;
-      used_ -= r->footprint;
-      --running_;
-      const double lat = std::chrono::duration<double>(r->t_end - r->t_submit).count();
-      latencies_.push_back(lat);
-      vectors_ += r->count;
-      last_end_ = r->t_end;
+      complete(*r);
     }
     cv_.notify_all();
   }
@@ -220,6 +332,7 @@ Server::Result Server::wait(int64_t id) {
   out.count = r->count;
   out.basis = std::move(r->basis);
   out.stats = r->stats;
+  out.packed = r->packed;
   out.latency = std::chrono::duration<double>(r->t_end - r->t_submit).count();
   out.queued = std::chrono::duration<double>(r->t_start - r->t_submit).count();
   all_.erase(it);

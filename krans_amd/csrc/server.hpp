@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "packed.hpp"
 #include "solver.hpp"
 
 namespace kh {
@@ -36,6 +37,7 @@ public:
     std::vector<double> basis;  // n x count, column-major
     CyclicStats stats;
     double latency = 0, queued = 0;
+    int packed = 0;  // members of the pack its Golub-Kahan cycles ran in (0: solo)
   };
   Result wait(int64_t id);
 
@@ -45,6 +47,11 @@ public:
   };
   Stats stats();
   void reset_clock();
+  // Packing of small requests: a request with m + n <= pack_len (host CSR,
+  // CGS2) is solved in a pack with up to pack_max - 1 compatible queued
+  // requests (same Krylov precision and ncv), alone in a pack of one
+  // otherwise; pack_max <= 1 turns packing off (every request solo).
+  void set_packing(int pack_max, int64_t pack_len);
 
 private:
   struct Req;
@@ -52,6 +59,10 @@ private:
   int64_t enqueue(std::shared_ptr<Req> r);
   void work(int slot);
   int workers_alive() const;
+  bool packs(const Req &r) const;
+  void complete(Req &r);  // bookkeeping of a finished request (lock held)
+  int pack_max_ = 16;
+  int64_t pack_len_ = int64_t(1) << 19;
 
   int device_;
   double budget_;

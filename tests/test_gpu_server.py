@@ -1,5 +1,7 @@
 """Continuous-batching server: each result is the same alone and in the mix
-(bitwise), admission by HBM footprint, error isolation."""
+(bitwise), packed small requests the same alone and in any pack (bitwise) and
+within 1e-10 of the direct solve, admission by HBM footprint, error
+isolation."""
 import numpy as np
 import pytest
 
@@ -47,7 +49,7 @@ def test_same_alone_and_in_the_mix(graphs):
         assert res <= 1e-12 * max(1.0, r.frob_norm_est), (i, res)
         assert np.abs(r.right_null.T @ r.right_null - np.eye(k)).max() <= 1e-10
     print("[case] server mix", flush=True)
-    srv = K.NullspaceServer(slots=4)
+    srv = K.NullspaceServer(slots=4, pack_max=1)
     ids = [srv.submit(A, opts(k, graphs), prec, seed=100 + i)
            for i, (A, k, prec) in enumerate(reqs)]
     for i, rid in enumerate(ids):
@@ -60,6 +62,53 @@ def test_same_alone_and_in_the_mix(graphs):
     st = srv.stats()
     assert st["done"] == len(reqs) and st["p99_s"] >= st["p50_s"] > 0
     srv.close()
+
+
+def small_requests():
+    """Small requests of differing sizes and nullities (a pack's members
+    carry locked bases of different sizes)."""
+    return [(W.regular_incidence(4, 1024, seed=11), 4),
+            (W.shifted_difference(6000, 3, 0.7), 3),
+            (W.grid_incidence(2, 32, seed=12, weighted=False), 2),
+            (W.regular_incidence(8, 512, seed=13), 8),
+            (W.rg_block(3000, p=3, offdiag=8, gnnz=16, seed=14)[0], 3),
+            (W.regular_incidence(2, 3000, seed=15), 2)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_packed_same_alone_and_in_a_pack(prec):
+    """Each request alone in a pack of one and among the others in a pack
+    gives the same bits; both are within 1e-10 (principal sine) of the direct
+    solve and meet the residual threshold."""
+    reqs = small_requests()
+    print(f"[case] packed alone ({prec})", flush=True)
+    srv = K.NullspaceServer(slots=1, pack_max=16)
+    alone = []
+    for i, (A, k) in enumerate(reqs):
+        U, info, _, _ = srv.wait(srv.submit(A, opts(k), prec, seed=200 + i))
+        assert info["packed"] == 1, (i, info["packed"])
+        alone.append(U)
+    print(f"[case] packed together ({prec})", flush=True)
+    ids = [srv.submit(A, opts(k), prec, seed=200 + i) for i, (A, k) in enumerate(reqs)]
+    sizes = []
+    for i, rid in enumerate(ids):
+        U, info, lat, q = srv.wait(rid)
+        sizes.append(info["packed"])
+        assert U.shape == alone[i].shape, i
+        assert np.array_equal(U.view(np.uint64), alone[i].view(np.uint64)), (i, info["packed"])
+        assert lat >= q >= 0
+    # the first request starts on its own; the others queue behind it and pack
+    assert max(sizes) > 1, sizes
+    srv.close()
+    print(f"[case] packed vs direct ({prec})", flush=True)
+    for i, (A, k) in enumerate(reqs):
+        r = K.find_nullspace_cyclic_lanczos_ex(K.HipCsrMatrixAdapter.from_csr(A),
+                                               K.HipVectorFactory(A.n, 200 + i), opts(k), prec)
+        assert alone[i].shape[1] == k == r.right_null.shape[1], i
+        assert principal_sine(alone[i], r.right_null) <= 1e-10, i
+        res = np.linalg.norm(A.to_scipy() @ alone[i], axis=0).max()
+        assert res <= 1e-12 * max(1.0, r.frob_norm_est), (i, res)
+        assert np.abs(alone[i].T @ alone[i] - np.eye(k)).max() <= 1e-10
 
 
 def test_admission_under_a_small_budget_and_resident_matrices():
@@ -79,16 +128,20 @@ def test_admission_under_a_small_budget_and_resident_matrices():
     srv.close()
 
 
-def test_bad_request_fails_alone():
-    srv = K.NullspaceServer(slots=2)
+@pytest.mark.parametrize("pack_max", [1, 16])
+def test_bad_request_fails_alone(pack_max):
+    """A malformed matrix fails its own request, solo or inside a pack."""
+    srv = K.NullspaceServer(slots=2, pack_max=pack_max)
     bad = W.CsrMatrix(2, 2, np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32),
                       np.ones(2))
     good = W.regular_incidence(2, 512, seed=8)
     print("[case] server bad request", flush=True)
     rb = srv.submit(bad, opts(1), "fp64", seed=1)
-    rg = srv.submit(good, opts(2), "fp64", seed=1)
+    rg = [srv.submit(good, opts(2), "fp64", seed=s) for s in (1, 2, 3)]
     with pytest.raises(ValueError):
         srv.wait(rb)
-    U, _, _, _ = srv.wait(rg)
-    assert U.shape[1] == 2
+    Kx = W.component_indicators(2, 512)
+    for r in rg:
+        U, _, _, _ = srv.wait(r)
+        assert U.shape[1] == 2 and principal_sine(U, Kx) <= 1e-10
     srv.close()
