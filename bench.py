@@ -39,10 +39,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--serve", type=int, default=0,
-                    help="serving mode: number of requests of the mixed stream")
-    ap.add_argument("--serve-nmax", type=float, default=1e6)
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per s")
+                    help="serving mode: length of the request stream (2000 for the full one)")
+    ap.add_argument("--serve-ids", default="", help="a:b, the slice of the stream to run")
+    ap.add_argument("--serve-nmax", type=float, default=4e6)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrivals per s (0: all at once, measuring capacity)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-pack", type=int, default=16,
+                    help="largest pack of small requests (1: every request solo)")
+    ap.add_argument("--serve-fp64", action="store_true",
+                    help="also solve the slice one request at a time in FP64")
+    ap.add_argument("--serve-mixed-serial", action="store_true",
+                    help="also solve the slice one request at a time in mixed precision")
+    ap.add_argument("--serve-cpu", type=float, default=0.0,
+                    help="seconds of oracle throughput on the slice's n <= 1e5 requests")
+    ap.add_argument("--serve-out", default="", help="file for the per-request records")
+    ap.add_argument("--serve-merge", nargs="*", default=None,
+                    help="summarise the records of several --serve-out files")
     ap.add_argument("--config", default="c2", choices=["c2", "c3a", "c3b", "c4"],
                     help="c2: the headline incidence solve; c3a/c3b: precision variants "
                          "(shifted difference n=2^24, weighted grid incidence); c4: LOBPCG "
@@ -84,6 +97,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+    if args.serve_merge:
+        serve_merge(args.serve_merge)
+        return
     if args.serve > 0:
         serve(args)
         return
@@ -457,156 +473,293 @@ def variant(args):
     print(json.dumps(line))
 
 
-def serving_requests(count, nmax, seed=7):
-    """The serving stream: each request draws uniformly from the three
-    generator families (shifted difference, incidence of random 4-regular
-    graphs, [R | RG]); n log-uniform in [1e4, nmax], kernel dimension uniform
-    in 1..32, tolerance 1e-12."""
+C5_REQUESTS = 2000
+C5_NMAX = 4e6
+
+
+def c5_request(i, nmax=C5_NMAX):
+    """Request i of the serving stream, drawn from its own seed (7, i): the
+    family uniform over {shifted difference (c = 0.5), incidence (random
+    4-regular graphs or unweighted grids, even odds), [R | RG]}, n
+    log-uniform in [1e4, nmax], kernel dimension k uniform in 1..32,
+    tolerance 1e-12. Returns (family, CsrMatrix, k)."""
     from krans_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        fam = int(rng.integers(0, 3))
-        n = int(np.exp(rng.uniform(np.log(1e4), np.log(nmax))))
-        k = int(rng.integers(1, 33))
-        if fam == 0:
-            A = W.shifted_difference(n, k, 0.5)
-        elif fam == 1:
-            verts = max(64, n // k)
-            A = W.regular_incidence(k, verts, seed=1000 + i)
-        else:
-            A = W.rg_block(max(n - k, 64), p=k, offdiag=8, gnnz=16, seed=1000 + i)[0]
-        out.append((["shiftdiff", "incidence", "rg"][fam], A, k))
+    rng = np.random.default_rng([7, i])
+    fam = int(rng.integers(0, 3))
+    n = int(np.exp(rng.uniform(np.log(1e4), np.log(nmax))))
+    k = int(rng.integers(1, 33))
+    grid = bool(rng.integers(0, 2))
+    seed = 1000 + i
+    if fam == 0:
+        return "shiftdiff", W.shifted_difference(n, k, 0.5), k
+    if fam == 1 and not grid:
+        return "incidence", W.regular_incidence(k, max(64, n // k), seed=seed), k
+    if fam == 1:
+        side = max(8, int(round(np.sqrt(n / k))))
+        return "grid", W.grid_incidence(k, side, seed=seed, weighted=False), k
+    return "rg", W.rg_block(max(n - k, 64), p=k, offdiag=8, gnnz=16, seed=seed)[0], k
+
+
+def _c5_make(args):
+    i, nmax = args
+    return c5_request(i, nmax)
+
+
+def c5_requests(ids, nmax=C5_NMAX, procs=16):
+    """The requests `ids` of the stream, generated in a process pool."""
+    import multiprocessing as mp
+    ids = list(ids)
+    if len(ids) <= 2:
+        return [c5_request(i, nmax) for i in ids]
+    with mp.get_context("spawn").Pool(max(1, min(procs, len(ids)))) as pool:
+        return pool.map(_c5_make, [(i, nmax) for i in ids], chunksize=1)
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    pos = q * (len(xs) - 1)
+    lo = int(np.floor(pos))
+    hi = min(lo + 1, len(xs) - 1)
+    return float(xs[lo] + (xs[hi] - xs[lo]) * (pos - lo))
+
+
+def serve_opts(K, k):
+    return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
+                                  max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS, cg_maxit=1000)
+
+
+def rel_residual(A, U):
+    """max_j ||A u_j|| / ||A||_F (0 for an empty basis)."""
+    if U.shape[1] == 0:
+        return 0.0
+    R = A.to_scipy() @ U
+    return float(np.sqrt((R * R).sum(axis=0)).max() / A.fro())
+
+
+def run_server(K, ids, reqs, rate, slots, pack_max, seed, log=None):
+    """Submits `reqs` to a fresh server (mixed precision, solver seed = 1000 +
+    request id) at Poisson arrivals of `rate` per second (rate 0: all at
+    once, the closed loop that measures capacity) and waits for every
+    result. Returns (per-request records, wall seconds, server stats)."""
+    import queue
+    import threading
+    arrivals = (np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, size=len(reqs)))
+                if rate > 0 else np.zeros(len(reqs)))
+    srv = K.NullspaceServer(slots=slots, pack_max=pack_max)
+    srv.reset_clock()
+    q = queue.Queue()
+    t0 = time.perf_counter()
+
+    def submitter():
+        for j, ((fam, A, k), ta) in enumerate(zip(reqs, arrivals)):
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            q.put((j, srv.submit(A, serve_opts(K, k), "mixed", seed=1000 + ids[j])))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    recs = []
+    for _ in range(len(reqs)):
+        j, rid = q.get()
+        U, info, lat, qd = srv.wait(rid)
+        fam, A, k = reqs[j]
+        recs.append({"id": ids[j], "family": fam, "m": A.m, "n": A.n, "nnz": A.nnz, "k": k,
+                     "found": int(U.shape[1]), "sojourn_s": lat, "queued_s": qd,
+                     "service_s": lat - qd, "packed": info["packed"],
+                     "restarts": info["restarts"], "res_rel": rel_residual(A, U)})
+        if log:
+            log(recs[-1])
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return recs, wall, st
+
+
+def run_serial(K, ids, reqs, precision="fp64", log=None):
+    """The same requests one at a time (direct solve, same options and
+    solver seed), plain FP64 by default: per-request records and wall
+    seconds."""
+    recs, t0 = [], time.perf_counter()
+    for j, (fam, A, k) in enumerate(reqs):
+        H = K.HipCsrMatrixAdapter.from_csr(A)
+        ts = time.perf_counter()
+        r = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 1000 + ids[j]),
+                                               serve_opts(K, k), precision)
+        dt = time.perf_counter() - ts
+        recs.append({"id": ids[j], "family": fam, "n": A.n, "k": k,
+                     "found": int(r.right_null.shape[1]), "service_s": dt,
+                     "res_rel": rel_residual(A, r.right_null)})
+        del H
+        if log:
+            log(recs[-1])
+    return recs, time.perf_counter() - t0
+
+
+def summarize(recs, wall):
+    found = sum(r["found"] for r in recs)
+    out = {"requests": len(recs), "vectors": {"found": found,
+                                              "requested": sum(r["k"] for r in recs)},
+           "wall_s": round(wall, 3), "vectors_per_s": round(found / wall, 3),
+           "requests_per_s": round(len(recs) / wall, 4),
+           "max_res_rel": max((r["res_rel"] for r in recs), default=0.0),
+           "short": [r["id"] for r in recs if r["found"] < r["k"]]}
+    for key in ("service_s", "sojourn_s"):
+        xs = [r[key] for r in recs if key in r]
+        if xs:
+            out[key] = {"p50": round(pct(xs, 0.5), 4), "p99": round(pct(xs, 0.99), 4)}
     return out
 
 
 def serve(args):
-    """Serving stream on one GPU: the continuous-batching server (mixed
-    precision, --serve-slots concurrent solves, HBM-footprint admission)
-    against plain FP64 one-at-a-time solves of the same requests."""
-    import threading
+    """The serving stream (2,000 requests, n in [1e4, 4e6]) on one GPU, a
+    slice at a time (--serve-ids a:b): the continuous-batching server in
+    mixed precision (--serve-slots solver slots, small requests packed up to
+    --serve-pack), at Poisson arrivals of --serve-rate per second (0: all at
+    once, which measures capacity); with --serve-fp64 the same requests one at
+    a time in FP64; with --serve-cpu the oracle's throughput (one process per
+    core) on the slice's requests of n <= 1e5 beside a closed-loop server run
+    of exactly those requests. Prints one JSON line; --serve-out keeps the
+    per-request records for --serve-merge."""
     import krans_amd as K
-    reqs = serving_requests(args.serve, args.serve_nmax)
-    rng = np.random.default_rng(11)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(reqs)))
+    a, b = (int(x) for x in args.serve_ids.split(":")) if args.serve_ids else (0, args.serve)
+    ids = list(range(a, min(b, args.serve)))
+    t = time.perf_counter()
+    reqs = c5_requests(ids, args.serve_nmax)
+    gen_s = time.perf_counter() - t
+    print(f"[serve] generated requests {a}..{ids[-1]} in {gen_s:.1f}s", file=sys.stderr,
+          flush=True)
 
-    def opt(k):
-        return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
-                                      max_restarts=MAX_RESTARTS, graphs=HIPGRAPHS, cg_maxit=1000)
+    def log(tag):
+        def f(r):
+            print(f"[{tag}] id {r['id']} {r['family']} n={r['n']} k={r['k']} "
+                  f"found={r['found']} service {r['service_s']:.3f}s", file=sys.stderr,
+                  flush=True)
+        return f
 
-    # batched mixed-precision server
-    srv = K.NullspaceServer(slots=args.serve_slots)
-    srv.reset_clock()
-    ids = []
-    t0 = time.perf_counter()
-    for (fam, A, k), ta in zip(reqs, arrivals):
-        dt = ta - (time.perf_counter() - t0)
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(srv.submit(A, opt(k), "mixed", seed=42))
-    res = []
-    for i, r in enumerate(ids):
-        res.append(srv.wait(r))
-        print(f"[serve] done {i + 1}/{len(ids)} latency {res[-1][2]:.3f}s", file=sys.stderr,
-              flush=True)
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    found = sum(r[0].shape[1] for r in res)
-    want = sum(k for _, _, k in reqs)
-    worst = 0.0
-    for (fam, A, k), (U, info, lat, q) in zip(reqs, res):
-        S = A.to_scipy()
-        for j in range(U.shape[1]):
-            worst = max(worst, np.linalg.norm(S @ U[:, j]) / A.fro())
-    lat = sorted(r[2] for r in res)
+    line = {"metric": "null vectors/sec + p50/p99 service and sojourn at "
+                      "||Au||/||A||_F<=1e-12, serving stream, 1 MI355X",
+            "unit": "null vectors/s", "n_gpus": 1, "stream": C5_REQUESTS,
+            "ids": [a, ids[-1] + 1], "n_range": [1e4, args.serve_nmax],
+            "rate_per_s": args.serve_rate, "slots": args.serve_slots,
+            "pack_max": args.serve_pack, "generate_s": round(gen_s, 1),
+            "families": {f: sum(1 for r in reqs if r[0] == f)
+                         for f in ("shiftdiff", "incidence", "grid", "rg")}}
+    out = {"line": None, "server": [], "server_wall": 0.0}
+    if args.serve_slots > 0:
+        recs, wall, st = run_server(K, ids, reqs, args.serve_rate, args.serve_slots,
+                                    args.serve_pack, seed=11 + a, log=log("server"))
+        line["server"] = summarize(recs, wall)
+        line["server"]["stats"] = st
+        line["value"] = line["server"]["vectors_per_s"]
+        out["server"], out["server_wall"] = recs, wall
+    if args.serve_fp64:
+        frecs, fwall = run_serial(K, ids, reqs, "fp64", log=log("fp64"))
+        line["fp64_serial"] = summarize(frecs, fwall)
+        if "server" in line:
+            line["server_over_fp64_serial"] = round(
+                line["server"]["vectors_per_s"] / line["fp64_serial"]["vectors_per_s"], 3)
+        out["fp64"], out["fp64_wall"] = frecs, fwall
+    if args.serve_mixed_serial:
+        mrecs, mwall = run_serial(K, ids, reqs, "mixed", log=log("mixed"))
+        line["mixed_serial"] = summarize(mrecs, mwall)
+        out["mixed"], out["mixed_wall"] = mrecs, mwall
+    if args.serve_cpu > 0:
+        line["cpu_baseline"] = serve_cpu_baseline(K, ids, reqs, args, args.serve_cpu)
+    out["line"] = line
+    if args.serve_out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.serve_out)), exist_ok=True)
+        with open(args.serve_out, "w") as f:
+            json.dump(out, f)
+    print(json.dumps(line))
 
-    # plain FP64, one request at a time on the default context
-    t1 = time.perf_counter()
-    serial_found, serial_t = 0, []
-    for fam, A, k in reqs:
-        H = K.HipCsrMatrixAdapter.from_csr(A)
-        ts = time.perf_counter()
-        r = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 42), opt(k), "fp64")
-        serial_t.append(time.perf_counter() - ts)
-        serial_found += r.right_null.shape[1]
-        print(f"[serial fp64] {len(serial_t)}/{len(reqs)} {fam} n={A.n} k={k} "
-              f"{serial_t[-1]:.3f}s found={r.right_null.shape[1]}", file=sys.stderr, flush=True)
-    serial_wall = time.perf_counter() - t1
 
-    def pct(xs, q):
-        xs = sorted(xs)
-        pos = q * (len(xs) - 1)
-        lo = int(np.floor(pos))
-        hi = min(lo + 1, len(xs) - 1)
-        return xs[lo] + (xs[hi] - xs[lo]) * (pos - lo)
-
-    cpu = None if args.no_cpu else serve_cpu_baseline(reqs, args.cpu_seconds)
-    value = found / wall
-    print(json.dumps({
-        "metric": "null vectors/sec + p99 latency at ||Au||/||A||_F<=1e-12, serving stream, "
-                  "batched mixed vs one-at-a-time FP64, 1 MI355X",
-        "value": round(value, 3), "unit": "null vectors/s", "n_gpus": 1,
-        "requests": len(reqs), "rate_per_s": args.serve_rate, "slots": args.serve_slots,
-        "n_range": [1e4, args.serve_nmax],
-        "families": {f: sum(1 for r in reqs if r[0] == f) for f in ("shiftdiff", "incidence", "rg")},
-        "vectors": {"found": found, "requested": want},
-        "latency_s": {"p50": round(pct(lat, 0.5), 4), "p99": round(pct(lat, 0.99), 4)},
-        "server_stats": st,
-        "wall_s": round(wall, 3),
-        "max_res_rel": worst,
-        "fp64_serial": {"value": round(serial_found / serial_wall, 3), "found": serial_found,
-                        "wall_s": round(serial_wall, 3),
-                        "service_s": {"p50": round(pct(serial_t, 0.5), 4),
-                                      "p99": round(pct(serial_t, 0.99), 4)}},
-        "batched_over_fp64_serial": round(value / (serial_found / serial_wall), 3),
-        "cpu_baseline": cpu,
-    }))
+def serve_merge(paths):
+    """One summary of several slices' records (--serve-out files): vectors/s
+    over the summed wall time of the slices, percentiles over all requests."""
+    recs, wall, lines, frecs, fwall = [], 0.0, [], [], 0.0
+    for p in paths:
+        with open(p) as f:
+            d = json.load(f)
+        recs += d["server"]
+        wall += d["server_wall"]
+        lines.append(d["line"])
+        frecs += d.get("fp64", [])
+        fwall += d.get("fp64_wall", 0.0)
+    out = {"slices": [{"ids": ln["ids"], "rate_per_s": ln["rate_per_s"],
+                       "wall_s": ln["server"]["wall_s"]} for ln in lines],
+           "server": summarize(recs, wall)}
+    fam = {}
+    for r in recs:
+        d = fam.setdefault(r["family"], {"requests": 0, "found": 0, "requested": 0})
+        d["requests"] += 1
+        d["found"] += r["found"]
+        d["requested"] += r["k"]
+    out["families"] = fam
+    out["packed_requests"] = sum(1 for r in recs if r["packed"] > 1)
+    if frecs:
+        out["fp64_serial"] = summarize(frecs, fwall)
+    print(json.dumps(out))
 
 
 def _cpu_solve(item):
     """One request on the oracle (single thread, FP64): (vectors, seconds)."""
     import oracle as O
-    m, n, rowptr, col, val, k = item
+    m, n, rowptr, col, val, k, seed = item
     kw = O.cyclic_defaults(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=PRESCREEN,
                            max_restarts=MAX_RESTARTS, cg_maxit=1000)
     t = time.perf_counter()
-    U, _, _ = O.cyclic(O.Operator.csr(m, n, rowptr, col, val), O.Factory(42), kw)
+    U, _, _ = O.cyclic(O.Operator.csr(m, n, rowptr, col, val), O.Factory(seed), kw)
     return U.shape[1], time.perf_counter() - t
 
 
-def serve_cpu_baseline(reqs, budget_s):
-    """Throughput mode of the CPU baseline on the serving stream: the oracle
-    (FP64, single-threaded per process) in one process per host core (at most
-    16), over the stream's requests of n <= 1e5 in arrival order, for about
-    budget_s seconds of wall time. Rate = vectors found / wall time."""
+def serve_cpu_baseline(K, ids, reqs, args, budget_s):
+    """Throughput of the CPU baseline on the slice's requests of n <= 1e5:
+    the oracle (FP64, single-threaded) in one process per host core (at most
+    16), every such request in arrival order within about budget_s seconds;
+    beside it, the server (closed loop) on exactly the requests the CPU
+    finished. Rate = vectors found / wall time, on both sides."""
     import multiprocessing as mp
     cores = max(1, min(16, os.cpu_count() or 1))
-    small = sorted(((A.m, A.n, A.rowptr, A.col, A.val, k) for fam, A, k in reqs
-                    if A.n <= 100_000), key=lambda it: it[1] * it[5])
+    small = [j for j, (fam, A, k) in enumerate(reqs) if A.n <= 100_000]
     if not small:
         return None
-    found, done, t0 = 0, 0, time.perf_counter()
+    items = [(reqs[j][1].m, reqs[j][1].n, reqs[j][1].rowptr, reqs[j][1].col, reqs[j][1].val,
+              reqs[j][2], 1000 + ids[j]) for j in small]
+    found, finished, t0 = 0, [], time.perf_counter()
     with mp.get_context("spawn").Pool(cores) as pool:
-        pending = [pool.apply_async(_cpu_solve, (it,)) for it in small]
+        pending = [(j, pool.apply_async(_cpu_solve, (it,))) for j, it in zip(small, items)]
         while pending and time.perf_counter() - t0 < budget_s:
             still = []
-            for r in pending:
+            for j, r in pending:
                 if r.ready():
                     found += r.get()[0]
-                    done += 1
+                    finished.append(j)
                 else:
-                    still.append(r)
+                    still.append((j, r))
             pending = still
-            time.sleep(0.05)
+            time.sleep(0.02)
         wall = time.perf_counter() - t0
         pool.terminate()
-    return {"value": round(found / wall, 3), "unit": "null vectors/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{done} of the stream's {len(small)} requests with n <= 1e5, "
-                      f"cheapest first (FP64 oracle, one process per core, {wall:.1f} s "
-                      "wall, requests finished within it); larger requests are slower per "
-                      "vector, so this overstates the CPU rate on the full stream"}
+    if not finished:
+        return None
+    finished.sort()
+    grecs, gwall, _ = run_server(K, [ids[j] for j in finished], [reqs[j] for j in finished],
+                                 0.0, args.serve_slots, args.serve_pack, seed=0)
+    gpu = summarize(grecs, gwall)
+    cpu_rate = found / wall
+    return {"value": round(cpu_rate, 3), "unit": "null vectors/s", "cores": cores,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"{len(finished)} of the slice's {len(small)} requests with n <= 1e5, "
+                      f"in arrival order (FP64 oracle, one process per core, {wall:.1f} s "
+                      "wall, requests finished within it); gpu_same_requests is the server "
+                      "(closed loop) on exactly those requests",
+            "gpu_same_requests": {"vectors_per_s": gpu["vectors_per_s"],
+                                  "wall_s": gpu["wall_s"]},
+            "gpu_over_cpu": round(gpu["vectors_per_s"] / cpu_rate, 2) if cpu_rate > 0 else None}
 
 
 def copy_probe(K, ctx, mib=256):

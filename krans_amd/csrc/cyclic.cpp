@@ -421,27 +421,28 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
     S_.spmtv += qa;
     sub_cols(c_, Prec::F64, n_, qa, Vw, n_, Tn, n_, Vp, n_, nullptr);
     gs_basis_block(Vp, qa);
+    // ||v'||, A v' and ||A v'|| in one pass and one read-back; v' and A v'
+    // are then normalised in place
+    block_product(c_, A_, Prec::F64, qa, Vp, n_, AVp, m_, false, nullptr, rin_, rout_);
     col_norms(c_, Prec::F64, Vp, n_, qa, n_, wsc_);
-    std::vector<double> np(qa);
-    readback(c_, wsc_, qa, np.data());
-    std::vector<double> sc(qa, 0.0);
+    col_norms(c_, Prec::F64, AVp, m_, qa, m_, wsc_ + 64);
+    std::vector<double> nr((std::size_t)(64 + qa));
+    readback(c_, wsc_, 64 + qa, nr.data());
+    std::vector<double> sc(qa, 0.0), rp(qa, 0.0);
     std::vector<int> ok(qa, 0);
     for (int a = 0; a < qa; ++a) {
-      const double v = np[(std::size_t)a];
+      const double v = nr[(std::size_t)a];
       if (std::isfinite(v) && v >= 1e-14) {
         ok[(std::size_t)a] = 1;
         sc[(std::size_t)a] = 1.0 / v;
+        rp[(std::size_t)a] = nr[(std::size_t)(64 + a)] / v;
       }
     }
     upload_doubles(c_, sc.data(), qa, wsc_ + 1024);
     scale_cols(c_, Prec::F64, n_, qa, Vp, n_, wsc_ + 1024, true);
-    upload_mask(ok);
-    block_product(c_, A_, Prec::F64, qa, Vp, n_, AVp, m_, false, mask_.as<int>(), rin_, rout_);
+    scale_cols(c_, Prec::F64, m_, qa, AVp, m_, wsc_ + 1024, true);
     for (int a = 0; a < qa; ++a)
       S_.spmv += ok[(std::size_t)a];
-    col_norms(c_, Prec::F64, AVp, m_, qa, m_, wsc_ + 2048);
-    std::vector<double> rp(qa);
-    readback(c_, wsc_ + 2048, qa, rp.data());
     for (int a = 0; a < qa; ++a) {
       const int i = act[(std::size_t)a];
       if (!ok[(std::size_t)a] || !(rp[(std::size_t)a] < res[(std::size_t)i])) {
@@ -805,6 +806,64 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
     if (ce >= 1e-20)
       cl.push_back(jbest);
   }
+  // coefficient column of candidate jb (ny_used entries); Y_{ny_used-1} ..
+  // Y_0 are contiguous (reversed), so the coefficients are stored in
+  // reverse order
+  auto coeffs = [&](int jb, double *dst) {
+    for (int64_t i = 0; i < l; i += 2)
+      dst[(std::size_t)(ny_used - 1 - i / 2)] = V((int)i, jb);
+  };
+  // Mixed precision: every candidate of the restart (at most 64) is screened
+  // in FP32 once, before any of them is tested: w = Y c, ||A w|| / ||w||,
+  // one read-back. The candidates are orthogonal to the locked basis to
+  // FP32 rounding (the basis is part of every y-side reorthogonalisation),
+  // and removing basis components from w can only shrink ||w||, so a
+  // candidate whose FP32 residual clears the prescreen threshold by more
+  // than the FP32 error margin is rejected as the FP64 test would reject
+  // it, against the basis of any later point of the restart (its FP32
+  // vector becomes the best-so-far restart candidate when it is the best).
+  // Every other candidate (possible null vectors) is evaluated in FP64
+  // exactly as without the screen. Screening switches off for the solve
+  // when it keeps failing to reject candidates (thresholds above the
+  // spectrum).
+  const int ncl = (int)std::min<std::size_t>(cl.size(), 64);
+  std::vector<char> rej32((std::size_t)ncl, 0);
+  std::vector<double> nv32((std::size_t)ncl, 0.0), r32((std::size_t)ncl, 0.0);
+  float *W32 = nullptr;
+  if (screen_ && ncl > 0) {
+    ensure_window(ncl);
+    W32 = Wv32_.as<float>();
+    float *AV32 = AV32_.as<float>();
+    std::vector<double> Ch((std::size_t)ny_used * ncl, 0.0);
+    for (int c = 0; c < ncl; ++c)
+      coeffs(cl[(std::size_t)c], Ch.data() + (std::size_t)c * ny_used);
+    upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
+    gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, ncl, Prec::F32, W32, n_, n_);
+    block_product(c_, A_, Prec::F32, ncl, W32, n_, AV32, m_, false, nullptr, rin_, rout_);
+    col_norms(c_, Prec::F32, W32, n_, ncl, n_, wsc_);
+    col_norms(c_, Prec::F32, AV32, m_, ncl, m_, wsc_ + 64);
+    std::vector<double> nr((std::size_t)(64 + ncl));
+    readback(c_, wsc_, 64 + ncl, nr.data());
+    // FP32 error of ||A v|| for a unit v: a few units of 6e-8 times
+    // ||A||_2, estimated (from below) by the largest singular value of
+    // the projected problem; the margin allows 1e-5 of twice that
+    const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
+    int kept = 0;
+    for (int c = 0; c < ncl; ++c) {
+      const double nv = nr[(std::size_t)c], r = nr[(std::size_t)(64 + c)] / nv;
+      nv32[(std::size_t)c] = nv;
+      r32[(std::size_t)c] = r;
+      if (std::isfinite(nv) && nv >= 1e-5 && std::isfinite(r) && r > prescreen_thr_ + margin)
+        rej32[(std::size_t)c] = 1;
+      else
+        ++kept;
+    }
+    if (ncl >= 4) {
+      screen_miss_ = kept * 4 > ncl * 3 ? screen_miss_ + 1 : 0;
+      if (screen_miss_ >= 3)
+        screen_ = false;
+    }
+  }
   double best_res = std::numeric_limits<double>::infinity();
   bool have_best_v = false, accepted_cycle = false, polish_stalled = false;
   int64_t tested = 0;
@@ -815,92 +874,28 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
     int wn = single_next ? 1 : (int)std::min<int64_t>((int64_t)(cl.size() - pos), test_cap - tested);
     wn = std::min(wn, 64);
     ensure_window(wn);
-    // coefficient block C (ny_used x wn); Y_{ny_used-1} .. Y_0 are
-    // contiguous (reversed), so the coefficients are stored in reverse order
-    std::vector<double> Ch((std::size_t)ny_used * wn, 0.0);
-    for (int c = 0; c < wn; ++c) {
-      const int jb = cl[pos + (std::size_t)c];
-      for (int64_t i = 0; i < l; i += 2)
-        Ch[(std::size_t)c * ny_used + (std::size_t)(ny_used - 1 - i / 2)] = V((int)i, jb);
-    }
-    upload_doubles(c_, Ch.data(), (int)Ch.size(), C_);
     double *Wv = Wv_.as<double>(), *AV = AV_.as<double>();
     std::vector<ColOut> out((std::size_t)wn);
-    // Mixed precision: the window is screened in FP32 first (Y C, two
-    // Gram-Schmidt passes against the FP32 copy of the basis, ||A v||).
-    // A column whose FP32 residual clears the prescreen threshold by more
-    // than the FP32 error margin is rejected as the FP64 test would reject
-    // it (its FP32 vector becomes the best-so-far restart candidate when
-    // it is the best); every other column (possible null vectors, columns
-    // near the basis span) is evaluated in FP64 exactly as without the
-    // screen. Screening switches off for the solve when it keeps failing to
-    // reject columns (thresholds above the spectrum).
     std::vector<int> sub;             // window columns evaluated in FP64
     std::vector<char> screened((std::size_t)wn, 0);
-    float *W32 = nullptr;
-    if (screen_) {
-      W32 = Wv32_.as<float>();
-      float *AV32 = AV32_.as<float>();
-      gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, wn, Prec::F32, W32, n_, n_);
-      const int nb = (int)basis_.count;
-      if (nb > 0) {
-        const void *B32 = VBb + (std::size_t)ycap_ * ldy_ * ke;
-        for (int pass = 0; pass < 2; ++pass) {
-          gemm_tn_f32_mfma(c_, static_cast<const float *>(B32), ldy_, nb, W32, n_, wn, n_, H_);
-          gemm_nn_sub(c_, Prec::F32, B32, ldy_, nb, H_, wn, Prec::F32, W32, n_, n_);
-        }
-      }
-      col_norms(c_, Prec::F32, W32, n_, wn, n_, wsc_);
-      std::vector<double> nv32(wn);
-      readback(c_, wsc_, wn, nv32.data());
-      std::vector<double> sc32(wn, 0.0);
-      std::vector<int> ok32(wn, 0);
-      for (int c = 0; c < wn; ++c)
-        if (std::isfinite(nv32[(std::size_t)c]) && nv32[(std::size_t)c] >= 1e-5) {
-          ok32[(std::size_t)c] = 1;
-          sc32[(std::size_t)c] = 1.0 / nv32[(std::size_t)c];
-        }
-      upload_doubles(c_, sc32.data(), wn, wsc_ + 1024);
-      scale_cols(c_, Prec::F32, n_, wn, W32, n_, wsc_ + 1024, true);
-      upload_mask(ok32);
-      block_product(c_, A_, Prec::F32, wn, W32, n_, AV32, m_, false, mask_.as<int>(), rin_,
-                    rout_);
-      col_norms(c_, Prec::F32, AV32, m_, wn, m_, wsc_ + 2048);
-      std::vector<double> r32(wn);
-      readback(c_, wsc_ + 2048, wn, r32.data());
-      // FP32 error of ||A v|| for a unit v: a few units of 6e-8 times
-      // ||A||_2, estimated (from below) by the largest singular value of
-      // the projected problem; the margin allows 1e-5 of twice that
-      const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
-      for (int c = 0; c < wn; ++c) {
-        const double r = r32[(std::size_t)c];
-        if (ok32[(std::size_t)c] && std::isfinite(r) && r > prescreen_thr_ + margin) {
-          screened[(std::size_t)c] = 1;
-          ColOut &o = out[(std::size_t)c];
-          o.valid = true;
-          o.nv = nv32[(std::size_t)c];
-          o.res_raw = o.res = r;
-        } else {
-          sub.push_back(c);
-        }
-      }
-      S_.screened += (int64_t)(wn - (int)sub.size());
-      if (wn >= 4) {
-        screen_miss_ = (int)sub.size() * 4 > wn * 3 ? screen_miss_ + 1 : 0;
-        if (screen_miss_ >= 3)
-          screen_ = false;
-      }
-      if (!sub.empty() && (int)sub.size() < wn) {  // compact the FP64 coefficients
-        std::vector<double> Cs((std::size_t)ny_used * sub.size());
-        for (std::size_t t = 0; t < sub.size(); ++t)
-          std::copy(Ch.begin() + (std::ptrdiff_t)((std::size_t)sub[t] * ny_used),
-                    Ch.begin() + (std::ptrdiff_t)((std::size_t)(sub[t] + 1) * ny_used),
-                    Cs.begin() + (std::ptrdiff_t)(t * ny_used));
-        upload_doubles(c_, Cs.data(), (int)Cs.size(), C_);
-      }
-    } else {
-      for (int c = 0; c < wn; ++c)
+    for (int c = 0; c < wn; ++c) {
+      const std::size_t g = pos + (std::size_t)c;
+      if (g < rej32.size() && rej32[g]) {
+        screened[(std::size_t)c] = 1;
+        ColOut &o = out[(std::size_t)c];
+        o.valid = true;
+        o.nv = nv32[g];
+        o.res_raw = o.res = r32[g];
+        ++S_.screened;
+      } else {
         sub.push_back(c);
+      }
+    }
+    if (!sub.empty()) {  // FP64 coefficients of the columns in `sub`
+      std::vector<double> Cs((std::size_t)ny_used * sub.size(), 0.0);
+      for (std::size_t t = 0; t < sub.size(); ++t)
+        coeffs(cl[pos + (std::size_t)sub[t]], Cs.data() + t * ny_used);
+      upload_doubles(c_, Cs.data(), (int)Cs.size(), C_);
     }
     // FP64 evaluation of the columns in `sub` (window column sub[t] sits in
     // column t of Wv / AV)
@@ -910,36 +905,36 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
       t64[(std::size_t)sub[(std::size_t)t]] = t;
     slot_of_.assign((std::size_t)std::max(ws, 1), -1);
     if (ws > 0) {
+      // w = Y c, Gram-Schmidt against the basis; A w, ||w|| and ||A w|| in
+      // one pass and one read-back (v = w / ||w||, so ||A v|| = ||A w|| /
+      // ||w||); v and A v are then scaled in place for the polish
       gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, ws, Prec::F64, Wv, n_, n_);
       gs_basis_block(Wv, ws);
+      block_product(c_, A_, Prec::F64, ws, Wv, n_, AV, m_, false, nullptr, rin_, rout_);
       col_norms(c_, Prec::F64, Wv, n_, ws, n_, wsc_);
-      std::vector<double> nv(ws);
-      readback(c_, wsc_, ws, nv.data());
+      col_norms(c_, Prec::F64, AV, m_, ws, m_, wsc_ + 64);
+      std::vector<double> nr((std::size_t)(64 + ws));
+      readback(c_, wsc_, 64 + ws, nr.data());
       std::vector<ColOut> o64((std::size_t)ws);
       std::vector<double> sc(ws, 0.0);
-      std::vector<int> valid(ws, 0);
       for (int t = 0; t < ws; ++t) {
-        o64[(std::size_t)t].nv = nv[(std::size_t)t];
-        if (std::isfinite(nv[(std::size_t)t]) && nv[(std::size_t)t] >= 1e-8) {
+        const double nv = nr[(std::size_t)t];
+        o64[(std::size_t)t].nv = nv;
+        if (std::isfinite(nv) && nv >= 1e-8) {
           o64[(std::size_t)t].valid = true;
-          valid[(std::size_t)t] = 1;
-          sc[(std::size_t)t] = 1.0 / nv[(std::size_t)t];
+          sc[(std::size_t)t] = 1.0 / nv;
         }
       }
       upload_doubles(c_, sc.data(), ws, wsc_ + 1024);
       scale_cols(c_, Prec::F64, n_, ws, Wv, n_, wsc_ + 1024, true);
-      upload_mask(valid);
-      block_product(c_, A_, Prec::F64, ws, Wv, n_, AV, m_, false, mask_.as<int>(), rin_, rout_);
-      col_norms(c_, Prec::F64, AV, m_, ws, m_, wsc_ + 2048);
-      std::vector<double> rr(ws);
-      readback(c_, wsc_ + 2048, ws, rr.data());
+      scale_cols(c_, Prec::F64, m_, ws, AV, m_, wsc_ + 1024, true);
       std::vector<int> pol;
       for (int t = 0; t < ws; ++t) {
         ColOut &o = o64[(std::size_t)t];
         if (!o.valid)
           continue;
         ++S_.spmv;
-        o.res_raw = o.res = rr[(std::size_t)t];
+        o.res_raw = o.res = nr[(std::size_t)(64 + t)] / o.nv;
         if (std::isfinite(o.res_raw) && !(o.res_raw > prescreen_thr_) && !polish_stalled &&
             o.res_raw > accept_thr_)
           pol.push_back(t);
@@ -967,9 +962,13 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
       if (o.res_raw > prescreen_thr_ || polish_stalled) {
         if (o.res_raw < best_res) {
           best_res = o.res_raw;
-          if (screened[(std::size_t)c])
-            convert(c_, Prec::F32, W32 + (int64_t)c * n_, Prec::F64, best_v_.p, n_);
-          else
+          if (screened[(std::size_t)c]) {  // the unnormalised FP32 screen vector
+            const double sc = 1.0 / o.nv;
+            convert(c_, Prec::F32, W32 + (int64_t)(pos + (std::size_t)c) * n_, Prec::F64,
+                    best_v_.p, n_);
+            upload_doubles(c_, &sc, 1, wsc_ + 1024);
+            scale_cols(c_, Prec::F64, n_, 1, best_v_.p, n_, wsc_ + 1024, true);
+          } else
             KH_CHECK(hipMemcpyAsync(best_v_.p, Wv + (int64_t)t64[(std::size_t)c] * n_,
                                     (std::size_t)n_ * 8, hipMemcpyDeviceToDevice, c_.stream));
           have_best_v = true;

@@ -18,11 +18,14 @@
 // Everything else of a solve (the beta_j replay, the projected problem, the
 // candidates, the projector polish, restarts, fallback) is the member's own
 // CyclicDriver, run on the member's thread and stream exactly as in a solo
-// solve (CyclicLanczosSolver.cpp:291-616). Members meet once per round: the
-// leader waits until every member still in the pack is ready (or has left),
-// builds the cycle of the ready ones on its own stream after their events,
-// and releases them with the cycle's completion event.
+// solve (CyclicLanczosSolver.cpp:291-616). Rounds are opportunistic: once a
+// member is ready the leader gathers whoever else becomes ready within a
+// short window (all remaining members at most), builds the cycle of that set
+// on its own stream after their events, and releases each with an event of
+// its own recorded after the cycle. A member busy with its own restart work
+// joins a later round instead of holding the others back.
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <limits>
 #include <mutex>
@@ -45,17 +48,23 @@ int64_t target_of(const CyclicParams &P) {
   return std::max<int64_t>(1, P.nullity_known >= 0 ? P.nullity_known : P.k);
 }
 
-// The members' round-by-round rendezvous with the leader.
+// How long the leader waits, after the first member of a round is ready,
+// for the others to join it.
+constexpr auto kGatherWindow = std::chrono::microseconds(300);
+
+// The members' rendezvous with the leader.
 struct Rendezvous {
   std::mutex mu;
   std::condition_variable cv;
   int total = 0, settled = 0, left = 0;
-  std::vector<int> ready;           // members waiting for this round's cycle
+  std::vector<int> ready;           // members waiting for a cycle
   std::vector<int64_t> ready_nb;    // their locked-basis sizes
   std::vector<hipEvent_t> ready_ev; // their start vectors' events
-  int64_t round = 0;
-  hipEvent_t done = nullptr;        // completion of the last cycle (leader stream)
-  int64_t queued = 0, nmv = 0, nmtv = 0;
+  // per member: set when its cycle is queued, with that cycle's counts and
+  // its release event (recorded on the leader stream after the cycle)
+  std::vector<char> go;
+  std::vector<CyclicDriver::CycleQueued> out;
+  std::vector<hipEvent_t> release;
   bool failed = false;
   std::string error;
 };
@@ -257,6 +266,19 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
 
   Rendezvous R;
   R.total = P;
+  R.go.assign((std::size_t)P, 0);
+  R.out.resize((std::size_t)P);
+  R.release.assign((std::size_t)P, nullptr);
+  struct Events {  // destroys the release events when the pack ends
+    std::vector<hipEvent_t> &v;
+    ~Events() {
+      for (hipEvent_t e : v)
+        if (e)
+          (void)hipEventDestroy(e);
+    }
+  } release_guard{R.release};
+  for (int i = 0; i < P; ++i)
+    KH_CHECK(hipEventCreateWithFlags(&R.release[(std::size_t)i], hipEventDisableTiming));
   auto member = [&](int i) {
     PackRequest &q = *reqs[(std::size_t)i];
     Context &mc = *members[(std::size_t)i];
@@ -275,6 +297,7 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
       R.cv.notify_all();
     };
     hipEvent_t E = nullptr;
+    const hipEvent_t F = R.release[(std::size_t)i];
     try {
       KH_CHECK(hipSetDevice(mc.device));
       std::unique_ptr<DeviceCsr> A(csr_create(mc, q.m, q.n, q.rowptr, q.col, q.val));
@@ -297,19 +320,17 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
         while (d.cycle_begin()) {
           KH_CHECK(hipEventRecord(E, mc.stream));
           CyclicDriver::CycleQueued cq{};
-          hipEvent_t F = nullptr;
           {
             std::unique_lock<std::mutex> lk(R.mu);
+            R.go[(std::size_t)i] = 0;
             R.ready.push_back(i);
             R.ready_nb.push_back(d.basis_count());
             R.ready_ev.push_back(E);
             R.cv.notify_all();
-            const int64_t mine = R.round;
-            R.cv.wait(lk, [&] { return R.round != mine || R.failed; });
+            R.cv.wait(lk, [&] { return R.go[(std::size_t)i] || R.failed; });
             if (R.failed)
               throw std::runtime_error("packed cycle failed: " + R.error);
-            cq = {R.queued, R.nmv, R.nmtv};
-            F = R.done;
+            cq = R.out[(std::size_t)i];
           }
           KH_CHECK(hipStreamWaitEvent(mc.stream, F, 0));
           st = d.cycle_end(cq.queued, cq.nmv, cq.nmtv);
@@ -371,9 +392,7 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
     if (reqs[(std::size_t)i]->on_done)
       reqs[(std::size_t)i]->on_done();
 
-  hipEvent_t F = nullptr;
   try {
-    KH_CHECK(hipEventCreateWithFlags(&F, hipEventDisableTiming));
     {
       std::unique_lock<std::mutex> lk(R.mu);
       R.cv.wait(lk, [&] { return R.settled == R.total; });
@@ -384,12 +403,14 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
       std::vector<hipEvent_t> evs;
       {
         std::unique_lock<std::mutex> lk(R.mu);
-        R.cv.wait(lk, [&] { return (int)R.ready.size() + R.left == R.total; });
+        R.cv.wait(lk, [&] { return !R.ready.empty() || R.left == R.total; });
         if (R.ready.empty())
           break;  // every member has left
-        rd = R.ready;
-        nbs = R.ready_nb;
-        evs = R.ready_ev;
+        R.cv.wait_for(lk, kGatherWindow,
+                      [&] { return (int)R.ready.size() + R.left == R.total; });
+        rd.swap(R.ready);
+        nbs.swap(R.ready_nb);
+        evs.swap(R.ready_ev);
       }
       int nbmax = 0;
       for (auto &S : segs)
@@ -404,17 +425,14 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
       for (hipEvent_t ev : evs)
         KH_CHECK(hipStreamWaitEvent(leader.stream, ev, 0));
       const CyclicDriver::CycleQueued cq = cyc.run(nbmax);
-      KH_CHECK(hipEventRecord(F, leader.stream));
+      for (int i : rd)
+        KH_CHECK(hipEventRecord(R.release[(std::size_t)i], leader.stream));
       {
         std::lock_guard<std::mutex> lk(R.mu);
-        R.ready.clear();
-        R.ready_nb.clear();
-        R.ready_ev.clear();
-        R.queued = cq.queued;
-        R.nmv = cq.nmv;
-        R.nmtv = cq.nmtv;
-        R.done = F;
-        ++R.round;
+        for (int i : rd) {
+          R.out[(std::size_t)i] = cq;
+          R.go[(std::size_t)i] = 1;
+        }
         R.cv.notify_all();
       }
     }
@@ -427,8 +445,6 @@ void solve_packed(Context &leader, const std::vector<Context *> &members,
   for (auto &t : threads)
     t.join();
   leader.sync();
-  if (F)
-    (void)hipEventDestroy(F);
 }
 
 }  // namespace kh
