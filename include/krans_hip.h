@@ -209,6 +209,15 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
                       int precision, const double *y0, double *right,
                       int64_t right_cap, double *left, int64_t left_cap,
                       kh_cyclic_result *res);
+/* kh_cyclic_lanczos with the bases written to device memory of the
+ * matrix's device (right_dev n x right_cap, left_dev m x left_cap, column-
+ * major; either may be NULL): the reference's result vectors of a device
+ * VectorFactory (CyclicLanczosSolver.hpp:170-174, HipVectorFactory here),
+ * with no host copy. `y0` stays a host vector. */
+int kh_cyclic_lanczos_dev(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
+                          int precision, const double *y0, double *right_dev,
+                          int64_t right_cap, double *left_dev, int64_t left_cap,
+                          kh_cyclic_result *res);
 /* find_nullspace_lobpcg_ex (LOBPCGSolver.hpp:142-149); theta and
  * residuals_Av per accepted vector; initial_block n x n_initial or NULL. */
 int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt,
@@ -246,9 +255,10 @@ int kh_server_reset_clock(kh_server *s);
 /* Batched launches for small requests: a request with m + n <= pack_len
  * (host CSR, reorth 2) is solved in a pack with up to pack_max - 1
  * compatible queued requests (same precision and ncv), whose Golub-Kahan
- * cycles run in shared segmented launches (defaults 16 and 2^19); alone it
- * runs in a pack of one, so its result is bitwise the same alone and in any
- * mix. pack_max = 1 solves every request on its own. */
+ * cycles run in shared segmented launches; alone it runs in a pack of one,
+ * so its result is bitwise the same alone and in any mix. pack_max = 1 (the
+ * default, with pack_len 2^19) solves every request on its own. No
+ * counterpart in the reference, which has no server. */
 int kh_server_set_packing(kh_server *s, int pack_max, int64_t pack_len);
 
 /* --- device buffers and kernel entry points (tests, benchmarks) ---------- */

@@ -114,6 +114,8 @@ SIGNATURES = {
     "kh_tiny_svd_right": (I32, [P, I32, I32, I32, P, P]),
     "kh_cyclic_lanczos": (I32, [P, P, C.POINTER(_CyclicOpts), I32, P, P, I64, P, I64,
                                 C.POINTER(_CyclicRes)]),
+    "kh_cyclic_lanczos_dev": (I32, [P, P, C.POINTER(_CyclicOpts), I32, P, P, I64, P, I64,
+                                    C.POINTER(_CyclicRes)]),
     "kh_lobpcg": (I32, [P, P, C.POINTER(_LobpcgOpts), I32, P, I64, P, I64, P, P,
                         C.POINTER(_LobpcgRes)]),
     "kh_server_create": (I32, [I32, I32, D, C.POINTER(P)]),
@@ -504,6 +506,7 @@ class CyclicLanczosResult:
     graph_captures: int = 0  # Golub-Kahan cycles captured as HIP graphs
     graph_replays: int = 0   # ... and replayed
     refined: int = 0  # mixed: locked vectors replaced by the final FP64 projector pass
+    n_right: int = -1  # columns written to the device block (find_nullspace_cyclic_lanczos_dev)
 
 
 def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
@@ -536,6 +539,70 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
         refined=res.refined)
+
+
+class DeviceBlock:
+    """Column-major block of `cols` FP64 device vectors of length `rows` on a
+    context's device (kh_dev_alloc): the result storage of
+    find_nullspace_cyclic_lanczos_dev."""
+
+    def __init__(self, ctx, rows, cols):
+        self.ctx, self.rows, self.cols = ctx, int(rows), int(cols)
+        p = P()
+        _check(lib().kh_dev_alloc(ctx.h, self.rows * self.cols * 8, C.byref(p)))
+        self.ptr = p
+
+    def to_host(self, count=None):
+        """The first `count` columns as an (rows x count) array."""
+        count = self.cols if count is None else int(count)
+        out = np.empty((count, self.rows))
+        if count:
+            _check(lib().kh_d2h(self.ctx.h, _ptr(out), self.ptr, count * self.rows * 8))
+        return out.T.copy()
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().kh_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def find_nullspace_cyclic_lanczos_dev(A, vf, opt, out, precision="fp64"):
+    """find_nullspace_cyclic_lanczos_ex with the right basis left on the
+    device (kh_cyclic_lanczos_dev): its columns are written to the
+    DeviceBlock `out` (A.n rows, at least k columns); the result's right_null
+    is empty and n_right counts the columns written."""
+    if not isinstance(A, HipMatrix):
+        raise ValueError("find_nullspace_cyclic_lanczos: backend mismatch (HIP matrix required)")
+    if opt.return_left_space:
+        raise ValueError("find_nullspace_cyclic_lanczos_dev returns the right space only")
+    if out.rows != A.n:
+        raise ValueError("output block rows must equal A.n")
+    o = opt._c()
+    y0 = None
+    if opt.initial_right_space:
+        y0 = np.ascontiguousarray(opt.initial_right_space[0], dtype=np.float64)
+        if y0.shape != (A.n,):
+            raise ValueError("initial_right_space[0] size mismatch")
+    res = _CyclicRes()
+    _check(lib().kh_cyclic_lanczos_dev(A.h, vf.h, C.byref(o), PREC[precision],
+                                       None if y0 is None else _ptr(y0), out.ptr, out.cols,
+                                       None, 0, C.byref(res)))
+    times = {k: getattr(res, "t_" + k) for k in ("gk", "tiny", "cand", "polish", "fallback", "rng")}
+    r = CyclicLanczosResult(
+        right_null=np.zeros((A.n, 0)), left_null=np.zeros((A.m, 0)),
+        restarts=res.restarts, krylov_steps_total=res.krylov_steps_total,
+        frob_norm_est=res.frob_norm_est, converged_to_target=bool(res.converged_to_target),
+        cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
+        spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
+        graph_captures=res.graph_captures, graph_replays=res.graph_replays,
+        refined=res.refined, n_right=res.n_right)
+    return r
 
 
 def find_nullspace_cyclic_lanczos(A, vf, opt, precision="fp64"):
@@ -692,7 +759,7 @@ class NullspaceServer:
     admitted in arrival order when one of `slots` solver slots (one device
     stream each) is free and their HBM footprint fits the budget."""
 
-    def __init__(self, slots=4, hbm_budget_bytes=0.0, device=0, pack_max=16, pack_len=1 << 19):
+    def __init__(self, slots=4, hbm_budget_bytes=0.0, device=0, pack_max=1, pack_len=1 << 19):
         h = P()
         _check(lib().kh_server_create(device, slots, float(hbm_budget_bytes), C.byref(h)))
         self.h = h
@@ -702,7 +769,7 @@ class NullspaceServer:
     def set_packing(self, pack_max, pack_len=1 << 19):
         """Small requests (m + n <= pack_len) run in packs of up to pack_max
         whose Golub-Kahan cycles share segmented launches (kh_server_set_packing);
-        pack_max=1 solves every request on its own."""
+        pack_max=1 (the default) solves every request on its own."""
         _check(lib().kh_server_set_packing(self.h, int(pack_max), int(pack_len)))
 
     def close(self):

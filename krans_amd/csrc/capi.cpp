@@ -91,13 +91,15 @@ kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
   return P;
 }
 
-void put_basis(kh::Context &ctx, const kh::DeviceBasis &B, double *out, int64_t cap) {
+// out: host memory, or device memory of the context's device (dev)
+void put_basis(kh::Context &ctx, const kh::DeviceBasis &B, double *out, int64_t cap,
+               bool dev = false) {
   if (!out)
     return;
   const int64_t k = std::min(B.count, cap);
   if (k > 0)
     KH_CHECK(hipMemcpyAsync(out, B.buf.p, (std::size_t)(k * B.len) * 8,
-                            hipMemcpyDeviceToHost, ctx.stream));
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx.stream));
   ctx.sync();
 }
 }  // namespace
@@ -346,9 +348,10 @@ int kh_tiny_svd_right(const double *a, int32_t rows, int32_t cols, int32_t mode,
   });
 }
 
-int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
-                      int precision, const double *y0, double *right, int64_t right_cap,
-                      double *left, int64_t left_cap, kh_cyclic_result *res) {
+namespace {
+int cyclic_entry(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt, int precision,
+                 const double *y0, double *right, int64_t right_cap, double *left,
+                 int64_t left_cap, kh_cyclic_result *res, bool dev) {
   return guard([&] {
     need(A && rng && opt && res, "null argument");
     auto &c = *A->owner->ctx;
@@ -359,7 +362,7 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     kh::DeviceBasis R = kh::cyclic_right(c, *op, rng->s, P, S);
     std::memset(res, 0, sizeof(*res));
     res->n_right = R.count;
-    put_basis(c, R, right, right_cap);
+    put_basis(c, R, right, right_cap, dev);
     kh::CyclicStats SL;
     if (P.return_left_space && A->rows() > 0 && A->cols() > 0) {
       // CyclicLanczosSolver.cpp:626-659: a second run on A^T with the listed
@@ -375,7 +378,7 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
       const auto opt_t = A->view(true);
       kh::DeviceBasis Lb = kh::cyclic_right(c, *opt_t, rng->s, L, SL);
       res->n_left = Lb.count;
-      put_basis(c, Lb, left, left_cap);
+      put_basis(c, Lb, left, left_cap, dev);
     }
     res->restarts = S.restarts;
     res->krylov_steps_total = S.krylov_steps_total;
@@ -397,6 +400,21 @@ int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
     res->graph_captures = S.graph_captures + SL.graph_captures;
     res->graph_replays = S.graph_replays + SL.graph_replays;
   });
+}
+}  // namespace
+
+int kh_cyclic_lanczos(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
+                      int precision, const double *y0, double *right, int64_t right_cap,
+                      double *left, int64_t left_cap, kh_cyclic_result *res) {
+  return cyclic_entry(A, rng, opt, precision, y0, right, right_cap, left, left_cap, res, false);
+}
+
+int kh_cyclic_lanczos_dev(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt,
+                          int precision, const double *y0, double *right_dev,
+                          int64_t right_cap, double *left_dev, int64_t left_cap,
+                          kh_cyclic_result *res) {
+  return cyclic_entry(A, rng, opt, precision, y0, right_dev, right_cap, left_dev, left_cap, res,
+                      true);
 }
 
 int kh_lobpcg(kh_matrix *A, kh_rng *rng, const kh_lobpcg_options *opt, int precision,

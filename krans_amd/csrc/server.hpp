@@ -50,7 +50,9 @@ public:
   // Packing of small requests: a request with m + n <= pack_len (host CSR,
   // CGS2) is solved in a pack with up to pack_max - 1 compatible queued
   // requests (same Krylov precision and ncv), alone in a pack of one
-  // otherwise; pack_max <= 1 turns packing off (every request solo).
+  // otherwise; pack_max <= 1 turns packing off (every request solo, the
+  // default). Packs trade latency for launch sharing: a pack's cycles read
+  // every member's rows, so they pay off when small requests dominate.
   void set_packing(int pack_max, int64_t pack_len);
 
 private:
@@ -61,7 +63,7 @@ private:
   int workers_alive() const;
   bool packs(const Req &r) const;
   void complete(Req &r);  // bookkeeping of a finished request (lock held)
-  int pack_max_ = 16;
+  int pack_max_ = 1;  // packing off until set_packing
   int64_t pack_len_ = int64_t(1) << 19;
 
   int device_;

@@ -234,7 +234,32 @@ def test_run_twice_bitwise_identical(precision):
         (r2.restarts, r2.cg_solves, r2.krylov_steps_total)
 
 
-@pytest.mark.parametrize("operand,precision", [("csr", "fp64"), ("csr", "mixed"),
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_result_same_as_host_result(precision):
+    """kh_cyclic_lanczos_dev leaves the basis in a device block: the same
+    bits and counters as kh_cyclic_lanczos with a host basis; a block with
+    fewer columns than found receives the first ones."""
+    A = W.regular_incidence(4, 2048, seed=9)
+    print(f"[case] device result {precision}", flush=True)
+    H = hip(A)
+    opt = K.CyclicLanczosOptions(k=4, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3)
+    r1 = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 5), opt, precision)
+    blk = K.DeviceBlock(H.ctx, A.n, 4)
+    r2 = K.find_nullspace_cyclic_lanczos_dev(H, K.HipVectorFactory(A.n, 5), opt, blk, precision)
+    assert r2.n_right == r1.right_null.shape[1] == 4
+    U = blk.to_host(r2.n_right)
+    assert np.array_equal(U.view(np.uint64), r1.right_null.view(np.uint64))
+    assert (r1.restarts, r1.cg_solves, r1.spmv) == (r2.restarts, r2.cg_solves, r2.spmv)
+    small = K.DeviceBlock(H.ctx, A.n, 2)
+    r3 = K.find_nullspace_cyclic_lanczos_dev(H, K.HipVectorFactory(A.n, 5), opt, small, precision)
+    assert r3.n_right == 4
+    assert np.array_equal(small.to_host(2).view(np.uint64), r1.right_null[:, :2].view(np.uint64))
+    with pytest.raises(ValueError):
+        K.find_nullspace_cyclic_lanczos_dev(H, K.HipVectorFactory(A.n, 5), opt,
+                                            K.DeviceBlock(H.ctx, A.n + 1, 4), precision)
+
+
+@pytest.mark.parametrize("operand,precision",[("csr", "fp64"), ("csr", "mixed"),
                                                ("stencil", "fp64"), ("stencil", "mixed")])
 def test_graph_replay_same_as_direct_launches(operand, precision):
     """Golub-Kahan cycles replayed from captured HIP graphs (graphs=1)
