@@ -52,6 +52,26 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   return t;
 }
 
+// Sum of pc[0 .. nb) in k_finish's order (per thread a stride-kBlock run
+// with eight loads in flight, then block_sum); valid in thread 0. Every
+// thread of the block must call it.
+__device__ __forceinline__ double sum_partials(const double *pc, int nb, double *lds) {
+  double acc = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * kBlock < nb; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = pc[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      acc += v[u];
+  }
+  for (; i < nb; i += kBlock)
+    acc += pc[i];
+  return block_sum(acc, lds);
+}
+
 template <class T> struct Acc;
 template <> struct Acc<double> { using type = double; };
 template <> struct Acc<float> { using type = float; };
@@ -344,22 +364,7 @@ __global__ void __launch_bounds__(kBlock)
     k_finish(const double *__restrict__ part, int nb, double *__restrict__ out,
              int do_sqrt) {
   __shared__ double lds[kWaves];
-  const double *pc = part + (int64_t)blockIdx.x * nb;
-  double acc = 0.0;
-  int i = threadIdx.x;
-  // eight loads in flight before their (in-order) sums
-  for (; i + 7 * kBlock < nb; i += 8 * kBlock) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      v[u] = pc[i + u * kBlock];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      acc += v[u];
-  }
-  for (; i < nb; i += kBlock)
-    acc += pc[i];
-  acc = block_sum(acc, lds);
+  const double acc = sum_partials(part + (int64_t)blockIdx.x * nb, nb, lds);
   if (threadIdx.x == 0)
     out[blockIdx.x] = do_sqrt ? sqrt(fmax(0.0, acc)) : acc;
 }
@@ -2006,41 +2011,45 @@ __global__ void __launch_bounds__(kBlock)
     part[(int64_t)c * nb + blockIdx.x] = acc;
 }
 
-__global__ void k_cg_alpha(const double *__restrict__ part, int nb,
-                           CgState *__restrict__ st, int *__restrict__ act,
-                           double breakdown_tol) {
-  __shared__ double lds[kWaves];
-  const int c = blockIdx.x;
-  if (!act[c])
-    return;
-  double a = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kBlock)
-    a += part[(int64_t)c * nb + i];
-  a = block_sum(a, lds);
-  if (threadIdx.x == 0) {
-    st[c].pAp = a;
-    if (!isfinite(a) || fabs(a) <= breakdown_tol) {
-      st[c].done = kDone;
-      st[c].reason = 2;
-      act[c] = 0;
-    }
-  }
-}
-
-// w += alpha p; r -= alpha Ap; z = M^{-1} r; partial r.z
+// pAp = the sum of the column's p.Ap partials (every block forms it, in
+// the order of a per-column sum over blocks); a breakdown (non-finite or
+// |pAp| <= breakdown_tol) stops the column, otherwise w += alpha p;
+// r -= alpha Ap; z = M^{-1} r and the partials of r.z (alpha = rz / pAp).
+// Block 0 records pAp and the stop.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_update(int64_t m, T *__restrict__ w, T *__restrict__ r,
                 T *__restrict__ z, const T *__restrict__ p,
                 const T *__restrict__ Ap, const double *__restrict__ d,
-                const CgState *__restrict__ st, const int *__restrict__ act,
-                double *__restrict__ part, int nb) {
+                CgState *__restrict__ st, int *__restrict__ act,
+                const double *__restrict__ pap_part, double *__restrict__ part, int nb,
+                double breakdown_tol) {
 #pragma clang fp contract(off)
   __shared__ double lds[kWaves];
+  __shared__ int on;
+  __shared__ double pap;
   const int c = blockIdx.y;
-  if (!act[c])
+  if (threadIdx.x == 0)
+    on = act[c];  // one read per block: block 0 may clear it meanwhile
+  __syncthreads();
+  if (!on)
     return;
-  const double alpha = st[c].rz / st[c].pAp;
+  const double sum = sum_partials(pap_part + (int64_t)c * nb, nb, lds);
+  if (threadIdx.x == 0)
+    pap = sum;
+  __syncthreads();
+  const bool brk = !isfinite(pap) || fabs(pap) <= breakdown_tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[c].pAp = pap;
+    if (brk) {
+      st[c].done = kDone;
+      st[c].reason = 2;
+      act[c] = 0;
+    }
+  }
+  if (brk)
+    return;
+  const double alpha = st[c].rz / pap;
   const T a = (T)alpha, na = (T)(-alpha);
   const int64_t o = (int64_t)c * m;
   double acc = 0.0;
@@ -2104,14 +2113,19 @@ __global__ void k_cg_beta(const double *__restrict__ part, int nb,
 
 // p = beta p + z (columns still running; act==2 marks a maxit stop after the
 // update of w and r, where p is no longer used).
+// (act == 2 is cleared to 0 here: the column's p is dead.)
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_cg_dir(int64_t m, T *__restrict__ p, const T *__restrict__ z,
              const CgState *__restrict__ st, int *__restrict__ act) {
 #pragma clang fp contract(off)
   const int c = blockIdx.y;
-  if (act[c] != 1)
+  const int a = act[c];
+  if (a != 1) {
+    if (a == 2 && blockIdx.x == 0 && threadIdx.x == 0)
+      act[c] = 0;
     return;
+  }
   const T b = (T)st[c].pAp;
   const int64_t o = (int64_t)c * m;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
@@ -2119,11 +2133,6 @@ __global__ void __launch_bounds__(kBlock)
     p[o + i] = p[o + i] * b + z[o + i];
 }
 
-__global__ void k_act_normalise(int s, int *__restrict__ act) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < s && act[c] != 1)
-    act[c] = 0;
-}
 
 }  // namespace
 
@@ -3144,7 +3153,10 @@ template <class T>
 static void cg_post_t(Context &ctx, CgBatch &B, T *w, const double *d, int64_t maxit,
                       double bt) {
   const int nb = reduce_blocks(B.m);
-  double *part = ctx.partials((std::size_t)nb * B.s);
+  // p.Ap partials, then r.z partials (the update reads the former while it
+  // writes the latter)
+  double *part = ctx.partials((std::size_t)nb * B.s * 2);
+  double *part_rz = part + (std::size_t)nb * B.s;
   dim3 grid(nb, B.s);
   T *r = B.r.as<T>(), *p = B.pv.as<T>(), *Ap = B.Ap.as<T>();
   T *z = d ? B.z.as<T>() : r;
@@ -3152,15 +3164,11 @@ static void cg_post_t(Context &ctx, CgBatch &B, T *w, const double *d, int64_t m
   CgState *st = B.st.as<CgState>();
   hipLaunchKernelGGL(k_cg_pap<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, p, Ap, act, part,
                      nb);
-  hipLaunchKernelGGL(k_cg_alpha, dim3(B.s), dim3(kBlock), 0, ctx.stream, part, nb, st, act,
-                     bt);
   hipLaunchKernelGGL(k_cg_update<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, w, r, z, p, Ap,
-                     d, st, act, part, nb);
-  hipLaunchKernelGGL(k_cg_beta, dim3(B.s), dim3(kBlock), 0, ctx.stream, part, nb, st, act,
+                     d, st, act, part, part_rz, nb, bt);
+  hipLaunchKernelGGL(k_cg_beta, dim3(B.s), dim3(kBlock), 0, ctx.stream, part_rz, nb, st, act,
                      maxit);
   hipLaunchKernelGGL(k_cg_dir<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, p, z, st, act);
-  hipLaunchKernelGGL(k_act_normalise, dim3((B.s + 63) / 64), dim3(64), 0, ctx.stream, B.s,
-                     act);
   KH_CHECK(hipGetLastError());
 }
 
