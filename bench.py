@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="approximate budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving-stream block of the default line")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving mode: length of the request stream (2000 for the full one)")
@@ -309,7 +311,31 @@ def main():
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }
+    if not args.no_serve:
+        del H, out  # the serving block's requests bring their own matrices
+        line["serving"] = serving_block(K)
     print(json.dumps(line))
+
+
+SERVE_BLOCK_IDS = (192, 208)
+
+
+def serving_block(K):
+    """A fixed slice of the serving stream, requests 192..207 (16 requests,
+    n up to 2.4e6, all four families): the server (4 slots, closed loop: all
+    submitted at once) against the same requests solved one at a time in
+    plain FP64, throughput and service percentiles of both."""
+    ids = list(range(*SERVE_BLOCK_IDS))
+    reqs = c5_requests(ids)
+    recs, wall, st = run_server(K, ids, reqs, 0.0, 4, 1, seed=0)
+    srv = summarize(recs, wall)
+    frecs, fwall = run_serial(K, ids, reqs, "fp64")
+    f64 = summarize(frecs, fwall)
+    return {"requests": f"serving stream ids {ids[0]}..{ids[-1]} (closed loop)",
+            "families": {f: sum(1 for r in reqs if r[0] == f)
+                         for f in ("shiftdiff", "incidence", "grid", "rg")},
+            "server": srv, "fp64_one_at_a_time": f64,
+            "server_over_fp64": round(srv["vectors_per_s"] / f64["vectors_per_s"], 3)}
 
 
 def principal_sine(U, Q):

@@ -21,7 +21,7 @@
 #include <cstdint>
 #include <thread>
 
-#include "solver.hpp"
+#include "refstream.hpp"
 
 namespace kh {
 
