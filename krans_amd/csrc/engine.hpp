@@ -151,6 +151,12 @@ struct DeviceCsr {
   // (0 = not timed yet, 1 = column by column, 2 = row-major). Both give
   // bitwise the same products.
   mutable std::atomic<signed char> block_path[2][2][2]{};
+  // Row-major block products whose gathered block is larger than the MALL
+  // run in column chunks (the gathered rows of one chunk then stay in the
+  // MALL): the chunk width, per [FP64|FP32][A|A^T], picked by timing on
+  // first use (0 = not timed yet; widths >= the block's mean no chunking).
+  // Every width gives bitwise the same products.
+  mutable std::atomic<int> rows_chunk[2][2]{};
 };
 
 // Checks a host canonical CSR (rowptr[0] = 0, monotone within [0, nnz],

@@ -1238,24 +1238,26 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // --------------------------------------------- row-major block products --
-// Y = op(A) X for a block of s columns stored row-major (element (r, c) at
-// r * s + c). Thread t handles (row t / s, column t % s): the threads of one
-// row read the row's indices and values (cache-served broadcasts) and, per
-// nonzero, one contiguous s-element row of X, so every gather is coalesced
-// however large the block. Sequential ascending sum, no contraction: the
-// same rounding as the one-lane-per-row kernels.
+// Y = op(A) X for columns c0 .. c0 + cw - 1 of a block of s columns stored
+// row-major (element (r, c) at r * s + c). Thread t handles (row t / cw,
+// column c0 + t % cw): the threads of one row read the row's indices and
+// values (cache-served broadcasts) and, per nonzero, one contiguous
+// cw-element run of a row of X, so every gather is coalesced however large
+// the block. Sequential ascending sum, no contraction: the same rounding as
+// the one-lane-per-row kernels, whatever the chunking.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_spmm_rows(int64_t rows, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
-                const T *__restrict__ val, int s, const T *__restrict__ X, T *__restrict__ Y) {
-  const int64_t total = rows * (int64_t)s;
+                const T *__restrict__ val, int s, int c0, int cw, const T *__restrict__ X,
+                T *__restrict__ Y) {
+  const int64_t total = rows * (int64_t)cw;
   for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * kBlock) {
-    const int64_t i = t / s;
-    const int c = (int)(t - i * s);
+    const int64_t i = t / cw;
+    const int c = c0 + (int)(t - i * cw);
     const T *xc = X + c;
-    Y[t] = (T)row_dot<T>(ptr[i], ptr[i + 1], idx, val,
-                         [=](int32_t j) { return xc[(int64_t)j * s]; });
+    Y[i * s + c] = (T)row_dot<T>(ptr[i], ptr[i + 1], idx, val,
+                                 [=](int32_t j) { return xc[(int64_t)j * s]; });
   }
 }
 
@@ -2772,18 +2774,58 @@ void spmm_rows(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s, cons
                 (e + 4.0) * A.nnz + 4.0 * (rows + 1) + e * s * ((trans ? A.m : A.n) + rows));
   const int32_t *ptr = trans ? A.cp.as<int32_t>() : A.rp.as<int32_t>();
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
-  const int64_t total = rows * (int64_t)s;
-  int64_t g = (total + kBlock - 1) / kBlock;
-  g = std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)1 << 30));
-  if (p == Prec::F64)
-    hipLaunchKernelGGL(k_spmm_rows<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, rows,
-                       ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(), s,
-                       (const double *)X, (double *)Y);
-  else
-    hipLaunchKernelGGL(k_spmm_rows<float>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream, rows,
-                       ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(), s,
-                       (const float *)X, (float *)Y);
-  KH_CHECK(hipGetLastError());
+  auto run = [&](int cw) {  // all columns, in chunks of cw
+    for (int c0 = 0; c0 < s; c0 += cw) {
+      const int w = std::min(cw, s - c0);
+      int64_t g = (rows * (int64_t)w + kBlock - 1) / kBlock;
+      g = std::max<int64_t>(1, std::min<int64_t>(g, (int64_t)1 << 30));
+      if (p == Prec::F64)
+        hipLaunchKernelGGL(k_spmm_rows<double>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream,
+                           rows, ptr, idx, trans ? A.vt.as<double>() : A.va.as<double>(), s, c0,
+                           w, (const double *)X, (double *)Y);
+      else
+        hipLaunchKernelGGL(k_spmm_rows<float>, dim3((unsigned)g), dim3(kBlock), 0, ctx.stream,
+                           rows, ptr, idx, trans ? A.vt32.as<float>() : A.va32.as<float>(), s,
+                           c0, w, (const float *)X, (float *)Y);
+      KH_CHECK(hipGetLastError());
+    }
+  };
+  // The gathered block (input length x s) beyond 160 MB does not stay in the
+  // 256 MB MALL: chunks of 16 or 32 columns may then be faster. The choice
+  // is timed once per precision and direction (on a block of 32 or more
+  // columns, outside stream capture) and kept on the matrix.
+  const double xbytes = e * s * (double)(trans ? A.m : A.n);
+  int cw = s;
+  if (s >= 32 && xbytes > 160e6) {
+    std::atomic<int> &slot = A.rows_chunk[p == Prec::F64 ? 0 : 1][trans ? 1 : 0];
+    int pick = slot.load(std::memory_order_relaxed);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KH_CHECK(hipStreamIsCapturing(ctx.stream, &cs));
+    if (pick == 0 && cs == hipStreamCaptureStatusNone) {
+      hipEvent_t e0, e1;
+      KH_CHECK(hipEventCreate(&e0));
+      KH_CHECK(hipEventCreate(&e1));
+      float best = 0.0f;
+      for (int cand : {1 << 30, 32, 16}) {
+        run(std::min(cand, s));  // warm
+        KH_CHECK(hipEventRecord(e0, ctx.stream));
+        run(std::min(cand, s));
+        KH_CHECK(hipEventRecord(e1, ctx.stream));
+        KH_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        KH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (pick == 0 || ms < best)
+          best = ms, pick = cand;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      slot.store(pick, std::memory_order_relaxed);
+      return;  // Y holds the product (every candidate computes it fully)
+    }
+    if (pick > 0)
+      cw = std::min(pick, s);
+  }
+  run(cw);
 }
 
 void cols_to_rows(Context &ctx, Prec p, int64_t len, int s, const void *X, int64_t ldx, void *R) {
