@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-stream block of the default line")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--solves-only", action="store_true",
+                    help="c2: run the solves only (no roofline trace, replay, CPU or "
+                         "serving block) and print their times; --steps 0 skips the mixed "
+                         "ones (for kernel-trace profiles of one precision)")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving mode: length of the request stream (2000 for the full one)")
     ap.add_argument("--serve-ids", default="", help="a:b, the slice of the stream to run")
@@ -150,10 +154,10 @@ def main():
             r = K.find_nullspace_cyclic_lanczos_dev(H, vf, opt, out, precision=precision)
             H.ctx.sync()
             dt = time.perf_counter() - t
-            if args.verbose:
-                print(f"[{precision}] solve {i}: {dt:.3f}s found={r.n_right} "
-                      f"restarts={r.restarts} cg={r.cg_solves} times={r.times}",
-                      file=sys.stderr, flush=True)
+            print(f"[{precision}] solve {i}: {dt:.3f}s found={r.n_right} "
+                  f"restarts={r.restarts} cg={r.cg_solves}"
+                  + (f" times={r.times}" if args.verbose else ""),
+                  file=sys.stderr, flush=True)
             if i >= warmup:
                 lat.append(dt)
                 found.append(r.n_right)
@@ -166,6 +170,16 @@ def main():
         return lat, found, chk, info
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else args.steps
+    if args.solves_only:
+        out_ = {}
+        for prec, st in (("mixed", args.steps), ("fp64", fp64_steps)):
+            if st > 0:
+                lat, found, chk, info = run(prec, st, args.warmup)
+                out_[prec] = {"latency_s": [round(x, 4) for x in lat], "vectors": found,
+                              "max_res_rel": chk[0], "sine_vs_exact": chk[1],
+                              "restarts": info.restarts, "phase_s": info.times}
+        print(json.dumps({"solves_only": out_, "config": "c2"}))
+        return
     copy_probe(K, H.ctx)
     lat_m, found_m, chk_m, info_m = run("mixed", args.steps, args.warmup)
     if fp64_steps > 0:
@@ -327,9 +341,17 @@ def serving_block(K):
     plain FP64, throughput and service percentiles of both."""
     ids = list(range(*SERVE_BLOCK_IDS))
     reqs = c5_requests(ids)
-    recs, wall, st = run_server(K, ids, reqs, 0.0, 4, 1, seed=0)
+
+    def log(tag):
+        def f(r):
+            print(f"[serving {tag}] id {r['id']} {r['family']} n={r['n']} k={r['k']} "
+                  f"found={r['found']} service {r['service_s']:.3f}s", file=sys.stderr,
+                  flush=True)
+        return f
+
+    recs, wall, st = run_server(K, ids, reqs, 0.0, 4, 1, seed=0, log=log("server"))
     srv = summarize(recs, wall)
-    frecs, fwall = run_serial(K, ids, reqs, "fp64")
+    frecs, fwall = run_serial(K, ids, reqs, "fp64", log=log("fp64"))
     f64 = summarize(frecs, fwall)
     return {"requests": f"serving stream ids {ids[0]}..{ids[-1]} (closed loop)",
             "families": {f: sum(1 for r in reqs if r[0] == f)
@@ -868,6 +890,8 @@ def cpu_baseline(budget_s):
         points.append({"graphs": g, "n": A.n, "k": g, "vectors": int(U.shape[1]),
                        "seconds": round(last, 3),
                        "vectors_per_s": round(U.shape[1] / last, 4)})
+        print(f"[cpu] {g} graphs: {U.shape[1]} vectors in {last:.1f}s", file=sys.stderr,
+              flush=True)
     top = points[-1]
     ext = None
     if len(points) >= 2:

@@ -79,6 +79,9 @@ typedef struct kh_cyclic_result {
   /* serving only: the number of requests in the pack whose shared launches
    * ran this request's Golub-Kahan cycles (0: solved on its own) */
   int64_t packed;
+  /* mixed precision: projector CGs whose FP32 recurrence did not converge
+   * (stalled, cg_maxit or breakdown) and were solved again in FP64 */
+  int64_t escalated;
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and

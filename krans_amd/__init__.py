@@ -57,7 +57,7 @@ class _CyclicRes(C.Structure):
         ("t_gk", C.c_double), ("t_tiny", C.c_double), ("t_cand", C.c_double),
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
         ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
-        ("refined", C.c_int64), ("packed", C.c_int64),
+        ("refined", C.c_int64), ("packed", C.c_int64), ("escalated", C.c_int64),
     ]
 
 
@@ -506,6 +506,7 @@ class CyclicLanczosResult:
     graph_captures: int = 0  # Golub-Kahan cycles captured as HIP graphs
     graph_replays: int = 0   # ... and replayed
     refined: int = 0  # mixed: locked vectors replaced by the final FP64 projector pass
+    escalated: int = 0  # mixed: FP32 projector CGs that did not converge, solved again in FP64
     n_right: int = -1  # columns written to the device block (find_nullspace_cyclic_lanczos_dev)
 
 
@@ -538,7 +539,7 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
-        refined=res.refined)
+        refined=res.refined, escalated=res.escalated)
 
 
 class DeviceBlock:
@@ -601,7 +602,7 @@ def find_nullspace_cyclic_lanczos_dev(A, vf, opt, out, precision="fp64"):
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
-        refined=res.refined, n_right=res.n_right)
+        refined=res.refined, escalated=res.escalated, n_right=res.n_right)
     return r
 
 

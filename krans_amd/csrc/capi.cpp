@@ -397,6 +397,7 @@ int cyclic_entry(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt, int pr
     res->t_rng = S.t.rng + SL.t.rng;
     res->screened = S.screened + SL.screened;
     res->refined = S.refined + SL.refined;
+    res->escalated = S.escalated + SL.escalated;
     res->graph_captures = S.graph_captures + SL.graph_captures;
     res->graph_replays = S.graph_replays + SL.graph_replays;
   });
@@ -536,6 +537,7 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->t_rng = r.stats.t.rng;
     res->screened = r.stats.screened;
     res->refined = r.stats.refined;
+    res->escalated = r.stats.escalated;
     res->packed = r.packed;
     res->graph_captures = r.stats.graph_captures;
     res->graph_replays = r.stats.graph_replays;

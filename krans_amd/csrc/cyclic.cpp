@@ -79,6 +79,11 @@ void upload_vec(Context &ctx, const double *h, int64_t n, double *d) {
 // launch cost), so the automatic setting launches directly.
 constexpr int64_t kGraphAutoLen = 0;
 
+// Stall window of the FP32 projector CG of the mixed-precision polish: a
+// column whose residual norm has not halved within this many iterations has
+// reached the FP32 rounding level and goes to FP64.
+constexpr int64_t kStall32 = 64;
+
 
 }  // namespace
 
@@ -239,8 +244,10 @@ void CyclicDriver::upload_mask(const std::vector<int> &mask) {
 }
 
 void CyclicDriver::cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
-                            int64_t maxit, double rel_tol, std::vector<CgState> &st) {
-  cg_solve_ops(c_, A_, cg_, q, rhs, w, pc, diag, maxit, rel_tol, P_.cg_breakdown_tol, st);
+                            int64_t maxit, double rel_tol, std::vector<CgState> &st,
+                            int64_t stall) {
+  cg_solve_ops(c_, A_, cg_, q, rhs, w, pc, diag, maxit, rel_tol, P_.cg_breakdown_tol, st,
+               stall);
   for (int c = 0; c < q; ++c) {
     const CgState &s = st[(std::size_t)c];
     // an iteration that stops on breakdown (reason 2) or a non-finite r.z
@@ -356,8 +363,37 @@ void CyclicDriver::polish_cg(int q, const double *rhs, double *w, Prec pc, const
   float *W32 = Wm_.as<float>();
   float *R32 = W32 + (int64_t)wcap_ * m_;
   convert(c_, Prec::F64, rhs, Prec::F32, R32, (int64_t)q * m_);
-  cg_solve(q, R32, W32, Prec::F32, diag, P_.cg_maxit, std::max(P_.cg_rel_tol, 1e-6), st);
+  cg_solve(q, R32, W32, Prec::F32, diag, P_.cg_maxit, std::max(P_.cg_rel_tol, 1e-6), st,
+           kStall32);
   convert(c_, Prec::F32, W32, Prec::F64, w, (int64_t)q * m_);
+  // A column whose FP32 CG did not converge (its residual stalled at the
+  // FP32 rounding level, reached cg_maxit or broke down) is solved again in
+  // FP64 from its FP64 right-hand side, as the FP64 polish would: when the
+  // residual of A v lies on the smallest singular values, FP32 rounding of
+  // A A^T w can keep the recurrence from ever reaching its tolerance, and w
+  // then carries error along the null space of A^T large enough to spoil
+  // A^T w. Its state (iterations, stop reason) replaces the FP32 one.
+  std::vector<int> redo;
+  for (int a = 0; a < q; ++a)
+    if (st[(std::size_t)a].reason >= 2)
+      redo.push_back(a);
+  if (redo.empty())
+    return;
+  const int qr = (int)redo.size();
+  double *Rr = AVp_.as<double>();       // gathered FP64 right-hand sides
+  double *Wr = Wm_.as<double>();        // FP64 solutions (the FP32 block is spent)
+  for (int t = 0; t < qr; ++t)
+    KH_CHECK(hipMemcpyAsync(Rr + (int64_t)t * m_, rhs + (int64_t)redo[(std::size_t)t] * m_,
+                            (std::size_t)m_ * 8, hipMemcpyDeviceToDevice, c_.stream));
+  std::vector<CgState> st64;
+  cg_solve(qr, Rr, Wr, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st64);
+  for (int t = 0; t < qr; ++t) {
+    const int a = redo[(std::size_t)t];
+    KH_CHECK(hipMemcpyAsync(w + (int64_t)a * m_, Wr + (int64_t)t * m_, (std::size_t)m_ * 8,
+                            hipMemcpyDeviceToDevice, c_.stream));
+    st[(std::size_t)a] = st64[(std::size_t)t];
+  }
+  S_.escalated += qr;
 }
 
 // Batched projector polish of the window columns `cols` (indices into the
@@ -455,8 +491,8 @@ void CyclicDriver::polish_batch(const std::vector<int> &cols, std::vector<ColOut
     }
     if (P_.verbose)
       for (int a = 0; a < qa; ++a)
-        std::fprintf(stderr, "[Cyclic-HIP] polish pass=%d cg_its=%d ||A v'||=%.3e\n", pass,
-                    st[(std::size_t)a].k, rp[(std::size_t)a]);
+        std::fprintf(stderr, "[Cyclic-HIP] polish pass=%d cg_its=%d reason=%d ||A v'||=%.3e\n",
+                     pass, st[(std::size_t)a].k, st[(std::size_t)a].reason, rp[(std::size_t)a]);
   }
   for (int i = 0; i < q; ++i)
     out[(std::size_t)cols[(std::size_t)i]].res = res[(std::size_t)i];
@@ -501,6 +537,9 @@ bool CyclicDriver::fallback_one() {
   A_.mv(c_, Prec::F64, v, tm);
   ++S_.spmv;
   double res = norm_dev(Prec::F64, tm, m_, wsc_);
+  if (P_.verbose)
+    std::fprintf(stderr, "[Cyclic-HIP] fallback: cg_its=%d rnorm=%.3e ||A v||=%.3e thr=%.3e\n",
+                 its, rnorm, res, accept_thr_);
   if (res <= accept_thr_) {
     push_basis(v);
     return true;
@@ -523,6 +562,8 @@ bool CyclicDriver::fallback_one() {
     ++S_.spmv;
     res = norm_dev(Prec::F64, tm, m_, wsc_);
   }
+  if (P_.verbose)
+    std::fprintf(stderr, "[Cyclic-HIP] fallback gradient: ||A v||=%.3e\n", res);
   if (res <= accept_thr_) {
     push_basis(v);
     return true;
@@ -1210,7 +1251,7 @@ void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, 
 
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
                   Prec p, const double *diag, int64_t maxit, double rel_tol,
-                  double breakdown_tol, std::vector<CgState> &st) {
+                  double breakdown_tol, std::vector<CgState> &st, int64_t stall) {
   const int64_t m = A.rows(), n = A.cols();
   B.ensure(q, m, n, p);
   cg_init(c, B, rhs, w, diag, rel_tol, maxit);
@@ -1246,7 +1287,7 @@ void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *
         A.mtm(c, p, q, B.pv.p, m, B.tmp_n.p, n, act);
         A.mm(c, p, q, B.tmp_n.p, n, B.Ap.p, m, act);
       }
-      cg_after_products(c, B, w, diag, maxit, breakdown_tol);
+      cg_after_products(c, B, w, diag, maxit, breakdown_tol, stall);
     }
     iters_done += it;
     chunk = std::min(chunk * 2, 16);

@@ -122,7 +122,7 @@ private:
   bool ext_ = false;
   void *vbp_ = nullptr, *xp_ = nullptr, *wxp_ = nullptr, *wyp_ = nullptr;
   void cg_solve(int q, const void *rhs, void *w, Prec pc, const double *diag,
-                int64_t maxit, double rel_tol, std::vector<CgState> &st);
+                int64_t maxit, double rel_tol, std::vector<CgState> &st, int64_t stall = 0);
   CgState cg_callback(const double *rhs, double *w, int64_t maxit, double rel_tol);
   // the projector CGs of the polish passes on q columns (rhs, w: m x q)
   void polish_cg(int q, const double *rhs, double *w, Prec pc, const double *diag,

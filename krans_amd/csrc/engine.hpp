@@ -308,6 +308,10 @@ void diag_solve(Context &ctx, Prec p, int64_t len, const double *d,
 struct CgState {  // per column, device resident
   double rz, rnorm, tol, pAp, rz_new, rhsn;
   int32_t k, done, reason, pad;
+  // stall test: the smallest residual norm so far that halved the one
+  // before it, and the iteration that reached it
+  double best;
+  int32_t kbest, pad2;
 };
 struct CgBatch {
   int s = 0;
@@ -324,8 +328,10 @@ void cg_init(Context &ctx, CgBatch &B, const void *rhs, void *w,
 // The scalar part of one CG iteration, after the caller has formed
 // B.Ap = A (A^T B.pv) for the active columns (B.act): pAp, alpha, the w/r/z
 // updates, rho, beta and the new direction. No-ops for stopped columns.
+// stall > 0: a column whose residual norm has not halved within `stall`
+// iterations stops with reason 5 (w and r updated, as at maxit).
 void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
-                       int64_t maxit, double breakdown_tol);
+                       int64_t maxit, double breakdown_tol, int64_t stall = 0);
 
 // The device scratch of kh_cgs2_dev_noalloc (noalloc.hip): noalloc_prepare
 // looks up its address on `device` (once per device, that device current);

@@ -1979,6 +1979,8 @@ __global__ void k_cg_init_finish(int s, const double *__restrict__ part_rr,
       } else {
         S.rz = rz;
         S.rnorm = sqrt(rz);
+        S.best = S.rnorm;
+        S.kbest = 0;
         S.tol = fmax(rel_tol * S.rnorm, 1e-30);
         if (S.rnorm <= S.tol) {
           S.done = kDone;
@@ -2075,7 +2077,7 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void k_cg_beta(const double *__restrict__ part, int nb,
                           CgState *__restrict__ st, int *__restrict__ act,
-                          int64_t maxit) {
+                          int64_t maxit, int64_t stall) {
   __shared__ double lds[kWaves];
   const int c = blockIdx.x;
   if (!act[c])
@@ -2100,21 +2102,28 @@ __global__ void k_cg_beta(const double *__restrict__ part, int nb,
         S.pAp = a / fmax(1e-300, S.rz);  // beta, consumed by k_cg_dir
         S.rz = a;
         S.k += 1;
+        if (S.rnorm < 0.5 * S.best) {
+          S.best = S.rnorm;
+          S.kbest = S.k;
+        }
         if (S.k >= maxit) {
           S.done = kDone;
           S.reason = 4;
+        } else if (stall > 0 && S.k - S.kbest >= stall) {
+          S.done = kDone;
+          S.reason = 5;
         }
       }
     }
     // p must still be updated for a column that stopped at maxit? No: the
     // reference leaves the loop, so p is dead; act gates k_cg_dir.
     st[c] = S;
-    act[c] = (S.done == kRun) ? 1 : (S.reason == 4 ? 2 : 0);
+    act[c] = (S.done == kRun) ? 1 : ((S.reason == 4 || S.reason == 5) ? 2 : 0);
   }
 }
 
-// p = beta p + z (columns still running; act==2 marks a maxit stop after the
-// update of w and r, where p is no longer used).
+// p = beta p + z (columns still running; act==2 marks a maxit or stall stop
+// after the update of w and r, where p is no longer used).
 // (act == 2 is cleared to 0 here: the column's p is dead.)
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -3193,7 +3202,7 @@ void cg_init(Context &ctx, CgBatch &B, const void *rhs, void *w, const double *d
 
 template <class T>
 static void cg_post_t(Context &ctx, CgBatch &B, T *w, const double *d, int64_t maxit,
-                      double bt) {
+                      double bt, int64_t stall) {
   const int nb = reduce_blocks(B.m);
   // p.Ap partials, then r.z partials (the update reads the former while it
   // writes the latter)
@@ -3209,17 +3218,17 @@ static void cg_post_t(Context &ctx, CgBatch &B, T *w, const double *d, int64_t m
   hipLaunchKernelGGL(k_cg_update<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, w, r, z, p, Ap,
                      d, st, act, part, part_rz, nb, bt);
   hipLaunchKernelGGL(k_cg_beta, dim3(B.s), dim3(kBlock), 0, ctx.stream, part_rz, nb, st, act,
-                     maxit);
+                     maxit, stall);
   hipLaunchKernelGGL(k_cg_dir<T>, grid, dim3(kBlock), 0, ctx.stream, B.m, p, z, st, act);
   KH_CHECK(hipGetLastError());
 }
 
 void cg_after_products(Context &ctx, CgBatch &B, void *w, const double *diag,
-                       int64_t maxit, double breakdown_tol) {
+                       int64_t maxit, double breakdown_tol, int64_t stall) {
   if (B.p == Prec::F64)
-    cg_post_t(ctx, B, (double *)w, diag, maxit, breakdown_tol);
+    cg_post_t(ctx, B, (double *)w, diag, maxit, breakdown_tol, stall);
   else
-    cg_post_t(ctx, B, (float *)w, diag, maxit, breakdown_tol);
+    cg_post_t(ctx, B, (float *)w, diag, maxit, breakdown_tol, stall);
 }
 
 // ------------------------------------------------- packed Golub-Kahan steps --

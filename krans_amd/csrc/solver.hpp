@@ -140,6 +140,7 @@ struct CyclicStats {
   int64_t cg_iters = 0;           // CG iterations over all columns
   int64_t screened = 0;           // candidates rejected by the FP32 prescreen (mixed)
   int64_t refined = 0;            // locked vectors replaced by the final FP64 pass (mixed)
+  int64_t escalated = 0;          // FP32 projector CGs solved again in FP64 (mixed)
   int64_t graph_captures = 0, graph_replays = 0;  // Golub-Kahan cycles as HIP graphs
   PhaseTimes t;
 };
@@ -157,9 +158,11 @@ struct CyclicStats {
 // kind runs and times both (bitwise the same results). Otherwise mm / mtm.
 void block_product(Context &c, const DeviceOp &A, Prec p, int q, const void *X, int64_t ldx,
                    void *Y, int64_t ldy, bool trans, const int *mask, DBuf &rin, DBuf &rout);
+// Batched projector CG A A^T w = rhs on q columns; stall > 0 stops a column
+// whose residual norm has not halved within that many iterations (reason 5).
 void cg_solve_ops(Context &c, const DeviceOp &A, CgBatch &B, int q, const void *rhs, void *w,
                   Prec p, const double *diag, int64_t maxit, double rel_tol,
-                  double breakdown_tol, std::vector<CgState> &st);
+                  double breakdown_tol, std::vector<CgState> &st, int64_t stall = 0);
 
 // A device block of `count` FP64 vectors of length `len` (column-major).
 struct DeviceBasis {

@@ -301,3 +301,61 @@ def test_c3b_weighted_grids_match_oracle(precision):
         assert sine <= max(1e-10, 2.0 * sine_o)
     else:
         assert sine <= 1e-10
+
+
+# ---- unweighted grid incidence: the FP32 projector CG's stall exit ---------
+
+def grid_kernel(grids, side):
+    """The component indicators of `grids` side x side grids, orthonormal."""
+    vpg = side * side
+    Q = np.zeros((grids * vpg, grids))
+    for g in range(grids):
+        Q[g * vpg:(g + 1) * vpg, g] = 1.0 / side
+    return Q
+
+
+def grid_opts(k):
+    return K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-4,
+                                  max_restarts=400, cg_maxit=1000)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_grid_incidence_small_finds_the_kernel(precision):
+    """6 unweighted 46 x 46 grids (n = 12,696): both precisions find the 6
+    indicators with max ||Au|| <= thr = 1e-12 ||A||_F. The smallest nonzero
+    singular value is sigma = 2 sin(pi / 92) ~ 0.068, so the acceptance test
+    alone bounds the principal-angle sine by thr / sigma ~ 3e-9: the FP64 path
+    (the reference's algorithm) is held to that bound, the mixed path, whose
+    locked vectors end with a batched FP64 projector pass to 0.01 thr, to
+    1e-10. Before the FP32 projector CG had its stall exit, every mixed
+    polish here ran to cg_maxit on candidates whose FP64 residual it then made
+    worse."""
+    print(f"[case] grid incidence 6 x 46^2 {precision}", flush=True)
+    A = W.grid_incidence(6, 46, seed=1202, weighted=False)
+    r = K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(A.n, 1202), grid_opts(6),
+                                           precision)
+    U = r.right_null
+    thr = 1e-12 * A.fro()
+    assert U.shape[1] == 6
+    assert residuals(A, U).max() <= thr
+    sine = principal_sine(U, grid_kernel(6, 46))
+    assert sine <= (1e-10 if precision == "mixed" else thr / (2 * np.sin(np.pi / 92)))
+    assert r.cg_maxit_hits == 0
+
+
+def test_grid_incidence_mixed_escalates_stalled_fp32_polish():
+    """5 unweighted 233 x 233 grids (n = 271,445; sigma_min ~ 0.013): the
+    candidates' residuals lie on the smallest singular values, the FP32
+    projector CG stalls at its rounding level and those columns are solved
+    again in FP64 (escalated > 0); all 5 indicators are found to the
+    threshold with sine <= 1e-10. (The FP64 path finds 1 of 5 within
+    stagnation_restarts here: its candidates' CGs stop at cg_maxit.)"""
+    print("[case] grid incidence 5 x 233^2 mixed", flush=True)
+    A = W.grid_incidence(5, 233, seed=1197, weighted=False)
+    r = K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(A.n, 1197), grid_opts(5),
+                                           "mixed")
+    U = r.right_null
+    assert r.escalated > 0
+    assert U.shape[1] == 5 and r.converged_to_target
+    assert residuals(A, U).max() <= 1e-12 * A.fro()
+    assert principal_sine(U, grid_kernel(5, 233)) <= 1e-10
