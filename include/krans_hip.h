@@ -349,6 +349,15 @@ int kh_trace_class(kh_context *ctx, int kernel_class, int64_t *launches, double 
  * HIP events on the library stream. */
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps,
                  double *avg_ms);
+/* The gather kernel of y = op(A) x for a CSR matrix in `precision`: *path
+ * receives 0 (not chosen yet: the first product times the variants), 1
+ * (gathers through L2), 2 or 3 (x staged in LDS per block of rows, first or
+ * second block size). With set != 0 the kernel is pinned to `*path` first
+ * (0 times again at the next product; a window the operator has no table
+ * for, or one larger than a workgroup's LDS in that precision, is refused).
+ * Every kernel gives bitwise the same y. Tuning control of this build; the
+ * reference has no counterpart. */
+int kh_spmv_kernel(kh_matrix *A, int transpose, int precision, int set, int *path);
 /* Launch shapes of one traced class (call before kh_trace_end): the distinct
  * (len, s) pairs its launches recorded (rows and columns of the V^T w
  * products of class 1; classes that record no shape report none), with the

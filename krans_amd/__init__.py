@@ -146,6 +146,7 @@ SIGNATURES = {
     "kh_cgs2_dev_ex": (I32, [P, I32, P, I32, P, I64, I32, P, P]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
     "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),
+    "kh_spmv_kernel": (I32, [P, I32, I32, I32, C.POINTER(I32)]),
     "kh_trace_begin": (I32, [P, I32]),
     "kh_trace_end": (I32, [P, C.POINTER(I64), C.POINTER(D), C.POINTER(D)]),
     "kh_trace_begin_mask": (I32, [P, C.c_uint]),
@@ -359,6 +360,15 @@ class HipCsrMatrixAdapter(HipMatrix):
         out = D()
         _check(lib().kh_time_spmv(self.h, int(transpose), PREC[precision], reps, C.byref(out)))
         return out.value
+
+    def spmv_kernel(self, transpose=False, precision="fp64", pin=None):
+        """The gather kernel of y = op(A) x (0 not chosen yet, 1 gathers
+        through L2, 2/3 x staged in LDS per block of rows); pin= sets it
+        first (0: time again at the next product)."""
+        v = I32(0 if pin is None else int(pin))
+        _check(lib().kh_spmv_kernel(self.h, int(transpose), PREC[precision],
+                                    0 if pin is None else 1, C.byref(v)))
+        return v.value
 
 
 for _name in ("multiply", "multiply_transpose", "time_spmv"):

@@ -910,6 +910,26 @@ int kh_time_vtw(kh_context *ctx, int precision, int64_t len, int s, int reps, do
   });
 }
 
+int kh_spmv_kernel(kh_matrix *A, int transpose, int precision, int set, int *path) {
+  return guard([&] {
+    need(A && path, "null argument");
+    const kh::DeviceCsr &M = A->csr("kh_spmv_kernel");
+    const kh::Prec p = prec_of(precision);
+    if (set) {
+      const int v = *path;
+      need(v >= 0 && v <= 3, "path must be 0..3");
+      const int d = transpose ? 1 : 0;
+      need(v < 2 || M.span_rb[d][v - 2] > 0, "no window table for that block size");
+      if (v >= 2 && (int64_t)M.span_len[d][v - 2] * (p == kh::Prec::F64 ? 8 : 4) >
+                        A->owner->ctx->lds_max)
+        throw std::invalid_argument("that window exceeds the LDS of one workgroup (" +
+                                    std::to_string(A->owner->ctx->lds_max) + " bytes)");
+      kh::set_spmv_path(M, transpose != 0, p, v);
+    }
+    *path = kh::spmv_path(M, transpose != 0, p);
+  });
+}
+
 int kh_time_spmv(kh_matrix *A, int transpose, int precision, int reps, double *avg_ms) {
   return guard([&] {
     need(A && avg_ms && reps >= 1, "bad argument");

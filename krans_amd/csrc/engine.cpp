@@ -33,6 +33,7 @@ Context::Context(int dev, std::size_t part_doubles) : device(dev) {
   hipDeviceProp_t prop;
   KH_CHECK(hipGetDeviceProperties(&prop, dev));
   cu_count = prop.multiProcessorCount;
+  lds_max = (int)prop.sharedMemPerBlock;
 }
 
 hipEvent_t Context::take_event() {
@@ -190,6 +191,50 @@ template <class T> static DBuf upload(Context &ctx, const T *h, std::size_t coun
   return d;
 }
 
+// Window tables of the LDS-staged gathers for the rows of one direction
+// (ptr/idx: CSR rows of A or of the CSC twin) when every row holds R entries
+// (R in 1..4 or 8) and there are at least 65,536 rows: the largest block of
+// 512 x 2^j rows (at most 32 / R rows per thread, 8 at most) that still
+// gives 256 blocks, and half of it; a block size is kept when its largest
+// window (from the lowest entry index rounded down to a multiple of 4 to
+// the highest, rounded up to a multiple of 4) holds at most 40,960 elements
+// (160 KB of FP32).
+static void build_spans(Context &ctx, DeviceCsr &A, int d, int64_t rows, int R,
+                        const int32_t *ptr, const int32_t *idx) {
+  if (!(R >= 1 && R <= 4) && R != 8)
+    return;
+  if (rows < 65536)
+    return;
+  int rows_pt = std::min(8, 32 / R);
+  int rb = rows_pt * 512;
+  while (rb > 512 && (rows + rb - 1) / rb < 256)
+    rb /= 2;
+  for (int k = 0; k < 2 && rb >= 512; ++k, rb /= 2) {
+    const int64_t nb = (rows + rb - 1) / rb;
+    std::vector<int32_t> st((std::size_t)nb);
+    int64_t len = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      const int64_t i0 = b * rb, i1 = std::min<int64_t>(rows, i0 + rb);
+      int32_t lo = INT32_MAX, hi = -1;
+      for (int64_t p = ptr[i0]; p < ptr[i1]; ++p) {
+        lo = std::min(lo, idx[p]);
+        hi = std::max(hi, idx[p]);
+      }
+      if (hi < 0)
+        lo = hi = 0;
+      const int32_t s0 = lo & ~3;
+      st[(std::size_t)b] = s0;
+      len = std::max<int64_t>(len, ((int64_t)hi + 1 - s0 + 3) & ~(int64_t)3);
+    }
+    if (len > 40960)
+      continue;
+    A.span_rb[d][k] = rb;
+    A.span_len[d][k] = (int)len;
+    A.span[d][k] = upload(ctx, st.data(), st.size());
+    ctx.sync();  // st dies with this iteration
+  }
+}
+
 int validate_csr(int64_t m, int64_t n, const int32_t *rowptr, const int32_t *col,
                  const double *val) {
   if (m < 0 || n < 0)
@@ -280,6 +325,8 @@ DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n, const int32_t *rowptr,
     A->perm = upload(ctx, perm.data(), (std::size_t)nnz);
     A->diag_AAt = upload(ctx, dAAt.data(), (std::size_t)m);
     A->diag_AAt_host = std::move(dAAt);
+    build_spans(ctx, *A, 0, m, A->uniform_rows, rowptr, col);
+    build_spans(ctx, *A, 1, n, A->uniform_cols, cp.data(), ri.data());
     ctx.sync();  // host staging vectors die at scope exit
   } catch (...) {
     delete A;

@@ -75,6 +75,7 @@ struct Context {
   std::size_t part_ext_cap = 0;  // doubles
   std::size_t part_capacity() const { return part_ext ? part_ext_cap : part_cap; }
   int cu_count = 0;
+  int lds_max = 0;  // bytes of LDS one workgroup may allocate
   static constexpr int kScalars = 1 << 16;
   // part_doubles: capacity of the reduction partials (default 16 Mi doubles)
   explicit Context(int dev, std::size_t part_doubles = std::size_t(1) << 24);
@@ -157,6 +158,19 @@ struct DeviceCsr {
   // first use (0 = not timed yet; widths >= the block's mean no chunking).
   // Every width gives bitwise the same products.
   mutable std::atomic<int> rows_chunk[2][2]{};
+  // LDS-staged gathers (operators whose rows all hold the same 1-4 or 8
+  // entries), per [A|A^T] and two block sizes k: rows in blocks of
+  // span_rb[d][k] rows, block b reading x only within [span[d][k][b],
+  // span[d][k][b] + span_len[d][k]) (starts multiples of 4). span_rb 0: no
+  // table (irregular rows, fewer than 65,536 rows, or windows above 160 KB of
+  // FP32).
+  int span_rb[2][2] = {};
+  int span_len[2][2] = {};
+  DBuf span[2][2];
+  // y = op(A) x per [FP64|FP32][A|A^T]: 0 not timed yet, 1 gathers from L2,
+  // 2 + k the LDS window of block size k; the fastest when first timed (all
+  // give bitwise the same y).
+  mutable std::atomic<signed char> spmv_path[2][2]{};
 };
 
 // Checks a host canonical CSR (rowptr[0] = 0, monotone within [0, nnz],
@@ -180,6 +194,10 @@ DeviceCsr *csr_create(Context &ctx, int64_t m, int64_t n,
 // With z != nullptr: y = A x - (*beta) z (beta on device).
 void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
           void *y, const void *z = nullptr, const double *beta = nullptr);
+// The kernel spmv picked for op(A) in precision p: 0 not timed yet, 1 gathers
+// from L2, 2 / 3 the LDS window of the first / second block size.
+int spmv_path(const DeviceCsr &A, bool trans, Prec p);
+void set_spmv_path(const DeviceCsr &A, bool trans, Prec p, int path);
 // Same operator on `s` columns at once (x: len_in x s, y: len_out x s).
 // Columns whose active flag is 0 are skipped (active may be nullptr).
 void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,

@@ -264,6 +264,85 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// LDS-staged gathers. Rows go in blocks of ROWS * kSpanThreads rows; the
+// entries of block b reference x only within the window [s_b, s_b + len)
+// (s_b a multiple of 4, from the operator's span table, len the largest
+// window of the table), so the block copies its window of x into LDS and
+// gathers from LDS: one L2 request per 128 bytes of window instead of one
+// per entry whose cache line the gather misses in L1. A thread first issues
+// the loads of its rows' entries, then its FW 16-byte loads of the window
+// (all in flight together), stores them to LDS, and after the barrier
+// gathers from LDS. The row arithmetic is k_spmv_ell's: y is bitwise the
+// same.
+constexpr int kSpanThreads = 512;
+
+template <class T, int R, int ROWS, int FW>
+__global__ void __launch_bounds__(kSpanThreads)
+    k_spmv_ell_span(int64_t rows, const int32_t *__restrict__ span, int len, int64_t lenx,
+                    const int32_t *__restrict__ idx, const T *__restrict__ val,
+                    const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ z,
+                    const double *__restrict__ beta) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char span_lds[];
+  T *xs = reinterpret_cast<T *>(span_lds);
+  const int64_t b = xcd_block();
+  const int64_t r0 = b * (int64_t)(ROWS * kSpanThreads) + threadIdx.x;
+  int32_t j[ROWS][R];
+  T v[ROWS][R], zi[ROWS];
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int64_t r = r0 + (int64_t)k * kSpanThreads;
+    // rows past the end take the block's last row (inside its window); their
+    // results are not stored
+    const int64_t i = r < rows ? r : rows - 1;
+    ell_entries<T, R>(i, idx, val, j[k], v[k]);
+    zi[k] = z ? z[i] : (T)0;
+  }
+  const T bt = z ? (T)(*beta) : (T)0;
+  const int64_t s0 = span[b];
+  const int64_t L = min((int64_t)len, lenx - s0);
+  using V16 = typename std::conditional<std::is_same<T, float>::value, float4, double2>::type;
+  constexpr int E = 16 / (int)sizeof(T);
+  const int64_t nv = L / E;  // 0 only when x ends less than 16 bytes past s_b
+  const V16 *xw = reinterpret_cast<const V16 *>(x + s0);
+  V16 *xs16 = reinterpret_cast<V16 *>(xs);
+  if (nv > 0) {
+    V16 t[FW];
+#pragma unroll
+    for (int u = 0; u < FW; ++u) {  // indices past the window repeat its last load
+      const int64_t e = threadIdx.x + (int64_t)u * kSpanThreads;
+      t[u] = xw[e < nv ? e : nv - 1];
+    }
+    // unconditional stores (past the window a thread rewrites the last
+    // vector with the value already there): a store under a condition would
+    // draw its load into the condition, one load in flight at a time
+#pragma unroll
+    for (int u = 0; u < FW; ++u) {
+      const int64_t e = threadIdx.x + (int64_t)u * kSpanThreads;
+      xs16[e < nv ? e : nv - 1] = t[u];
+    }
+  }
+  for (int64_t e = nv * E + threadIdx.x; e < L; e += kSpanThreads)
+    xs[e] = x[s0 + e];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k) {
+    const int64_t i = r0 + (int64_t)k * kSpanThreads;
+    T xv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      xv[r] = xs[j[k][r] - s0];
+    T out = (T)0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      out += v[k][r] * xv[r];
+    if (z)
+      out = out + (-bt) * zi[k];
+    if (i < rows)
+      y[i] = out;
+  }
+}
+
 // G lanes per row (G in 4,16,64): strided partial sums, shuffle tree.
 template <class T, int G>
 __global__ void __launch_bounds__(kBlock)
@@ -2245,6 +2324,140 @@ static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
   KH_CHECK(hipGetLastError());
 }
 
+#define KH_SPAN_LAUNCH(R, ROWS, FW)                                                          \
+  hipLaunchKernelGGL((k_spmv_ell_span<T, R, ROWS, FW>), dim3((unsigned)nb),                   \
+                     dim3(kSpanThreads), lds, ctx.stream, rows, span, len, lenx, idx, val, x, y, \
+                     z, beta)
+#define KH_SPAN_FW(R, ROWS)                                                                  \
+  switch (fw) {                                                                              \
+  case 8: KH_SPAN_LAUNCH(R, ROWS, 8); break;                                                 \
+  case 16: KH_SPAN_LAUNCH(R, ROWS, 16); break;                                               \
+  default: KH_SPAN_LAUNCH(R, ROWS, 20); break;                                               \
+  }
+#define KH_SPAN_ROWS(R)                                                                      \
+  switch (rows_pt) {                                                                         \
+  case 1: KH_SPAN_FW(R, 1) break;                                                            \
+  case 2: KH_SPAN_FW(R, 2) break;                                                            \
+  case 4: KH_SPAN_FW(R, 4) break;                                                            \
+  default: KH_SPAN_FW(R, 8) break;                                                           \
+  }
+
+// k_spmv_ell_span over `rows` rows of R entries in blocks of rb rows
+// (rb / kSpanThreads rows per thread, at most 32 / R; the window copied in
+// 8, 16 or 20 16-byte loads per thread).
+template <class T>
+static void launch_spmv_span(Context &ctx, int64_t rows, int R, int rb, const int32_t *span,
+                             int len, int64_t lenx, const int32_t *idx, const T *val,
+                             const T *x, T *y, const T *z, const double *beta) {
+  const int rows_pt = rb / kSpanThreads;
+  const int64_t nb = (rows + rb - 1) / rb;
+  const std::size_t lds = (std::size_t)len * sizeof(T);
+  const int64_t nvec = (int64_t)lds / 16;
+  const int fw = nvec <= 8 * kSpanThreads ? 8 : nvec <= 16 * kSpanThreads ? 16 : 20;
+  if (rows_pt * kSpanThreads != rb || rows_pt > 8 || rows_pt * R > 32 || nb > INT32_MAX ||
+      len < 4 || len % 4 != 0 || nvec > 20 * kSpanThreads)
+    throw std::logic_error("spmv: span block size or window out of range");
+  switch (R) {
+  case 1: KH_SPAN_ROWS(1) break;
+  case 2: KH_SPAN_ROWS(2) break;
+  case 3: KH_SPAN_ROWS(3) break;
+  case 4: KH_SPAN_ROWS(4) break;
+  default:
+    switch (rows_pt) {
+    case 1: KH_SPAN_FW(8, 1) break;
+    case 2: KH_SPAN_FW(8, 2) break;
+    default: KH_SPAN_FW(8, 4) break;
+    }
+  }
+  KH_CHECK(hipGetLastError());
+}
+#undef KH_SPAN_ROWS
+#undef KH_SPAN_FW
+#undef KH_SPAN_LAUNCH
+
+template <class T>
+static void spmv_dispatch(Context &ctx, const DeviceCsr &A, bool trans, int64_t rows,
+                          const int32_t *ptr, const int32_t *idx, const T *val, int lpr,
+                          int uni, const T *x, T *y, const T *z, const double *beta) {
+  const int ell = ell_width(uni, idx, val);
+  const int di = trans ? 1 : 0, pi = std::is_same<T, double>::value ? 0 : 1;
+  const int64_t lenx = trans ? A.m : A.n;
+  // window k usable for this call: a table, its window within the LDS a
+  // workgroup may hold, x aligned for the 16-byte window copy
+  auto usable = [&](int k) {
+    return ell > 0 && A.span_rb[di][k] > 0 &&
+           (int64_t)A.span_len[di][k] * (int64_t)sizeof(T) <= ctx.lds_max &&
+           reinterpret_cast<uintptr_t>(x) % 16 == 0;
+  };
+  auto run = [&](int path) {
+    if (path >= 2)
+      launch_spmv_span<T>(ctx, rows, ell, A.span_rb[di][path - 2], A.span[di][path - 2].as<int32_t>(),
+                          A.span_len[di][path - 2], lenx, idx, val, x, y, z, beta);
+    else
+      launch_spmv<T>(ctx, rows, ptr, idx, val, lpr, ell, x, y, z, beta);
+  };
+  int path = A.spmv_path[pi][di].load(std::memory_order_relaxed);
+  if (path == 0) {
+    bool table = false;  // a window this operator and precision can ever use
+    for (int k = 0; k < 2; ++k)
+      table = table || (ell > 0 && A.span_rb[di][k] > 0 &&
+                        (int64_t)A.span_len[di][k] * (int64_t)sizeof(T) <= ctx.lds_max);
+    if (!table) {
+      A.spmv_path[pi][di].store(1, std::memory_order_relaxed);
+      run(1);
+      return;
+    }
+    // first use: time the plain gather and each usable window (three
+    // launches each after one warm-up) when y is written from inputs it does
+    // not alias, x is aligned and the stream is not being captured; every
+    // variant writes the same y. Otherwise this call gathers from L2 and
+    // the choice waits for a later call.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KH_CHECK(hipStreamIsCapturing(ctx.stream, &cs));
+    const bool alias = (const void *)y == (const void *)x || (const void *)y == (const void *)z;
+    if (cs != hipStreamCaptureStatusNone || alias || reinterpret_cast<uintptr_t>(x) % 16 != 0) {
+      run(1);
+      return;
+    }
+    hipEvent_t e0, e1;
+    KH_CHECK(hipEventCreate(&e0));
+    KH_CHECK(hipEventCreate(&e1));
+    float best = 0.0f;
+    int pick = 1;
+    for (int cand : {1, 2, 3}) {
+      if (cand >= 2 && !usable(cand - 2))
+        continue;
+      run(cand);
+      KH_CHECK(hipEventRecord(e0, ctx.stream));
+      for (int r = 0; r < 3; ++r)
+        run(cand);
+      KH_CHECK(hipEventRecord(e1, ctx.stream));
+      KH_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      KH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (cand == 1 || ms < best) {
+        best = ms;
+        pick = cand;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A.spmv_path[pi][di].store((signed char)pick, std::memory_order_relaxed);
+    return;  // y holds the (identical) product of the last run
+  }
+  // a window path picked for aligned x falls back to the gathers otherwise
+  run(path >= 2 && usable(path - 2) ? path : 1);
+}
+
+int spmv_path(const DeviceCsr &A, bool trans, Prec p) {
+  return A.spmv_path[p == Prec::F64 ? 0 : 1][trans ? 1 : 0].load(std::memory_order_relaxed);
+}
+
+void set_spmv_path(const DeviceCsr &A, bool trans, Prec p, int path) {
+  A.spmv_path[p == Prec::F64 ? 0 : 1][trans ? 1 : 0].store((signed char)path,
+                                                          std::memory_order_relaxed);
+}
+
 void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
           void *y, const void *z, const double *beta) {
   const int64_t rows = trans ? A.n : A.m;
@@ -2259,14 +2472,16 @@ void spmv(Context &ctx, const DeviceCsr &A, bool trans, Prec p, const void *x,
   TraceScope tr(ctx, kTraceSpmv,
                 (e + 4.0) * A.nnz + (fixed ? 0.0 : 4.0 * (rows + 1)) + e * (trans ? A.m : A.n) +
                     e * rows * (z ? 2.0 : 1.0));
+  if (rows <= 0)
+    return;
   if (p == Prec::F64) {
     const double *val = trans ? A.vt.as<double>() : A.va.as<double>();
-    launch_spmv<double>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val),
-                        (const double *)x, (double *)y, (const double *)z, beta);
+    spmv_dispatch<double>(ctx, A, trans, rows, ptr, idx, val, lpr, uni, (const double *)x,
+                          (double *)y, (const double *)z, beta);
   } else {
     const float *val = trans ? A.vt32.as<float>() : A.va32.as<float>();
-    launch_spmv<float>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val),
-                       (const float *)x, (float *)y, (const float *)z, beta);
+    spmv_dispatch<float>(ctx, A, trans, rows, ptr, idx, val, lpr, uni, (const float *)x,
+                         (float *)y, (const float *)z, beta);
   }
 }
 

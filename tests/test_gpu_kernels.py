@@ -288,3 +288,70 @@ def test_trace_shapes_and_vtw_timer(ctx):
     assert 0.0 < ctx.time_vtw(n, s, "fp32", reps=5) < 100.0
     with pytest.raises(ValueError):
         ctx.time_vtw(n, 513, "fp32")
+
+
+@pytest.mark.parametrize("transpose", [0, 1])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_spmv_lds_window_bitwise(transpose, precision):
+    """y = A x and y = A^T x of the incidence of 8 random 4-regular graphs x
+    16,384 vertices (262,144 rows of 2 entries; 131,072 columns of 4): the
+    LDS-window kernels (x staged per block of rows; pinned one at a time)
+    give bitwise the gathers' y, which is the sequential sum; the first
+    product after unpinning times the kernels and keeps one. Windows are one
+    graph's x (64 KB FP32 / 128 KB FP64 for A, 128 KB FP32 for A^T); FP64
+    A^T x needs 256 KB and has no usable window."""
+    from krans_amd import workloads as W
+    print(f"[case] spmv lds window t={transpose} {precision}", flush=True)
+    A = W.regular_incidence(8, 16384, seed=5)
+    H = K.HipCsrMatrixAdapter.from_csr(A)
+    dt = np.float64 if precision == "fp64" else np.float32
+    x = np.random.default_rng(3).standard_normal(A.m if transpose else A.n).astype(dt)
+    mul = H.multiply_transpose if transpose else H.multiply
+    assert H.spmv_kernel(transpose, precision, pin=1) == 1
+    y1 = mul(x.astype(np.float64), precision).astype(dt)
+    ref = _sequential_rows(A, x[:, None], dt, transpose)[:, 0]
+    assert np.array_equal(y1, ref)
+    ran = 0
+    for path in (2, 3):
+        try:
+            H.spmv_kernel(transpose, precision, pin=path)
+        except ValueError:  # no table, or a window beyond one workgroup's LDS
+            continue
+        yp = mul(x.astype(np.float64), precision).astype(dt)
+        assert np.array_equal(yp, y1), path
+        ran += 1
+    assert ran >= (0 if (transpose and precision == "fp64") else 1)
+    H.spmv_kernel(transpose, precision, pin=0)
+    assert np.array_equal(mul(x.astype(np.float64), precision).astype(dt), y1)
+    assert H.spmv_kernel(transpose, precision) in (1, 2, 3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_same_bits_with_every_gather_kernel(precision):
+    """A cyclic solve (y = A x - beta z steps, CG products) on the 8-graph
+    incidence returns bitwise the same basis whichever gather kernel its
+    products take."""
+    from krans_amd import workloads as W
+    print(f"[case] solve with pinned gather kernels {precision}", flush=True)
+    A = W.regular_incidence(8, 16384, seed=5)
+    opt = K.CyclicLanczosOptions(k=8, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-4,
+                                 max_restarts=400)
+    bases = []
+    for path in (1, 2, 3):
+        H = K.HipCsrMatrixAdapter.from_csr(A)
+        pinned = 0
+        for t in (0, 1):
+            for p in ("fp64", "fp32"):
+                try:
+                    H.spmv_kernel(t, p, pin=path)
+                    pinned += 1
+                except ValueError:
+                    H.spmv_kernel(t, p, pin=1)
+        if path > 1 and pinned == 0:
+            continue
+        r = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(A.n, 9), opt, precision)
+        assert r.right_null.shape[1] == 8
+        bases.append(r.right_null)
+    assert len(bases) >= 2
+    for U in bases[1:]:
+        assert np.array_equal(U.view(np.uint64), bases[0].view(np.uint64))
