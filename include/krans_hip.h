@@ -47,7 +47,11 @@ typedef struct kh_rng_s kh_rng;
  * Golub-Kahan cycle from a HIP graph captured once per basis size when the
  * operator only enqueues kernels (CSR, shifted difference; not callbacks),
  * -1 launches every kernel directly, 0 (default) decides by problem size.
- * Results are bitwise the same either way. */
+ * Results are bitwise the same either way. `screen` (mixed precision with
+ * prescreen_factor < 0.1, also an engine choice) picks how candidates are
+ * screened before their FP64 test: 0 (default) from the singular values of
+ * the projected problem when use_harmonic is set and target_tau is zero,
+ * else as 1; 1 from FP32 candidate vectors and residuals; -1 not at all. */
 typedef struct kh_cyclic_options {
   int64_t k, ncv, max_restarts;
   double tol_abs, tol_rel;
@@ -62,6 +66,7 @@ typedef struct kh_cyclic_options {
   int64_t stagnation_restarts;
   int32_t return_left_space, verbose;
   int32_t precond_jacobi, graphs;
+  int32_t screen;
 } kh_cyclic_options;
 
 /* krans::solvers::CyclicLanczosResult (CyclicLanczosSolver.hpp:149-167) plus

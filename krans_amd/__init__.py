@@ -43,7 +43,7 @@ class _CyclicOpts(C.Structure):
         ("prescreen_factor", C.c_double),
         ("stagnation_restarts", C.c_int64),
         ("return_left_space", C.c_int32), ("verbose", C.c_int32),
-        ("precond_jacobi", C.c_int32), ("graphs", C.c_int32),
+        ("precond_jacobi", C.c_int32), ("graphs", C.c_int32), ("screen", C.c_int32),
     ]
 
 
@@ -483,6 +483,10 @@ class CyclicLanczosOptions:
     # captured HIP graphs where the operator allows it: 1 always, 0 by problem
     # size, -1 never
     graphs: int = 0
+    # engine choice (mixed precision, prescreen_factor < 0.1): candidates
+    # screened from the projected problem's singular values (0, when the
+    # target is zero), from FP32 candidate vectors (1), or not at all (-1)
+    screen: int = 0
 
     def _c(self):
         o = _CyclicOpts()

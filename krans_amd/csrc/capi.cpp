@@ -84,6 +84,8 @@ kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
   P.verbose = o.verbose != 0;
   P.jacobi_AAt = o.precond_jacobi != 0;
   P.graphs = o.graphs;
+  need(o.screen >= -1 && o.screen <= 1, "screen must be -1, 0 or 1");
+  P.screen = o.screen;
   P.krylov = prec_of(precision);
   P.polish = P.krylov;
   need(P.k >= 0 && P.ncv >= 0 && P.max_restarts >= 0 && P.cg_maxit >= 0,

@@ -607,7 +607,7 @@ int CyclicDriver::prepare() {
   prescreen_thr_ = std::max(0.0, P_.prescreen_factor) * std::max(1.0, S_.frob_norm_est);
   // unit vectors have ||A v|| <= ||A||_2 <= ||A||_F: a screen below a tenth
   // of the norm can reject candidates
-  screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1;
+  screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1 && P_.screen >= 0;
   graphs_ = !ext_ && A_.capturable() && c_.trace_mask == 0 &&
             (P_.graphs > 0 || (P_.graphs == 0 && m_ + n_ <= kGraphAutoLen));
 
@@ -867,11 +867,30 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
   // exactly as without the screen. Screening switches off for the solve
   // when it keeps failing to reject candidates (thresholds above the
   // spectrum).
+  //
+  // With refined extraction and a zero target (the defaults) the screen
+  // needs no vectors at all: T-bar then has a zero diagonal, the candidates
+  // are the singular vectors s of its even-column block, u = Z s = [0; Y c]
+  // and C u = Z_{l+1} T-bar s = [A Y c; 0], so ||A v|| / ||v|| for v = Y c
+  // is the singular value itself (X and Y orthonormal; in FP32 to a few
+  // units of 6e-8 ||A||_2 per step, far inside the margin). The candidate
+  // vector of the best screened candidate is formed only if it is still the
+  // best at the end of the restart.
   const int ncl = (int)std::min<std::size_t>(cl.size(), 64);
   std::vector<char> rej32((std::size_t)ncl, 0);
   std::vector<double> nv32((std::size_t)ncl, 0.0), r32((std::size_t)ncl, 0.0);
   float *W32 = nullptr;
-  if (screen_ && ncl > 0) {
+  const bool projected = screen_ && P_.screen == 0 && P_.use_harmonic && P_.target_tau == 0.0;
+  if (projected && ncl > 0) {
+    const double margin = 1e-5 * std::max(1.0, 2.0 * smax);
+    for (int c = 0; c < ncl; ++c) {
+      const double r = ev[(std::size_t)cl[(std::size_t)c]];
+      nv32[(std::size_t)c] = 1.0;
+      r32[(std::size_t)c] = r;
+      if (std::isfinite(r) && r > prescreen_thr_ + margin)
+        rej32[(std::size_t)c] = 1;
+    }
+  } else if (screen_ && ncl > 0) {
     ensure_window(ncl);
     W32 = Wv32_.as<float>();
     float *AV32 = AV32_.as<float>();
@@ -898,6 +917,9 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
         rej32[(std::size_t)c] = 1;
       else
         ++kept;
+      if (P_.verbose)
+        std::fprintf(stderr, "[Cyclic-HIP] screen cand=%d sigma=%.9e fp32 ||Aw||/||w||=%.9e ||w||=%.6e%s\n",
+                     c, ev[(std::size_t)cl[(std::size_t)c]], r, nv, rej32[(std::size_t)c] ? "" : " kept");
     }
     if (ncl >= 4) {
       screen_miss_ = kept * 4 > ncl * 3 ? screen_miss_ + 1 : 0;
@@ -907,6 +929,7 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
   }
   double best_res = std::numeric_limits<double>::infinity();
   bool have_best_v = false, accepted_cycle = false, polish_stalled = false;
+  int best_proj = -1;  // projected screen: candidate (index into cl) still to form as best_v
   int64_t tested = 0;
   std::size_t pos = 0;
   bool single_next = true;
@@ -1003,7 +1026,10 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
       if (o.res_raw > prescreen_thr_ || polish_stalled) {
         if (o.res_raw < best_res) {
           best_res = o.res_raw;
-          if (screened[(std::size_t)c]) {  // the unnormalised FP32 screen vector
+          best_proj = -1;
+          if (screened[(std::size_t)c] && projected) {
+            best_proj = (int)(pos + (std::size_t)c);
+          } else if (screened[(std::size_t)c]) {  // the unnormalised FP32 screen vector
             const double sc = 1.0 / o.nv;
             convert(c_, Prec::F32, W32 + (int64_t)(pos + (std::size_t)c) * n_, Prec::F64,
                     best_v_.p, n_);
@@ -1039,6 +1065,7 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
       polish_stalled = polish_stalled || o.hit;
       if (o.res < best_res) {
         best_res = o.res;
+        best_proj = -1;
         KH_CHECK(hipMemcpyAsync(best_v_.p, vfin, (std::size_t)n_ * 8,
                                 hipMemcpyDeviceToDevice, c_.stream));
         have_best_v = true;
@@ -1048,6 +1075,12 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
       break;
     pos += consumed;
     single_next = cut;
+  }
+  if (best_proj >= 0 && !done_all) {  // v = Y c of the best screened candidate, FP64
+    std::vector<double> Cb((std::size_t)ny_used, 0.0);
+    coeffs(cl[(std::size_t)best_proj], Cb.data());
+    upload_doubles(c_, Cb.data(), ny_used, C_);
+    gemm_nn(c_, kp, Ycol(ny_used - 1), ldy_, ny_used, C_, 1, Prec::F64, best_v_.p, n_, n_);
   }
   S_.t.cand += since(t0);
   if (done_all)

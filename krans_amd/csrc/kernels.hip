@@ -209,14 +209,15 @@ __device__ __forceinline__ void ell_entries(int64_t i, const int32_t *__restrict
   }
 }
 
+// act (optional): one column's flag; the launch writes nothing when it is 0.
 template <class T, int R>
 __global__ void __launch_bounds__(kBlock)
     k_spmv_ell(int64_t rows, const int32_t *__restrict__ idx, const T *__restrict__ val,
                const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ z,
-               const double *__restrict__ beta) {
+               const double *__restrict__ beta, const int *__restrict__ act) {
 #pragma clang fp contract(off)
   const int64_t i = xcd_block() * kBlock + threadIdx.x;
-  if (i >= rows)
+  if (i >= rows || (act && !*act))
     return;
   const T b = z ? (T)(*beta) : (T)0;
   const T zi = z ? z[i] : (T)0;
@@ -281,8 +282,10 @@ __global__ void __launch_bounds__(kSpanThreads)
     k_spmv_ell_span(int64_t rows, const int32_t *__restrict__ span, int len, int64_t lenx,
                     const int32_t *__restrict__ idx, const T *__restrict__ val,
                     const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ z,
-                    const double *__restrict__ beta) {
+                    const double *__restrict__ beta, const int *__restrict__ act) {
 #pragma clang fp contract(off)
+  if (act && !*act)  // one column's flag: the launch writes nothing when it is 0
+    return;
   extern __shared__ __align__(16) unsigned char span_lds[];
   T *xs = reinterpret_cast<T *>(span_lds);
   const int64_t b = xcd_block();
@@ -2285,20 +2288,21 @@ static int ell_width(int uniform, const int32_t *idx, const void *val) {
 #define KH_ELL_LAUNCH(KERNEL, R, STREAM, ...)                                                \
   hipLaunchKernelGGL((KERNEL<T, R>), dim3(grid_rows(rows)), dim3(kBlock), 0, STREAM, __VA_ARGS__)
 
+// act (fixed-length rows only): one column's device flag, nothing written when 0
 template <class T>
 static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
                         const int32_t *idx, const T *val, int lpr, int ell, const T *x,
-                        T *y, const T *z, const double *beta) {
+                        T *y, const T *z, const double *beta, const int *act = nullptr) {
   hipStream_t s = ctx.stream;
   if (rows <= 0)
     return;
   if (ell > 0) {
     switch (ell) {
-    case 1: KH_ELL_LAUNCH(k_spmv_ell, 1, s, rows, idx, val, x, y, z, beta); break;
-    case 2: KH_ELL_LAUNCH(k_spmv_ell, 2, s, rows, idx, val, x, y, z, beta); break;
-    case 3: KH_ELL_LAUNCH(k_spmv_ell, 3, s, rows, idx, val, x, y, z, beta); break;
-    case 4: KH_ELL_LAUNCH(k_spmv_ell, 4, s, rows, idx, val, x, y, z, beta); break;
-    default: KH_ELL_LAUNCH(k_spmv_ell, 8, s, rows, idx, val, x, y, z, beta); break;
+    case 1: KH_ELL_LAUNCH(k_spmv_ell, 1, s, rows, idx, val, x, y, z, beta, act); break;
+    case 2: KH_ELL_LAUNCH(k_spmv_ell, 2, s, rows, idx, val, x, y, z, beta, act); break;
+    case 3: KH_ELL_LAUNCH(k_spmv_ell, 3, s, rows, idx, val, x, y, z, beta, act); break;
+    case 4: KH_ELL_LAUNCH(k_spmv_ell, 4, s, rows, idx, val, x, y, z, beta, act); break;
+    default: KH_ELL_LAUNCH(k_spmv_ell, 8, s, rows, idx, val, x, y, z, beta, act); break;
     }
     KH_CHECK(hipGetLastError());
     return;
@@ -2327,7 +2331,7 @@ static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
 #define KH_SPAN_LAUNCH(R, ROWS, FW)                                                          \
   hipLaunchKernelGGL((k_spmv_ell_span<T, R, ROWS, FW>), dim3((unsigned)nb),                   \
                      dim3(kSpanThreads), lds, ctx.stream, rows, span, len, lenx, idx, val, x, y, \
-                     z, beta)
+                     z, beta, act)
 #define KH_SPAN_FW(R, ROWS)                                                                  \
   switch (fw) {                                                                              \
   case 8: KH_SPAN_LAUNCH(R, ROWS, 8); break;                                                 \
@@ -2348,7 +2352,8 @@ static void launch_spmv(Context &ctx, int64_t rows, const int32_t *ptr,
 template <class T>
 static void launch_spmv_span(Context &ctx, int64_t rows, int R, int rb, const int32_t *span,
                              int len, int64_t lenx, const int32_t *idx, const T *val,
-                             const T *x, T *y, const T *z, const double *beta) {
+                             const T *x, T *y, const T *z, const double *beta,
+                             const int *act) {
   const int rows_pt = rb / kSpanThreads;
   const int64_t nb = (rows + rb - 1) / rb;
   const std::size_t lds = (std::size_t)len * sizeof(T);
@@ -2378,7 +2383,8 @@ static void launch_spmv_span(Context &ctx, int64_t rows, int R, int rb, const in
 template <class T>
 static void spmv_dispatch(Context &ctx, const DeviceCsr &A, bool trans, int64_t rows,
                           const int32_t *ptr, const int32_t *idx, const T *val, int lpr,
-                          int uni, const T *x, T *y, const T *z, const double *beta) {
+                          int uni, const T *x, T *y, const T *z, const double *beta,
+                          const int *act = nullptr) {
   const int ell = ell_width(uni, idx, val);
   const int di = trans ? 1 : 0, pi = std::is_same<T, double>::value ? 0 : 1;
   const int64_t lenx = trans ? A.m : A.n;
@@ -2392,9 +2398,9 @@ static void spmv_dispatch(Context &ctx, const DeviceCsr &A, bool trans, int64_t 
   auto run = [&](int path) {
     if (path >= 2)
       launch_spmv_span<T>(ctx, rows, ell, A.span_rb[di][path - 2], A.span[di][path - 2].as<int32_t>(),
-                          A.span_len[di][path - 2], lenx, idx, val, x, y, z, beta);
+                          A.span_len[di][path - 2], lenx, idx, val, x, y, z, beta, act);
     else
-      launch_spmv<T>(ctx, rows, ptr, idx, val, lpr, ell, x, y, z, beta);
+      launch_spmv<T>(ctx, rows, ptr, idx, val, lpr, ell, x, y, z, beta, act);
   };
   int path = A.spmv_path[pi][di].load(std::memory_order_relaxed);
   if (path == 0) {
@@ -2409,13 +2415,15 @@ static void spmv_dispatch(Context &ctx, const DeviceCsr &A, bool trans, int64_t 
     }
     // first use: time the plain gather and each usable window (three
     // launches each after one warm-up) when y is written from inputs it does
-    // not alias, x is aligned and the stream is not being captured; every
-    // variant writes the same y. Otherwise this call gathers from L2 and
-    // the choice waits for a later call.
+    // not alias, x is aligned, no column flag may turn the launches into
+    // no-ops and the stream is not being captured; every variant writes the
+    // same y. Otherwise this call gathers from L2 and the choice waits for a
+    // later call.
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     KH_CHECK(hipStreamIsCapturing(ctx.stream, &cs));
     const bool alias = (const void *)y == (const void *)x || (const void *)y == (const void *)z;
-    if (cs != hipStreamCaptureStatusNone || alias || reinterpret_cast<uintptr_t>(x) % 16 != 0) {
+    if (cs != hipStreamCaptureStatusNone || alias || act ||
+        reinterpret_cast<uintptr_t>(x) % 16 != 0) {
       run(1);
       return;
     }
@@ -2526,7 +2534,21 @@ void spmm(Context &ctx, const DeviceCsr &A, bool trans, Prec p, int s,
   const int32_t *idx = trans ? A.ri.as<int32_t>() : A.ci.as<int32_t>();
   const int lpr = trans ? A.lpr_cols : A.lpr_rows;
   const int uni = trans ? A.uniform_cols : A.uniform_rows;
-  if (p == Prec::F64) {
+  // one column of an operator with fixed-length rows: the single-vector
+  // kernels (bitwise the same sums; they include the LDS-window gathers),
+  // which leave y untouched when the column's `active` flag is 0
+  const bool one = s == 1 && rows > 0 &&
+                   ell_width(uni, idx, p == Prec::F64 ? (trans ? A.vt.p : A.va.p)
+                                                      : (trans ? A.vt32.p : A.va32.p)) > 0;
+  if (one && p == Prec::F64) {
+    const double *val = trans ? A.vt.as<double>() : A.va.as<double>();
+    spmv_dispatch<double>(ctx, A, trans, rows, ptr, idx, val, lpr, uni, (const double *)x,
+                          (double *)y, nullptr, nullptr, active);
+  } else if (one) {
+    const float *val = trans ? A.vt32.as<float>() : A.va32.as<float>();
+    spmv_dispatch<float>(ctx, A, trans, rows, ptr, idx, val, lpr, uni, (const float *)x,
+                         (float *)y, nullptr, nullptr, active);
+  } else if (p == Prec::F64) {
     const double *val = trans ? A.vt.as<double>() : A.va.as<double>();
     launch_spmm<double>(ctx, rows, ptr, idx, val, lpr, ell_width(uni, idx, val), s,
                         (const double *)x, ldx, (double *)y, ldy, active);

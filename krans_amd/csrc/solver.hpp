@@ -125,6 +125,11 @@ struct CyclicParams {
   // that only enqueue kernels and with tracing off: 1 always, 0 automatic
   // (vectors short enough for the cycle to be launch-bound), -1 never
   int graphs = 0;
+  // mixed precision, prescreen_factor < 0.1: how candidates are screened
+  // before their FP64 evaluation. 0: from the projected problem when
+  // refined extraction has a zero target (no vector work), else 1; 1: FP32
+  // candidate vectors and their residuals; -1: no screen
+  int screen = 0;
 };
 
 struct PhaseTimes {

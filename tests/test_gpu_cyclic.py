@@ -310,3 +310,28 @@ def test_callback_operator_is_not_captured():
     r = K.find_nullspace_cyclic_lanczos_ex(F, K.HipVectorFactory(A.n, 3), opt, "mixed")
     assert r.graph_captures == 0 and r.graph_replays == 0
     assert r.right_null.shape == (A.n, 2)
+
+
+@pytest.mark.parametrize("screen", [0, 1])
+def test_mixed_candidate_screens(screen):
+    """Mixed precision on the incidence of 8 random 4-regular graphs x 16,384
+    vertices, prescreen 1e-4 (threshold below sigma_min): candidates are
+    screened from the projected problem's singular values (screen 0: with a
+    zero target ||A Y c|| equals the even-block singular value of T-bar) or
+    from FP32 candidate vectors (screen 1). Both find the 8 indicators to
+    the threshold with sine <= 1e-10, screen most candidates, and agree on
+    how many within a few (their restart vectors differ in rounding)."""
+    g, v = 8, 16384
+    A = W.regular_incidence(g, v, seed=42)
+    print(f"[case] mixed screen {screen}", flush=True)
+    opts = {s: K.CyclicLanczosOptions(k=g, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-4,
+                                      max_restarts=400, screen=s) for s in (0, 1)}
+    runs = {s: K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(A.n, 42), o,
+                                                  "mixed") for s, o in opts.items()}
+    r = runs[screen]
+    U = r.right_null
+    assert U.shape[1] == g and r.converged_to_target
+    assert residuals(A, U).max() <= 1e-12 * A.fro()
+    assert principal_sine(U, W.component_indicators(g, v)) <= 1e-10
+    assert r.screened >= 4 * r.restarts
+    assert abs(runs[0].screened - runs[1].screened) <= max(8, runs[1].screened // 20)
