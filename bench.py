@@ -144,14 +144,15 @@ def main():
     out = K.DeviceBlock(H.ctx, A.n, k)
     d2h_s = []
 
-    def run(precision, steps, warmup):
+    def run(precision, steps, warmup, o=None):
+        o = opt if o is None else o
         lat, found, chk, info = [], [], None, None
         seed = 42
         for i in range(warmup + steps):
             vf = K.HipVectorFactory(A.n, seed)
             H.ctx.sync()
             t = time.perf_counter()
-            r = K.find_nullspace_cyclic_lanczos_dev(H, vf, opt, out, precision=precision)
+            r = K.find_nullspace_cyclic_lanczos_dev(H, vf, o, out, precision=precision)
             H.ctx.sync()
             dt = time.perf_counter() - t
             print(f"[{precision}] solve {i}: {dt:.3f}s found={r.n_right} "
@@ -190,6 +191,23 @@ def main():
     total_m = sum(lat_m)
     value = sum(found_m) / total_m
     fp64_value = sum(found_d) / sum(lat_d) if lat_d else None
+    # the same mixed solve with the reference's unconditional second
+    # Gram-Schmidt pass (second_pass = 1), one timed solve: splits the ratio
+    # into its precision part and its selective-pass part
+    split = None
+    if fp64_value and args.steps > 0:
+        import dataclasses
+        opt2 = dataclasses.replace(opt, second_pass=1)
+        lat_2, found_2, chk_2, info_2 = run("mixed", 1, 0, opt2)
+        v2 = sum(found_2) / sum(lat_2)
+        split = {"mixed_both_passes": {"value": round(v2, 4), "latency_s": round(lat_2[0], 4),
+                                       "restarts": info_2.restarts,
+                                       "max_res_rel": chk_2[0], "sine_vs_exact": chk_2[1]},
+                 "precision_part": round(v2 / fp64_value, 3),
+                 "selective_pass_part": round(value / v2, 3),
+                 "note": "mixed_over_fp64 = precision_part x selective_pass_part; the "
+                         "selective second pass (DGKS test) is part of the mixed method, FP64 "
+                         "keeps the reference's two passes"}
 
     def pct(xs, q):
         xs = sorted(xs)
@@ -286,12 +304,15 @@ def main():
             "latency_s": {"p50": round(pct(lat_d, 0.5), 4), "p99": round(pct(lat_d, 0.99), 4)},
             "max_res_rel": chk_d[0], "sine_vs_exact": chk_d[1],
             "restarts": info_d.restarts, "cg_solves": info_d.cg_solves,
+            "cgs_pairs": info_d.cgs_pairs, "second_passes": info_d.second_passes,
         },
         "mixed_over_fp64": round(value / fp64_value, 3) if fp64_value else None,
+        "ratio_split": split,
         "accuracy": {"max_res_rel": chk_m[0], "sine_vs_exact": chk_m[1],
                      "vectors": found_m[0], "restarts": info_m.restarts,
                      "cg_solves": info_m.cg_solves, "phase_s": info_m.times,
                      "screened": info_m.screened,
+                     "cgs_pairs": info_m.cgs_pairs, "second_passes": info_m.second_passes,
                      "hip_graphs": {"captures": info_m.graph_captures,
                                     "replays": info_m.graph_replays}},
         "roofline": {

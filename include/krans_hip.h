@@ -51,7 +51,13 @@ typedef struct kh_rng_s kh_rng;
  * prescreen_factor < 0.1, also an engine choice) picks how candidates are
  * screened before their FP64 test: 0 (default) from the singular values of
  * the projected problem when use_harmonic is set and target_tau is zero,
- * else as 1; 1 from FP32 candidate vectors and residuals; -1 not at all. */
+ * else as 1; 1 from FP32 candidate vectors and residuals; -1 not at all.
+ * `second_pass` (reorth = 2, engine choice) sets the second pass of each
+ * Golub-Kahan step's Gram-Schmidt pair: 1 always (the reference's double
+ * reorthogonalisation); 2 only where the first pass cancelled more than
+ * half of ||w||^2 (the Daniel-Gragg-Kaufman-Stewart test with
+ * eta = 1/sqrt(2), as in ARPACK, decided on the device); 0 (default) 2 for
+ * mixed precision and 1 for FP64. */
 typedef struct kh_cyclic_options {
   int64_t k, ncv, max_restarts;
   double tol_abs, tol_rel;
@@ -66,7 +72,7 @@ typedef struct kh_cyclic_options {
   int64_t stagnation_restarts;
   int32_t return_left_space, verbose;
   int32_t precond_jacobi, graphs;
-  int32_t screen;
+  int32_t screen, second_pass;
 } kh_cyclic_options;
 
 /* krans::solvers::CyclicLanczosResult (CyclicLanczosSolver.hpp:149-167) plus
@@ -87,6 +93,9 @@ typedef struct kh_cyclic_result {
   /* mixed precision: projector CGs whose FP32 recurrence did not converge
    * (stalled, cg_maxit or breakdown) and were solved again in FP64 */
   int64_t escalated;
+  /* Gram-Schmidt pairs of the Golub-Kahan steps and how many of them ran
+   * their second pass (all of them unless the second pass is selective) */
+  int64_t cgs_pairs, second_passes;
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and
@@ -315,6 +324,15 @@ int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s,
  * use) or as four separate sweeps (fused == 0); the results are identical. */
 int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w,
                    int64_t len, int fused, double *h_out, double *norm_out);
+/* The selective pair of the mixed-precision Golub-Kahan steps (1 <= s < 512):
+ * the first pass always, the second only where the first cancelled more than
+ * half of ||w||^2 (Daniel-Gragg-Kaufman-Stewart, eta = 1/sqrt(2)), decided on
+ * the device. h_out gets the coefficients removed (h1, plus h2 when the
+ * second pass ran), norm_out ||w|| afterwards, second_out 1 when the second
+ * pass ran and 0 otherwise. With the second pass the results are bitwise
+ * those of kh_cgs2_dev. */
+int kh_cgs2_dev_select(kh_context *ctx, int precision, const void *V, int s, void *w,
+                       int64_t len, double *h_out, double *norm_out, int32_t *second_out);
 /* kh_cgs2_dev without allocating. No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). The same kernels as

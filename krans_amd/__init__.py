@@ -44,6 +44,7 @@ class _CyclicOpts(C.Structure):
         ("stagnation_restarts", C.c_int64),
         ("return_left_space", C.c_int32), ("verbose", C.c_int32),
         ("precond_jacobi", C.c_int32), ("graphs", C.c_int32), ("screen", C.c_int32),
+        ("second_pass", C.c_int32),
     ]
 
 
@@ -58,6 +59,7 @@ class _CyclicRes(C.Structure):
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
         ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
         ("refined", C.c_int64), ("packed", C.c_int64), ("escalated", C.c_int64),
+        ("cgs_pairs", C.c_int64), ("second_passes", C.c_int64),
     ]
 
 
@@ -144,6 +146,7 @@ SIGNATURES = {
     "kh_cgs2_dev": (I32, [P, I32, P, I32, P, I64, P, C.POINTER(D)]),
     "kh_cgs2_dev_noalloc": (I32, [P, I32, P, I32, P, I64, P, P]),
     "kh_cgs2_dev_ex": (I32, [P, I32, P, I32, P, I64, I32, P, P]),
+    "kh_cgs2_dev_select": (I32, [P, I32, P, I32, P, I64, P, P, P]),
     "kh_gram_dev": (I32, [P, I32, P, P, I32, I64, P, P]),
     "kh_time_spmv": (I32, [P, I32, I32, I32, C.POINTER(D)]),
     "kh_spmv_kernel": (I32, [P, I32, I32, I32, C.POINTER(I32)]),
@@ -487,6 +490,11 @@ class CyclicLanczosOptions:
     # screened from the projected problem's singular values (0, when the
     # target is zero), from FP32 candidate vectors (1), or not at all (-1)
     screen: int = 0
+    # engine choice (reorth 2): second Gram-Schmidt pass of each Golub-Kahan
+    # step always (1), only where the first pass cancelled more than half of
+    # ||w||^2 (2, the DGKS test decided on the device), or automatic (0: 2 for
+    # mixed precision, 1 for FP64)
+    second_pass: int = 0
 
     def _c(self):
         o = _CyclicOpts()
@@ -521,6 +529,8 @@ class CyclicLanczosResult:
     graph_replays: int = 0   # ... and replayed
     refined: int = 0  # mixed: locked vectors replaced by the final FP64 projector pass
     escalated: int = 0  # mixed: FP32 projector CGs that did not converge, solved again in FP64
+    cgs_pairs: int = 0  # Gram-Schmidt pairs of the Golub-Kahan steps
+    second_passes: int = 0  # ... that ran their second pass
     n_right: int = -1  # columns written to the device block (find_nullspace_cyclic_lanczos_dev)
 
 
@@ -553,7 +563,8 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
-        refined=res.refined, escalated=res.escalated)
+        refined=res.refined, escalated=res.escalated, cgs_pairs=res.cgs_pairs,
+        second_passes=res.second_passes)
 
 
 class DeviceBlock:
@@ -616,7 +627,8 @@ def find_nullspace_cyclic_lanczos_dev(A, vf, opt, out, precision="fp64"):
         cg_solves=res.cg_solves, cg_maxit_hits=res.cg_maxit_hits, spmv=res.spmv,
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
-        refined=res.refined, escalated=res.escalated, n_right=res.n_right)
+        refined=res.refined, escalated=res.escalated, cgs_pairs=res.cgs_pairs,
+        second_passes=res.second_passes, n_right=res.n_right)
     return r
 
 

@@ -86,6 +86,8 @@ kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
   P.graphs = o.graphs;
   need(o.screen >= -1 && o.screen <= 1, "screen must be -1, 0 or 1");
   P.screen = o.screen;
+  need(o.second_pass >= 0 && o.second_pass <= 2, "second_pass must be 0, 1 or 2");
+  P.second_pass = o.second_pass;
   P.krylov = prec_of(precision);
   P.polish = P.krylov;
   need(P.k >= 0 && P.ncv >= 0 && P.max_restarts >= 0 && P.cg_maxit >= 0,
@@ -402,6 +404,8 @@ int cyclic_entry(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt, int pr
     res->escalated = S.escalated + SL.escalated;
     res->graph_captures = S.graph_captures + SL.graph_captures;
     res->graph_replays = S.graph_replays + SL.graph_replays;
+    res->cgs_pairs = S.cgs_pairs + SL.cgs_pairs;
+    res->second_passes = S.second_passes + SL.second_passes;
   });
 }
 }  // namespace
@@ -543,6 +547,8 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->packed = r.packed;
     res->graph_captures = r.stats.graph_captures;
     res->graph_replays = r.stats.graph_replays;
+    res->cgs_pairs = r.stats.cgs_pairs;
+    res->second_passes = r.stats.second_passes;
     if (latency_s)
       *latency_s = r.latency;
     if (queued_s)
@@ -739,6 +745,32 @@ int kh_cgs2_dev_ex(kh_context *ctx, int precision, const void *V, int s, void *w
   });
 }
 
+int kh_cgs2_dev_select(kh_context *ctx, int precision, const void *V, int s, void *w,
+                       int64_t len, double *h_out, double *norm_out, int32_t *second_out) {
+  return guard([&] {
+    need(ctx && V && w && s >= 1 && s < 512 && len >= 1, "bad argument (1 <= s < 512)");
+    auto &c = *ctx->ctx;
+    const kh::Prec p = prec_of(precision);
+    double *g = c.dscal + 16, *h2 = g + 4096, *nrm = c.dscal + 8, *ran = c.dscal + 9;
+    kh::gemv_t(c, p, V, len, s, p, w, len, g, /*wsq=*/true);  // h1 and ||w||^2
+    kh::gemv_n_sub_t(c, p, V, len, s, g, p, w, len, h2, nullptr, g);
+    kh::gemv_n_sub(c, p, V, len, s, h2, p, w, len, nrm, g, ran);
+    std::vector<double> a((std::size_t)s), b((std::size_t)s);
+    double tail[2] = {0, 0};
+    KH_CHECK(hipMemcpyAsync(a.data(), g, (std::size_t)s * 8, hipMemcpyDeviceToHost, c.stream));
+    KH_CHECK(hipMemcpyAsync(b.data(), h2, (std::size_t)s * 8, hipMemcpyDeviceToHost, c.stream));
+    KH_CHECK(hipMemcpyAsync(tail, nrm, 16, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (h_out)
+      for (int j = 0; j < s; ++j)
+        h_out[j] = a[(std::size_t)j] + b[(std::size_t)j];
+    if (norm_out)
+      *norm_out = tail[0];
+    if (second_out)
+      *second_out = tail[1] > 0.5 ? 1 : 0;
+  });
+}
+
 int kh_cgs2_dev(kh_context *ctx, int precision, const void *V, int s, void *w, int64_t len,
                 double *h_out, double *norm_out) {
   return kh_cgs2_dev_ex(ctx, precision, V, s, w, len, s <= 1024 ? 1 : 0, h_out, norm_out);
@@ -863,13 +895,17 @@ int kh_trace_end(kh_context *ctx, int64_t *launches, double *bytes, double *ms) 
     auto &c = *ctx->ctx;
     c.sync();
     double tb = 0.0, tm = 0.0;
+    int64_t nl = 0;
     for (auto &r : c.trace) {
+      if (r.cls == kh::kTraceNone)  // a launch the device skipped (no bytes moved)
+        continue;
       float e = 0.0f;
       KH_CHECK(hipEventElapsedTime(&e, r.a, r.b));
       tb += r.bytes;
       tm += (double)e;
+      ++nl;
     }
-    *launches = (int64_t)c.trace.size();
+    *launches = nl;
     *bytes = tb;
     *ms = tm;
     c.trace_mask = 0;

@@ -112,12 +112,13 @@ void CyclicDriver::alloc() {
   tmp_m_ = DBuf((std::size_t)m_ * 8);
   tmp_n_ = DBuf((std::size_t)n_ * 8);
   const std::size_t hsz = (std::size_t)(k_target_ + ycap_ + xcap_ + 8);
-  // layout: bnorm[ncv+8] | h[4*hsz] | wsc[4096] | C[ycap*(ncv+2)] | H[...]
-  const std::size_t n_b = (std::size_t)ncv + 8, n_h = 4 * hsz, n_w = 4096,
+  // layout: bnorm[ncv+8] | pass2[ncv+8] | h[4*hsz] | wsc[4096] | C[ycap*(ncv+2)] | H[...]
+  const std::size_t n_b = 2 * ((std::size_t)ncv + 8), n_h = 4 * hsz, n_w = 4096,
                     n_c = (std::size_t)ycap_ * (std::size_t)(ncv + 2);
   const std::size_t n_H = (std::size_t)std::max<int64_t>(1, k_target_) * 512;
   small_ = DBuf((n_b + n_h + n_w + n_c + n_H) * 8);
   bnorm_ = small_.as<double>();
+  pass2_ = bnorm_ + ncv + 8;
   h_ = bnorm_ + n_b;
   wsc_ = h_ + n_h;
   C_ = wsc_ + n_w;
@@ -610,13 +611,15 @@ int CyclicDriver::prepare() {
   screen_ = kp_ == Prec::F32 && P_.prescreen_factor < 0.1 && P_.screen >= 0;
   graphs_ = !ext_ && A_.capturable() && c_.trace_mask == 0 &&
             (P_.graphs > 0 || (P_.graphs == 0 && m_ + n_ <= kGraphAutoLen));
+  select2_ = P_.reorth == 2 &&
+             (P_.second_pass == 2 || (P_.second_pass == 0 && kp_ == Prec::F32));
 
   double *y0 = y0_.as<double>();
   if (P_.y0_host)
     upload_vec(c_, P_.y0_host, n_, y0);
   else
     random_vec(y0);
-  bh_.assign((std::size_t)ncv_, 0.0);
+  bh_.assign(2 * ((std::size_t)ncv_ + 8), 0.0);
   restart_ = 0;
   restarts_since_accept_ = 0;
   return 0;
@@ -677,6 +680,8 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
   // shapes and buffers, with beta_j read on the device. Where the operator
   // only enqueues kernels it is captured once per basis size and replayed.
   auto build = [&]() {
+    stepkind_.assign((std::size_t)ncv, 0);
+    midrec_.assign((std::size_t)ncv, -1);
     for (int64_t j = 0; j < ncv; ++j) {
       const bool from_y = nyc > nxc;
       void *w = from_y ? wxp_ : wyp_;
@@ -701,16 +706,27 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
         // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
         // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
         // except that h2's partial sums are grouped per block of the fused
-        // kernel.
-        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
-        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h_ + g0.count);
+        // kernel. Selective (select2_): h1 comes with ||w||^2 (h_[s]), and
+        // where the first pass cancelled less than half of ||w||^2 the
+        // middle sweep does nothing and the last one applies w -= V h1 (one
+        // pass, two reads of V), decided on the device; pass2_[j] records it.
+        const bool sel = select2_ && g0.count < 512;
+        const double *g = sel ? h_ : nullptr;
+        double *h2 = h_ + g0.count + 1;
+        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_, sel);
+        const std::size_t nrec = c_.trace.size();
+        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h2, nullptr, g);
+        if (sel && c_.trace.size() > nrec)
+          midrec_[(std::size_t)j] = (int64_t)nrec;
         if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
-          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len,
-                               bnorm_ + j, from_y ? Xcol(nxc) : Ycol(nyc));
+          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h2, kp, w, len, bnorm_ + j,
+                               from_y ? Xcol(nxc) : Ycol(nyc), g, sel ? pass2_ + j : nullptr);
           stored = true;
         } else {
-          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_ + g0.count, kp, w, len, bnorm_ + j);
+          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h2, kp, w, len, bnorm_ + j, g,
+                     sel ? pass2_ + j : nullptr);
         }
+        stepkind_[(std::size_t)j] = sel ? 2 : 1;
       } else {
         for (int pass = 0; pass < passes; ++pass)
           cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
@@ -762,7 +778,23 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
   char *VBb = static_cast<char *>(vbp_);
   const int64_t ncv = ncv_;
   double *y0 = y0_.as<double>();
-  readback(c_, bnorm_, (int)queued, bh_.data());
+  // beta_j (and, after selective CGS pairs, whether their second pass ran)
+  bool any_sel = false;
+  for (int64_t j = 0; j < queued && j < (int64_t)stepkind_.size(); ++j)
+    any_sel = any_sel || stepkind_[(std::size_t)j] == 2;
+  readback(c_, bnorm_, (int)(any_sel ? ncv + 8 + queued : queued), bh_.data());
+  for (int64_t j = 0; j < queued && j < (int64_t)stepkind_.size(); ++j) {
+    const char kind = stepkind_[(std::size_t)j];
+    if (kind == 0)
+      continue;
+    ++S_.cgs_pairs;
+    if (kind == 1 || bh_[(std::size_t)(ncv + 8 + j)] > 0.5) {
+      ++S_.second_passes;
+    } else if (j < (int64_t)midrec_.size() && midrec_[(std::size_t)j] >= 0 &&
+               midrec_[(std::size_t)j] < (int64_t)c_.trace.size()) {
+      c_.trace[(std::size_t)midrec_[(std::size_t)j]].cls = kTraceNone;
+    }
+  }
   double beta_last = 0.0;
   int64_t local_steps = 0;
   for (int64_t j = 0; j < queued; ++j) {

@@ -90,6 +90,15 @@ private:
   CgBatch cg_;
   CapturedWork gk_graph_;  // the queued Golub-Kahan cycle, keyed by the basis size
   bool graphs_ = false;
+  bool select2_ = false;  // selective second CGS pass in the Golub-Kahan steps
+  double *pass2_ = nullptr;  // per step: 1 when its second pass ran (selective)
+  // per queued step of the current cycle: 0 no CGS pair, 1 pair with both
+  // passes, 2 selective pair (outcome in pass2_)
+  std::vector<char> stepkind_;
+  // with kernel tracing: per step, the trace record of a selective pair's
+  // middle sweep (-1 none); a sweep the device skipped is taken out of its
+  // class once the cycle's outcome is read back (it moved no bytes)
+  std::vector<int64_t> midrec_;
   int64_t gk_nyc_ = 0, gk_nxc_ = 0, gk_nmv_ = 0, gk_nmtv_ = 0;  // counts of the captured cycle
 
   void alloc();

@@ -227,36 +227,46 @@ void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void 
 // do_sqrt: out[0] = sqrt(max(0,sum)).
 void dot(Context &ctx, Prec pa, const void *a, Prec pb, const void *b,
          int64_t len, double *out, bool do_sqrt = false);
-// out[j] = V(:,j) . w for j < s (FP64 accumulate).
 // Average duration (ms) of one V^T w partial launch (k_vtw_partial, no
 // finishing sum) at len x s in precision p, over reps back-to-back launches
 // on the context's stream (V, w zero-filled, allocated here).
 double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps);
+// out[j] = V(:,j) . w for j < s (FP64 accumulate). With wsq (s < 512) also
+// out[s] = ||w||^2 from the same read of w: the input of the selective
+// second pass below.
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
-            const void *w, int64_t len, double *out);
+            const void *w, int64_t len, double *out, bool wsq = false);
 // Reduction partials (doubles) that a CGS2 of w (length len) against s
 // columns can take in the launches of gemv_t, gemv_n_sub_t and gemv_n_sub:
 // an upper bound over every kernel those launchers may choose.
 std::size_t cgs2_partials_bound(int64_t len, int s);
-// w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
-// w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
 // R(:, j) = AX(:, j) - theta_j X(:, j), j < q (neg_theta: q device doubles,
 // -theta_j), columns with leading dimension ld: bitwise a copy of AX followed
 // by axpby(-theta_j, X, 1) per column, in one launch.
 void resid_block(Context &ctx, Prec p, int64_t len, int q, const void *AX, const void *X,
                  int64_t ld, const double *neg_theta, void *R);
+// Selective CGS pair (g non-null): g holds the first pass's s coefficients
+// and ||w||^2 (gemv_t with wsq). When the first pass cancelled less than half
+// of ||w||^2 (the DGKS test, eta = 1/sqrt(2)), gemv_n_sub_t does nothing,
+// its out is zero, and the following gemv_n_sub / gemv_n_sub_normalise apply
+// the first pass's update w -= V g instead of w -= V h: one CGS pass, decided
+// on the device. *ran (optional) gets 1.0 when the second pass ran, else 0.
+//
+// w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
 // with the norm followed by scale_copy_dev), the norm and copy in one launch.
 void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                           const double *h, Prec pw, void *w, int64_t len, double *nrm,
-                          void *dst);
+                          void *dst, const double *g = nullptr, double *ran = nullptr);
+// w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                 const double *h, Prec pw, void *w, int64_t len,
-                double *nrm2_out = nullptr);
+                double *nrm2_out = nullptr, const double *g = nullptr, double *ran = nullptr);
 // w -= V h, then out = V^T w for the updated w (FP64): the middle of a CGS
 // pair with one read of V; the update as gemv_n_sub, out as gemv_t up to the
 // grouping of its partial sums. With nrm1, also ||w|| after the update.
 void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
-                  Prec pw, void *w, int64_t len, double *out, double *nrm1 = nullptr);
+                  Prec pw, void *w, int64_t len, double *out, double *nrm1 = nullptr,
+                  const double *g = nullptr);
 // out = V h (overwrite)
 void gemv_n(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
             const double *h, Prec po, void *out, int64_t len);

@@ -451,6 +451,41 @@ __global__ void __launch_bounds__(kBlock)
     out[blockIdx.x] = do_sqrt ? sqrt(fmax(0.0, acc)) : acc;
 }
 
+// Whether a classical Gram-Schmidt pass needs its second pass, by the test of
+// Daniel, Gragg, Kaufman and Stewart (Math. Comp. 30 (1976) 772-795) with
+// eta = 1/sqrt(2), the test ARPACK's Lanczos and Arnoldi steps use: the
+// second pass is skipped when the first kept ||w'|| >= eta ||w||, i.e. when
+// it cancelled less than half of ||w||^2; then w' is orthogonal to V to
+// working precision already and a second pass changes it only by rounding.
+// g[0 .. s) are the first pass's coefficients h1 = V^T w and g[s] = ||w||^2
+// (both FP64 sums); ||w'||^2 = ||w||^2 - ||h1||^2 for orthonormal V. Every
+// thread evaluates the same sums in the same order, so every block of every
+// launch reaches the same decision.
+__device__ __forceinline__ bool dgks_skip(const double *__restrict__ g, int s) {
+  double a = 0.0;
+  for (int i = 0; i < s; ++i)
+    a += g[i] * g[i];
+  const double w2 = g[s];
+  return isfinite(a) && isfinite(w2) && w2 > 0.0 && a < 0.5 * w2;
+}
+
+// k_finish for the second-pass coefficients of a selective CGS pair: with
+// the second pass skipped (dgks_skip over g and s = gridDim.x columns) the
+// coefficients are zero and the partials are not read.
+__global__ void __launch_bounds__(kBlock)
+    k_finish_pass2(const double *__restrict__ part, int nb, double *__restrict__ out,
+                   const double *__restrict__ g) {
+  __shared__ double lds[kWaves];
+  if (dgks_skip(g, (int)gridDim.x)) {
+    if (threadIdx.x == 0)
+      out[blockIdx.x] = 0.0;
+    return;
+  }
+  const double acc = sum_partials(part + (int64_t)blockIdx.x * nb, nb, lds);
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = acc;
+}
+
 // Reorthogonalisation products with memory-level parallelism: a block owns
 // kRB = 256*kR consecutive rows; thread t owns rows r0+t+k*256 (k < kR), so
 // for every column the block issues kR coalesced loads per thread before any
@@ -467,7 +502,7 @@ template <class TV> struct ColUnroll { static constexpr int value = sizeof(TV) =
 template <class TV, class TW>
 __global__ void __launch_bounds__(kBlock)
     k_vtw_partial(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
-                  const TW *__restrict__ w, double *__restrict__ part, int nb) {
+                  const TW *__restrict__ w, double *__restrict__ part, int nb, int wsq) {
   constexpr int CU = ColUnroll<TV>::value;
   extern __shared__ double red[];  // [s][kWaves]
   const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
@@ -521,8 +556,17 @@ __global__ void __launch_bounds__(kBlock)
     if (lane == 0)
       red[c * kWaves + wid] = acc;
   }
+  if (wsq) {  // ||w||^2 of the block's rows as column s
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kR; ++k)
+      acc += wv[k] * wv[k];
+    acc = wave_sum(acc);
+    if (lane == 0)
+      red[s * kWaves + wid] = acc;
+  }
   __syncthreads();
-  for (int c = threadIdx.x; c < s; c += kBlock) {
+  for (int c = threadIdx.x; c < s + wsq; c += kBlock) {
     double t = 0.0;
     for (int w2 = 0; w2 < kWaves; ++w2)
       t += red[c * kWaves + w2];
@@ -537,7 +581,7 @@ __global__ void __launch_bounds__(kBlock)
 // 16-byte aligned.
 __global__ void __launch_bounds__(kBlock)
     k_vtw4_partial(int64_t len, const float *__restrict__ V, int64_t ld, int s,
-                   const float *__restrict__ w, double *__restrict__ part, int nb) {
+                   const float *__restrict__ w, double *__restrict__ part, int nb, int wsq) {
   constexpr int G = 2, CU = 2;
   extern __shared__ double red[];  // [s][kWaves]
   const int64_t nq = len >> 2;
@@ -600,8 +644,19 @@ __global__ void __launch_bounds__(kBlock)
     if (lane == 0)
       red[c * kWaves + wid] = acc;
   }
+  if (wsq) {  // ||w||^2 of the block's rows as column s
+    double acc = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        acc += wv[g][e] * wv[g][e];
+    acc = wave_sum(acc);
+    if (lane == 0)
+      red[s * kWaves + wid] = acc;
+  }
   __syncthreads();
-  for (int c = threadIdx.x; c < s; c += kBlock) {
+  for (int c = threadIdx.x; c < s + wsq; c += kBlock) {
     double t = 0.0;
     for (int w2 = 0; w2 < kWaves; ++w2)
       t += red[c * kWaves + w2];
@@ -613,12 +668,22 @@ __global__ void __launch_bounds__(kBlock)
 template <class TV, class TW>
 __global__ void __launch_bounds__(kBlock)
     k_vh_sub(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
-             const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part) {
+             const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
+             const double *__restrict__ g, double *__restrict__ ran) {
   constexpr int CU = ColUnroll<TV>::value;
   extern __shared__ double hs[];  // [s]
   __shared__ double lds[kWaves];
+  // With g (the last sweep of a selective CGS pair; g = first-pass
+  // coefficients and ||w||^2): when dgks_skip holds, the middle sweep did
+  // nothing and this launch applies the first pass's update w -= V h1
+  // instead of the second pass's (the arithmetic of the middle sweep's
+  // update). ran[0] records whether the second pass ran (1) or not (0).
+  const bool skip = g != nullptr && dgks_skip(g, s);
+  const double *hsrc = skip ? g : h;
   for (int j = threadIdx.x; j < s; j += kBlock)
-    hs[j] = h[j];
+    hs[j] = hsrc[j];
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0)
+    ran[0] = skip ? 0.0 : 1.0;
   __syncthreads();
   const int64_t r0 = (int64_t)blockIdx.x * kRB + threadIdx.x;
   int64_t rr[kR];
@@ -700,9 +765,13 @@ template <class TV, class TW, int HMAX, bool MULTI, bool VEC>
 __global__ void __launch_bounds__(kBlock)
     k_vh_sub_vtw_lds(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
                      const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
-                     int nb, double *__restrict__ npart) {
+                     int nb, double *__restrict__ npart, const double *__restrict__ g) {
   static_assert(!VEC || (std::is_same<TV, float>::value && HMAX % 4 == 0),
                 "16-byte tile loads are for FP32 blocks");
+  // selective CGS pair: no second pass, nothing to do here (the last sweep
+  // applies the first pass's update)
+  if (g != nullptr && dgks_skip(g, s))
+    return;
   constexpr int NV = VEC ? HMAX / 4 : 1;  // 16-byte loads per thread
   extern __shared__ double sh[];
   double *wn = sh;  // [kLdsRows]
@@ -832,7 +901,9 @@ template <class TV, class TW>
 __global__ void __launch_bounds__(kBlock)
     k_vh_sub_vtw(int64_t len, const TV *__restrict__ V, int64_t ld, int s,
                  const double *__restrict__ h, TW *__restrict__ w, double *__restrict__ part,
-                 int nb) {
+                 int nb, const double *__restrict__ g) {
+  if (g != nullptr && dgks_skip(g, s))  // selective CGS pair, no second pass
+    return;
   constexpr int CU = ColUnroll<TV>::value;
   extern __shared__ double sh[];  // hs[s], then red[s][kWaves]
   double *hs = sh;
@@ -2590,27 +2661,30 @@ inline int vtw_blocks(int64_t len) {
 template <class TV, class TW>
 static void launch_vtw_partial(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
                                int64_t len, double *part, int nb, hipEvent_t ea = nullptr,
-                               hipEvent_t eb = nullptr) {
-  const std::size_t lds = (std::size_t)s * kWaves * sizeof(double);
+                               hipEvent_t eb = nullptr, int wsq = 0) {
+  const std::size_t lds = (std::size_t)(s + wsq) * kWaves * sizeof(double);
   if constexpr (std::is_same<TV, float>::value && std::is_same<TW, float>::value) {
     if (len % 4 == 0 && ld % 4 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
         reinterpret_cast<uintptr_t>(w) % 16 == 0) {
       hipExtLaunchKernelGGL(k_vtw4_partial, dim3(nb), dim3(kBlock), lds, ctx.stream, ea, eb, 0,
-                            len, V, ld, s, w, part, nb);
+                            len, V, ld, s, w, part, nb, wsq);
       return;
     }
   }
   hipExtLaunchKernelGGL((k_vtw_partial<TV, TW>), dim3(nb), dim3(kBlock), lds, ctx.stream, ea, eb,
-                        0, len, V, ld, s, w, part, nb);
+                        0, len, V, ld, s, w, part, nb, wsq);
 }
 
 template <class TV, class TW>
 static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW *w,
-                          int64_t len, double *out) {
+                          int64_t len, double *out, bool wsq = false) {
   const int nb = vtw_blocks(len);
+  if (wsq && s >= 512)
+    throw std::invalid_argument("gemv_t: ||w||^2 column with at most 511 columns");
   for (int c0 = 0; c0 < s; c0 += 512) {  // LDS holds 512 x 4 partials
     const int sc = std::min(512, s - c0);
-    double *part = ctx.partials((std::size_t)nb * sc);
+    const int sq = wsq ? 1 : 0;  // single chunk when wsq
+    double *part = ctx.partials((std::size_t)nb * (sc + sq));
     {
       // class 1 is the FP32 reorthogonalisation product (k_vtw4_partial for
       // aligned blocks), the kernel the roofline reports
@@ -2619,9 +2693,10 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
                     (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc,
                     len, sc, /*stamped=*/true);
       launch_vtw_partial(ctx, V + (int64_t)c0 * ld, ld, sc, w, len, part, nb, tr.start(),
-                         tr.stop());
+                         tr.stop(), sq);
     }
-    hipLaunchKernelGGL(k_finish, dim3(sc), dim3(kBlock), 0, ctx.stream, part, nb, out + c0, 0);
+    hipLaunchKernelGGL(k_finish, dim3(sc + sq), dim3(kBlock), 0, ctx.stream, part, nb, out + c0,
+                       0);
   }
   KH_CHECK(hipGetLastError());
 }
@@ -2676,7 +2751,7 @@ double time_vtw(Context &ctx, Prec p, int64_t len, int s, int reps) {
 
 std::size_t cgs2_partials_bound(int64_t len, int s) {
   const std::size_t nb = (std::size_t)vtw_blocks(len), sc = (std::size_t)std::max(s, 0);
-  std::size_t need = nb * std::min<std::size_t>(sc, 512);  // V^T w (launch_gemv_t)
+  std::size_t need = nb * std::min<std::size_t>(sc + 1, 512);  // V^T w (+ ||w||^2)
   need = std::max(need, nb * sc);                           // two-sweep middle (k_vh_sub_vtw)
   need = std::max(need, nb);                                // norm of the last update
   const std::size_t ntl = (std::size_t)((std::max<int64_t>(len, 0) + kLdsRows - 1) / kLdsRows);
@@ -2685,23 +2760,24 @@ std::size_t cgs2_partials_bound(int64_t len, int s) {
 }
 
 void gemv_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, Prec pw,
-            const void *w, int64_t len, double *out) {
+            const void *w, int64_t len, double *out, bool wsq) {
   if (s <= 0)
     return;
   if (pv == Prec::F64 && pw == Prec::F64)
-    launch_gemv_t(ctx, (const double *)V, ld, s, (const double *)w, len, out);
+    launch_gemv_t(ctx, (const double *)V, ld, s, (const double *)w, len, out, wsq);
   else if (pv == Prec::F32 && pw == Prec::F32)
-    launch_gemv_t(ctx, (const float *)V, ld, s, (const float *)w, len, out);
+    launch_gemv_t(ctx, (const float *)V, ld, s, (const float *)w, len, out, wsq);
   else if (pv == Prec::F64)
-    launch_gemv_t(ctx, (const double *)V, ld, s, (const float *)w, len, out);
+    launch_gemv_t(ctx, (const double *)V, ld, s, (const float *)w, len, out, wsq);
   else
-    launch_gemv_t(ctx, (const float *)V, ld, s, (const double *)w, len, out);
+    launch_gemv_t(ctx, (const float *)V, ld, s, (const double *)w, len, out, wsq);
 }
 
 template <class TV, class TW>
 static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
                               const double *h, TW *w, int64_t len, double *nrm,
-                              TW *dst = nullptr) {
+                              TW *dst = nullptr, const double *g = nullptr,
+                              double *ran = nullptr) {
   const int nb = vtw_blocks(len);
   double *part = nrm ? ctx.partials(nb) : nullptr;
   {
@@ -2709,7 +2785,7 @@ static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
                 (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * s);
   hipLaunchKernelGGL((k_vh_sub<TV, TW>), dim3(nb), dim3(kBlock),
                      (std::size_t)std::max(s, 1) * sizeof(double), ctx.stream, len, V, ld, s, h,
-                     w, part);
+                     w, part, g, ran);
   }
   // every block of the fused norm-and-copy re-reads the nb partials: used up
   // to 1024 partials (2^21 rows) on at most 2048 blocks (<= 16 MB of L2
@@ -2728,34 +2804,45 @@ static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
 
 void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                           const double *h, Prec pw, void *w, int64_t len, double *nrm,
-                          void *dst) {
+                          void *dst, const double *g, double *ran) {
   if (pv == Prec::F64 && pw == Prec::F64)
-    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm, (double *)dst);
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm, (double *)dst, g,
+                      ran);
   else if (pv == Prec::F32 && pw == Prec::F32)
-    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm, (float *)dst);
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm, (float *)dst, g, ran);
   else if (pv == Prec::F64)
-    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm, (float *)dst);
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm, (float *)dst, g,
+                      ran);
   else
-    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm, (double *)dst);
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm, (double *)dst, g,
+                      ran);
 }
 
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
-                const double *h, Prec pw, void *w, int64_t len, double *nrm2_out) {
+                const double *h, Prec pw, void *w, int64_t len, double *nrm2_out,
+                const double *g, double *ran) {
   if (s <= 0 && !nrm2_out)
     return;
   if (pv == Prec::F64 && pw == Prec::F64)
-    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm2_out);
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (double *)w, len, nrm2_out,
+                      (double *)nullptr, g, ran);
   else if (pv == Prec::F32 && pw == Prec::F32)
-    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm2_out);
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (float *)w, len, nrm2_out,
+                      (float *)nullptr, g, ran);
   else if (pv == Prec::F64)
-    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm2_out);
+    launch_gemv_n_sub(ctx, (const double *)V, ld, s, h, (float *)w, len, nrm2_out,
+                      (float *)nullptr, g, ran);
   else
-    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm2_out);
+    launch_gemv_n_sub(ctx, (const float *)V, ld, s, h, (double *)w, len, nrm2_out,
+                      (double *)nullptr, g, ran);
 }
 
 template <class TV, class TW>
 static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, const double *h,
-                                TW *w, int64_t len, double *out, double *nrm1) {
+                                TW *w, int64_t len, double *out, double *nrm1,
+                                const double *g = nullptr) {
+  if (g && nrm1)
+    throw std::invalid_argument("gemv_n_sub_t: a selective pair takes no norm here");
   // LDS-tiled form when its tile fits 64 KB and the columns fill it (below
   // about 12 columns the two-sweep kernel is faster)
   const std::size_t lds = kLdsRows * 8 + (std::size_t)s * kLdsStride * sizeof(TV);
@@ -2780,7 +2867,7 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
       const bool multi = ntl > nbl;
 #define KH_LDS_LAUNCH(H, M, VC)                                                                \
   hipLaunchKernelGGL((k_vh_sub_vtw_lds<TV, TW, H, M, VC>), dim3((unsigned)nbl), dim3(kBlock),  \
-                     lds, ctx.stream, len, V, ld, s, h, w, part, (int)nbl, npart)
+                     lds, ctx.stream, len, V, ld, s, h, w, part, (int)nbl, npart, g)
 #define KH_LDS_PICK(H)                                                                         \
   {                                                                                            \
     if (vec) {                                                                                 \
@@ -2800,7 +2887,11 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
 #undef KH_LDS_PICK
 #undef KH_LDS_LAUNCH
     }
-    hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl, out, 0);
+    if (g)
+      hipLaunchKernelGGL(k_finish_pass2, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl,
+                         out, g);
+    else
+      hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, (int)nbl, out, 0);
     if (nrm1)
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, ctx.stream, npart, (int)nbl, nrm1, 1);
     KH_CHECK(hipGetLastError());
@@ -2814,9 +2905,12 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
                   (double)len * s * sizeof(TV) + 2.0 * len * sizeof(TW) + 8.0 * nb * s);
     hipLaunchKernelGGL((k_vh_sub_vtw<TV, TW>), dim3(nb), dim3(kBlock),
                        (std::size_t)s * (1 + kWaves) * sizeof(double), ctx.stream, len, V, ld, s,
-                       h, w, part, nb);
+                       h, w, part, nb, g);
   }
-  hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
+  if (g)
+    hipLaunchKernelGGL(k_finish_pass2, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, g);
+  else
+    hipLaunchKernelGGL(k_finish, dim3(s), dim3(kBlock), 0, ctx.stream, part, nb, out, 0);
   KH_CHECK(hipGetLastError());
   if (nrm1)  // after the finish: the dot reuses the partial buffer
     dot(ctx, std::is_same<TW, double>::value ? Prec::F64 : Prec::F32, w,
@@ -2824,19 +2918,19 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
 }
 
 void gemv_n_sub_t(Context &ctx, Prec pv, const void *V, int64_t ld, int s, const double *h,
-                  Prec pw, void *w, int64_t len, double *out, double *nrm1) {
+                  Prec pw, void *w, int64_t len, double *out, double *nrm1, const double *g) {
   if (s <= 0)
     return;
   if (s > 1024)
     throw std::invalid_argument("gemv_n_sub_t: more than 1024 columns");
   if (pv == Prec::F64 && pw == Prec::F64)
-    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (double *)w, len, out, nrm1);
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (double *)w, len, out, nrm1, g);
   else if (pv == Prec::F32 && pw == Prec::F32)
-    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (float *)w, len, out, nrm1);
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (float *)w, len, out, nrm1, g);
   else if (pv == Prec::F64)
-    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (float *)w, len, out, nrm1);
+    launch_gemv_n_sub_t(ctx, (const double *)V, ld, s, h, (float *)w, len, out, nrm1, g);
   else
-    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out, nrm1);
+    launch_gemv_n_sub_t(ctx, (const float *)V, ld, s, h, (double *)w, len, out, nrm1, g);
 }
 
 // k_nn_mfma when the block fits it (q <= 128 output columns, H^T in 64 KB

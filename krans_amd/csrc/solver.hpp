@@ -130,6 +130,12 @@ struct CyclicParams {
   // refined extraction has a zero target (no vector work), else 1; 1: FP32
   // candidate vectors and their residuals; -1: no screen
   int screen = 0;
+  // second pass of each Golub-Kahan step's classical Gram-Schmidt pair
+  // (reorth = 2): 1 always (the reference's double reorthogonalisation);
+  // 2 only where the first pass cancelled more than half of ||w||^2 (the
+  // DGKS test with eta = 1/sqrt(2), decided on the device); 0 automatic:
+  // 2 for FP32 Krylov vectors, 1 for FP64
+  int second_pass = 0;
 };
 
 struct PhaseTimes {
@@ -147,6 +153,7 @@ struct CyclicStats {
   int64_t refined = 0;            // locked vectors replaced by the final FP64 pass (mixed)
   int64_t escalated = 0;          // FP32 projector CGs solved again in FP64 (mixed)
   int64_t graph_captures = 0, graph_replays = 0;  // Golub-Kahan cycles as HIP graphs
+  int64_t cgs_pairs = 0, second_passes = 0;  // Golub-Kahan CGS pairs, second passes run
   PhaseTimes t;
 };
 

@@ -162,6 +162,43 @@ def test_incidence_k24_wide_window_products(precision):
     print(precision, "restarts", r.restarts, "screened", r.screened)
 
 
+@pytest.mark.parametrize("operand", ["csr", "stencil"])
+@pytest.mark.parametrize("precision,second_pass", [("fp64", 0), ("mixed", 0), ("mixed", 1),
+                                                   ("fp64", 2)])
+def test_selective_second_pass(operand, precision, second_pass):
+    """The Golub-Kahan steps' second Gram-Schmidt pass: automatic (0) is the
+    reference's double reorthogonalisation for FP64 (every pair runs both
+    passes) and the selective DGKS pass for mixed precision (a pair runs its
+    second pass only where the first cancelled; after the three-term
+    recurrence hardly any do). Every setting returns the exact kernel at
+    tol_rel 1e-12 (principal-angle sine <= 1e-10, residual within the
+    threshold)."""
+    print(f"[case] second pass {operand} {precision} {second_pass}", flush=True)
+    if operand == "csr":
+        g, v, k = 8, 1024, 8
+        A = W.regular_incidence(g, v, seed=42)
+        H, Kx, n = hip(A), W.component_indicators(g, v), A.n
+    else:
+        n, N, c, k = 4096, 8, 0.5, 8  # nonzero singular values in [0.5, 1.5]
+        A = W.shifted_difference(n, N, c)
+        H, Kx = K.HipShiftedDifferenceOperator(n, N, c), W.shifted_difference_kernel(n, N, c)
+    opt = K.CyclicLanczosOptions(k=k, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
+                                 max_restarts=400, second_pass=second_pass)
+    r = K.find_nullspace_cyclic_lanczos_ex(H, K.HipVectorFactory(n, 42), opt, precision)
+    U = r.right_null
+    assert U.shape[1] == k and r.converged_to_target
+    assert residuals(A, U).max() <= 1e-12 * r.frob_norm_est
+    assert principal_sine(U, Kx) <= 1e-10
+    assert r.cgs_pairs > 0
+    selective = second_pass == 2 or (second_pass == 0 and precision == "mixed")
+    if selective:
+        assert r.second_passes < r.cgs_pairs / 4, (r.second_passes, r.cgs_pairs)
+    else:
+        assert r.second_passes == r.cgs_pairs
+    print(precision, second_pass, "restarts", r.restarts, "pairs", r.cgs_pairs, "second passes",
+          r.second_passes)
+
+
 def test_small_2x3_analytic():
     # tests/test_cyclic_lanczos.cpp:144-190
     A = csr_small_2x3()

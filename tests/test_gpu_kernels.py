@@ -215,6 +215,56 @@ def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
         assert np.abs(V.astype(np.float64).T @ wx.astype(np.float64)).max() <= 64 * eps * wn
 
 
+@pytest.mark.parametrize("s,n,prec", [(1, 5000, 0), (33, 100_003, 1), (64, 2**20, 1),
+                                      (97, 300_000, 0), (20, 2**21 + 12_344, 1)])
+@pytest.mark.parametrize("cancel", [False, True])
+def test_cgs2_selective_pair(ctx, s, n, prec, cancel):
+    """Selective second pass (kh_cgs2_dev_select, the form the mixed
+    Golub-Kahan steps use). The Daniel-Gragg-Kaufman-Stewart test with
+    eta = 1/sqrt(2) decides on the device: w mostly inside span(V) (the first
+    pass cancels most of ||w||^2) must take the second pass, and then the
+    results are bitwise kh_cgs2_dev's; w mostly outside span(V) must skip it,
+    and the single pass must still leave w orthogonal to V to working
+    precision (the "twice is enough" bound for an uncancelled pass) and equal
+    w - V V^T w to rounding."""
+    print(f"[case] cgs2 selective s={s} n={n} prec={prec} cancel={cancel}", flush=True)
+    rng = np.random.default_rng(7 * s + n)
+    V, _ = np.linalg.qr(rng.standard_normal((n, s)))
+    if cancel:  # ||V^T w||^2 > ||w||^2 / 2
+        w = V @ rng.standard_normal(s) + 1e-3 * rng.standard_normal(n) / np.sqrt(n)
+    else:  # ||V^T w||^2 about s / n of ||w||^2
+        w = rng.standard_normal(n) + 1e-2 * (V @ rng.standard_normal(s))
+    dt = np.float64 if prec == 0 else np.float32
+    V, w = V.astype(dt), w.astype(dt)
+    Vd, wd = V.astype(np.float64), w.astype(np.float64)
+    h1 = Vd.T @ wd
+    assert (h1 @ h1 > 0.5 * (wd @ wd)) == cancel  # the case is what it claims
+    dV, dw = Dev(ctx, V), Dev(ctx, w)
+    h, nrm, ran = np.zeros(s), np.zeros(1), np.zeros(1, dtype=np.int32)
+    K._check(K.lib().kh_cgs2_dev_select(ctx.h, prec, dV.p, s, dw.p, n,
+                                        h.ctypes.data_as(C.c_void_p),
+                                        nrm.ctypes.data_as(C.c_void_p),
+                                        ran.ctypes.data_as(C.c_void_p)))
+    wsel = dw.get()
+    assert int(ran[0]) == int(cancel)
+    eps = np.finfo(dt).eps
+    if cancel:
+        dV2, dw2 = Dev(ctx, V), Dev(ctx, w)
+        h2, n2 = np.zeros(s), np.zeros(1)
+        K._check(K.lib().kh_cgs2_dev(ctx.h, prec, dV2.p, s, dw2.p, n,
+                                     h2.ctypes.data_as(C.c_void_p),
+                                     n2.ctypes.data_as(C.POINTER(C.c_double))))
+        assert np.array_equal(wsel, dw2.get())
+        assert np.array_equal(h, h2) and np.array_equal(nrm, n2)
+    else:
+        ref = wd - Vd @ h1
+        wn = float(np.linalg.norm(wd))
+        assert np.abs(wsel.astype(np.float64) - ref).max() <= 4 * eps * wn
+        assert np.abs(h - h1).max() <= 1e-13 * wn
+        assert np.abs(Vd.T @ wsel.astype(np.float64)).max() <= 64 * eps * wn
+        assert abs(nrm[0] - np.linalg.norm(wsel.astype(np.float64))) <= 16 * eps * nrm[0]
+
+
 @pytest.mark.parametrize("s,q,n,prec", [(5, 7, 1000, 0), (16, 16, 1_000_003, 0), (11, 15, 257, 0),
                                         (64, 64, 300_001, 0), (17, 33, 70_001, 0),
                                         (64, 5, 1_000_003, 0), (97, 33, 70_000, 0),
