@@ -57,7 +57,10 @@ typedef struct kh_rng_s kh_rng;
  * reorthogonalisation); 2 only where the first pass cancelled more than
  * half of ||w||^2 (the Daniel-Gragg-Kaufman-Stewart test with
  * eta = 1/sqrt(2), as in ARPACK, decided on the device); 0 (default) 2 for
- * mixed precision and 1 for FP64. */
+ * mixed precision and 1 for FP64. Selective steps run one pass each; a
+ * cycle in which a step's test asks for the second pass is queued again from
+ * that step with both passes there, so each step ends with the passes its
+ * own test asks for. */
 typedef struct kh_cyclic_options {
   int64_t k, ncv, max_restarts;
   double tol_abs, tol_rel;
@@ -73,6 +76,7 @@ typedef struct kh_cyclic_options {
   int32_t return_left_space, verbose;
   int32_t precond_jacobi, graphs;
   int32_t screen, second_pass;
+  double second_pass_eta; /* eta of the test, 0 < eta <= 1 (0: 1/sqrt(2)) */
 } kh_cyclic_options;
 
 /* krans::solvers::CyclicLanczosResult (CyclicLanczosSolver.hpp:149-167) plus
@@ -94,8 +98,10 @@ typedef struct kh_cyclic_result {
    * (stalled, cg_maxit or breakdown) and were solved again in FP64 */
   int64_t escalated;
   /* Gram-Schmidt pairs of the Golub-Kahan steps and how many of them ran
-   * their second pass (all of them unless the second pass is selective) */
-  int64_t cgs_pairs, second_passes;
+   * their second pass (all of them unless the second pass is selective);
+   * selective: the steps are queued with one pass, and a cycle is queued
+   * again from a step whose test asks for the second pass (gk_requeued) */
+  int64_t cgs_pairs, second_passes, gk_requeued;
 } kh_cyclic_result;
 
 /* krans::solvers::LOBPCGOptions (LOBPCGSolver.hpp:71-115); M_inv and

@@ -44,7 +44,7 @@ class _CyclicOpts(C.Structure):
         ("stagnation_restarts", C.c_int64),
         ("return_left_space", C.c_int32), ("verbose", C.c_int32),
         ("precond_jacobi", C.c_int32), ("graphs", C.c_int32), ("screen", C.c_int32),
-        ("second_pass", C.c_int32),
+        ("second_pass", C.c_int32), ("second_pass_eta", C.c_double),
     ]
 
 
@@ -59,7 +59,7 @@ class _CyclicRes(C.Structure):
         ("t_polish", C.c_double), ("t_fallback", C.c_double), ("t_rng", C.c_double),
         ("screened", C.c_int64), ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
         ("refined", C.c_int64), ("packed", C.c_int64), ("escalated", C.c_int64),
-        ("cgs_pairs", C.c_int64), ("second_passes", C.c_int64),
+        ("cgs_pairs", C.c_int64), ("second_passes", C.c_int64), ("gk_requeued", C.c_int64),
     ]
 
 
@@ -495,6 +495,8 @@ class CyclicLanczosOptions:
     # ||w||^2 (2, the DGKS test decided on the device), or automatic (0: 2 for
     # mixed precision, 1 for FP64)
     second_pass: int = 0
+    # eta of that test (0 < eta <= 1; 1 asks for every second pass)
+    second_pass_eta: float = 0.7071067811865476
 
     def _c(self):
         o = _CyclicOpts()
@@ -531,6 +533,7 @@ class CyclicLanczosResult:
     escalated: int = 0  # mixed: FP32 projector CGs that did not converge, solved again in FP64
     cgs_pairs: int = 0  # Gram-Schmidt pairs of the Golub-Kahan steps
     second_passes: int = 0  # ... that ran their second pass
+    gk_requeued: int = 0  # cycles queued again from a step that needed its second pass
     n_right: int = -1  # columns written to the device block (find_nullspace_cyclic_lanczos_dev)
 
 
@@ -564,7 +567,7 @@ def find_nullspace_cyclic_lanczos_ex(A, vf, opt, precision="fp64"):
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
         refined=res.refined, escalated=res.escalated, cgs_pairs=res.cgs_pairs,
-        second_passes=res.second_passes)
+        second_passes=res.second_passes, gk_requeued=res.gk_requeued)
 
 
 class DeviceBlock:
@@ -628,7 +631,7 @@ def find_nullspace_cyclic_lanczos_dev(A, vf, opt, out, precision="fp64"):
         spmtv=res.spmtv, cg_iters=res.cg_iters, times=times, screened=res.screened,
         graph_captures=res.graph_captures, graph_replays=res.graph_replays,
         refined=res.refined, escalated=res.escalated, cgs_pairs=res.cgs_pairs,
-        second_passes=res.second_passes, n_right=res.n_right)
+        second_passes=res.second_passes, gk_requeued=res.gk_requeued, n_right=res.n_right)
     return r
 
 

@@ -88,6 +88,9 @@ kh::CyclicParams to_params(const kh_cyclic_options &o, int precision) {
   P.screen = o.screen;
   need(o.second_pass >= 0 && o.second_pass <= 2, "second_pass must be 0, 1 or 2");
   P.second_pass = o.second_pass;
+  need(o.second_pass_eta >= 0.0 && o.second_pass_eta <= 1.0, "second_pass_eta must be in [0, 1]");
+  if (o.second_pass_eta > 0.0)
+    P.second_pass_eta = o.second_pass_eta;
   P.krylov = prec_of(precision);
   P.polish = P.krylov;
   need(P.k >= 0 && P.ncv >= 0 && P.max_restarts >= 0 && P.cg_maxit >= 0,
@@ -406,6 +409,7 @@ int cyclic_entry(kh_matrix *A, kh_rng *rng, const kh_cyclic_options *opt, int pr
     res->graph_replays = S.graph_replays + SL.graph_replays;
     res->cgs_pairs = S.cgs_pairs + SL.cgs_pairs;
     res->second_passes = S.second_passes + SL.second_passes;
+    res->gk_requeued = S.gk_requeued + SL.gk_requeued;
   });
 }
 }  // namespace
@@ -549,6 +553,7 @@ int kh_server_wait(kh_server *s, int64_t id, double *right, int64_t cap, kh_cycl
     res->graph_replays = r.stats.graph_replays;
     res->cgs_pairs = r.stats.cgs_pairs;
     res->second_passes = r.stats.second_passes;
+    res->gk_requeued = r.stats.gk_requeued;
     if (latency_s)
       *latency_s = r.latency;
     if (queued_s)

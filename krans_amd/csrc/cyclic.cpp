@@ -613,6 +613,7 @@ int CyclicDriver::prepare() {
             (P_.graphs > 0 || (P_.graphs == 0 && m_ + n_ <= kGraphAutoLen));
   select2_ = P_.reorth == 2 &&
              (P_.second_pass == 2 || (P_.second_pass == 0 && kp_ == Prec::F32));
+  dgks_cancel_ = 1.0 - P_.second_pass_eta * P_.second_pass_eta;
 
   double *y0 = y0_.as<double>();
   if (P_.y0_host)
@@ -665,7 +666,7 @@ bool CyclicDriver::cycle_begin() {
 }
 
 // The Golub-Kahan build of one cycle (:374-421) on this solve's own storage.
-CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
+CyclicDriver::CycleQueued CyclicDriver::queue_cycle(int64_t j0, int64_t force2) {
   const Prec kp = kp_;
   char *Xb = static_cast<char *>(xp_);
   const int64_t ncv = ncv_;
@@ -675,14 +676,15 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
   // (non-finite or < 1e-14) is replayed on the host afterwards; steps
   // queued after a stop only write columns that are then ignored.
   const int nb = (int)basis_.count;
-  int64_t nyc = 1, nxc = 0, nmv = 0, nmtv = 0;
+  // step j starts from 1 + j / 2 y-vectors and (j + 1) / 2 x-vectors
+  int64_t nyc = 1 + j0 / 2, nxc = (j0 + 1) / 2, nmv = 0, nmtv = 0;
   // The queued cycle depends on nothing but the basis size: same kernels,
   // shapes and buffers, with beta_j read on the device. Where the operator
   // only enqueues kernels it is captured once per basis size and replayed.
   auto build = [&]() {
-    stepkind_.assign((std::size_t)ncv, 0);
-    midrec_.assign((std::size_t)ncv, -1);
-    for (int64_t j = 0; j < ncv; ++j) {
+    stepkind_.resize((std::size_t)ncv, 0);
+    std::fill(stepkind_.begin() + j0, stepkind_.end(), (char)0);
+    for (int64_t j = j0; j < ncv; ++j) {
       const bool from_y = nyc > nxc;
       void *w = from_y ? wxp_ : wyp_;
       const int64_t len = from_y ? m_ : n_;
@@ -702,31 +704,41 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
         groups.push_back({Ycol(nyc - 1), ldy_, (int)nyc + nb, kp});
       const Group &g0 = groups.front();
       bool stored = false;  // w / beta_j already written to its basis column
-      if (passes == 2 && g0.count > 0) {
+      if (passes == 2 && g0.count > 0 && select2_ && g0.count < 512 && j != force2) {
+        // Selective second pass, speculative: one CGS pass (h1 = V^T w with
+        // ||w||^2 from the same read of w; w -= V h1 with ||w||), and the
+        // DGKS test of the pass (its second pass is due when it cancelled
+        // half of ||w||^2 or more) recorded in pass2_[j]. The host reads
+        // the tests back with beta_j and, should a step need its second
+        // pass, queues the cycle again from that step with both passes
+        // there: the cycle ends as if each step had decided on the device,
+        // and the common step costs two reads of V and five launches.
+        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_, /*wsq=*/true);
+        if (nyc + nxc < ncv) {
+          gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, bnorm_ + j,
+                               from_y ? Xcol(nxc) : Ycol(nyc), h_, pass2_ + j, dgks_cancel_);
+          stored = true;
+        } else {
+          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, bnorm_ + j, h_,
+                     pass2_ + j, dgks_cancel_);
+        }
+        stepkind_[(std::size_t)j] = 3;
+      } else if (passes == 2 && g0.count > 0) {
         // CGS2 with three reads of V: h1 = V^T w; [w -= V h1, h2 = V^T w]
         // fused; w -= V h2 with ||w||. The arithmetic of two cgs_pass calls,
         // except that h2's partial sums are grouped per block of the fused
-        // kernel. Selective (select2_): h1 comes with ||w||^2 (h_[s]), and
-        // where the first pass cancelled less than half of ||w||^2 the
-        // middle sweep does nothing and the last one applies w -= V h1 (one
-        // pass, two reads of V), decided on the device; pass2_[j] records it.
-        const bool sel = select2_ && g0.count < 512;
-        const double *g = sel ? h_ : nullptr;
-        double *h2 = h_ + g0.count + 1;
-        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_, sel);
-        const std::size_t nrec = c_.trace.size();
-        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h2, nullptr, g);
-        if (sel && c_.trace.size() > nrec)
-          midrec_[(std::size_t)j] = (int64_t)nrec;
+        // kernel.
+        double *h2 = h_ + g0.count;
+        gemv_t(c_, g0.p, g0.V, g0.ld, g0.count, kp, w, len, h_);
+        gemv_n_sub_t(c_, g0.p, g0.V, g0.ld, g0.count, h_, kp, w, len, h2);
         if (nyc + nxc < ncv) {  // beta_j and the stored w / beta_j in one launch
           gemv_n_sub_normalise(c_, g0.p, g0.V, g0.ld, g0.count, h2, kp, w, len, bnorm_ + j,
-                               from_y ? Xcol(nxc) : Ycol(nyc), g, sel ? pass2_ + j : nullptr);
+                               from_y ? Xcol(nxc) : Ycol(nyc));
           stored = true;
         } else {
-          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h2, kp, w, len, bnorm_ + j, g,
-                     sel ? pass2_ + j : nullptr);
+          gemv_n_sub(c_, g0.p, g0.V, g0.ld, g0.count, h2, kp, w, len, bnorm_ + j);
         }
-        stepkind_[(std::size_t)j] = sel ? 2 : 1;
+        stepkind_[(std::size_t)j] = 1;
       } else {
         for (int pass = 0; pass < passes; ++pass)
           cgs_pass(w, kp, len, groups, pass == passes - 1 ? bnorm_ + j : nullptr);
@@ -743,7 +755,7 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
         ++nyc;
     }
   };
-  if (graphs_) {
+  if (graphs_ && j0 == 0 && force2 < 0) {
     if (!gk_graph_.holds(nb)) {
       gk_graph_.begin(c_);
       try {
@@ -755,8 +767,10 @@ CyclicDriver::CycleQueued CyclicDriver::queue_cycle() {
       gk_graph_.end(c_, nb, true);
       ++S_.graph_captures;
       gk_nyc_ = nyc, gk_nxc_ = nxc, gk_nmv_ = nmv, gk_nmtv_ = nmtv;
+      gk_kinds_ = stepkind_;
     } else {
       nyc = gk_nyc_, nxc = gk_nxc_, nmv = gk_nmv_, nmtv = gk_nmtv_;
+      stepkind_ = gk_kinds_;
     }
     gk_graph_.launch(c_);
     ++S_.graph_replays;
@@ -778,22 +792,42 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
   char *VBb = static_cast<char *>(vbp_);
   const int64_t ncv = ncv_;
   double *y0 = y0_.as<double>();
-  // beta_j (and, after selective CGS pairs, whether their second pass ran)
-  bool any_sel = false;
-  for (int64_t j = 0; j < queued && j < (int64_t)stepkind_.size(); ++j)
-    any_sel = any_sel || stepkind_[(std::size_t)j] == 2;
-  readback(c_, bnorm_, (int)(any_sel ? ncv + 8 + queued : queued), bh_.data());
-  for (int64_t j = 0; j < queued && j < (int64_t)stepkind_.size(); ++j) {
-    const char kind = stepkind_[(std::size_t)j];
+  // beta_j, and the DGKS tests of speculative single passes: the first step
+  // up to the stop whose test asks for the second pass is queued again with
+  // both passes there (and the steps after it), until none does
+  const auto kind_of = [&](int64_t j) {
+    return j < (int64_t)stepkind_.size() ? stepkind_[(std::size_t)j] : (char)0;
+  };
+  for (;;) {
+    bool spec = false;
+    for (int64_t j = 0; j < queued; ++j)
+      spec = spec || kind_of(j) == 3;
+    readback(c_, bnorm_, (int)(spec ? ncv + 8 + queued : queued), bh_.data());
+    int64_t redo = -1;
+    for (int64_t j = 0; spec && j < queued; ++j) {
+      if (kind_of(j) == 3 && bh_[(std::size_t)(ncv + 8 + j)] > 0.5) {
+        redo = j;
+        break;
+      }
+      const double bj = bh_[(std::size_t)j];
+      if (!std::isfinite(bj) || bj < 1e-14 || j + 1 >= ncv)
+        break;  // the stop step: later steps are ignored
+    }
+    if (redo < 0)
+      break;
+    ++S_.gk_requeued;
+    const CycleQueued q = queue_cycle(redo, redo);
+    queued = q.queued;
+    S_.spmv += q.nmv;
+    S_.spmtv += q.nmtv;
+  }
+  for (int64_t j = 0; j < queued; ++j) {
+    const char kind = kind_of(j);
     if (kind == 0)
       continue;
     ++S_.cgs_pairs;
-    if (kind == 1 || bh_[(std::size_t)(ncv + 8 + j)] > 0.5) {
+    if (kind == 1)
       ++S_.second_passes;
-    } else if (j < (int64_t)midrec_.size() && midrec_[(std::size_t)j] >= 0 &&
-               midrec_[(std::size_t)j] < (int64_t)c_.trace.size()) {
-      c_.trace[(std::size_t)midrec_[(std::size_t)j]].cls = kTraceNone;
-    }
   }
   double beta_last = 0.0;
   int64_t local_steps = 0;

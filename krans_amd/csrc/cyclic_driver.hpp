@@ -38,7 +38,9 @@ public:
   };
   int prepare();
   bool cycle_begin();
-  CycleQueued queue_cycle();
+  // The cycle from step j0 (0: the whole cycle); step force2 runs both
+  // Gram-Schmidt passes whatever the second-pass setting.
+  CycleQueued queue_cycle(int64_t j0 = 0, int64_t force2 = -1);
   int cycle_end(int64_t queued, int64_t nmv, int64_t nmtv);
   DeviceBasis after_loop();
   DeviceBasis finish();
@@ -91,14 +93,14 @@ private:
   CapturedWork gk_graph_;  // the queued Golub-Kahan cycle, keyed by the basis size
   bool graphs_ = false;
   bool select2_ = false;  // selective second CGS pass in the Golub-Kahan steps
-  double *pass2_ = nullptr;  // per step: 1 when its second pass ran (selective)
+  double dgks_cancel_ = 0.5;  // 1 - eta^2 of its test
+  // per step: 1 when the DGKS test of its speculative single pass asks for
+  // the second pass
+  double *pass2_ = nullptr;
   // per queued step of the current cycle: 0 no CGS pair, 1 pair with both
-  // passes, 2 selective pair (outcome in pass2_)
+  // passes, 3 speculative single pass (its test in pass2_)
   std::vector<char> stepkind_;
-  // with kernel tracing: per step, the trace record of a selective pair's
-  // middle sweep (-1 none); a sweep the device skipped is taken out of its
-  // class once the cycle's outcome is read back (it moved no bytes)
-  std::vector<int64_t> midrec_;
+  std::vector<char> gk_kinds_;  // stepkind_ of the captured cycle
   int64_t gk_nyc_ = 0, gk_nxc_ = 0, gk_nmv_ = 0, gk_nmtv_ = 0;  // counts of the captured cycle
 
   void alloc();

@@ -251,16 +251,21 @@ void resid_block(Context &ctx, Prec p, int64_t len, int q, const void *AX, const
 // its out is zero, and the following gemv_n_sub / gemv_n_sub_normalise apply
 // the first pass's update w -= V g instead of w -= V h: one CGS pass, decided
 // on the device. *ran (optional) gets 1.0 when the second pass ran, else 0.
+// `cancel` (gemv_n_sub / gemv_n_sub_normalise) is 1 - eta^2 of that test:
+// the second pass is due when ||h1||^2 >= cancel ||w||^2. With h = g the
+// launch is a speculative single pass whose *ran reports the test.
 //
 // w -= V h, then ||w|| into *nrm and dst = w / ||w|| (bitwise as gemv_n_sub
 // with the norm followed by scale_copy_dev), the norm and copy in one launch.
 void gemv_n_sub_normalise(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                           const double *h, Prec pw, void *w, int64_t len, double *nrm,
-                          void *dst, const double *g = nullptr, double *ran = nullptr);
+                          void *dst, const double *g = nullptr, double *ran = nullptr,
+                          double cancel = 0.5);
 // w -= V h  (h: s FP64 device scalars); with nrm2_out: also ||w_new||.
 void gemv_n_sub(Context &ctx, Prec pv, const void *V, int64_t ld, int s,
                 const double *h, Prec pw, void *w, int64_t len,
-                double *nrm2_out = nullptr, const double *g = nullptr, double *ran = nullptr);
+                double *nrm2_out = nullptr, const double *g = nullptr, double *ran = nullptr,
+                double cancel = 0.5);
 // w -= V h, then out = V^T w for the updated w (FP64): the middle of a CGS
 // pair with one read of V; the update as gemv_n_sub, out as gemv_t up to the
 // grouping of its partial sums. With nrm1, also ||w|| after the update.

@@ -132,10 +132,13 @@ struct CyclicParams {
   int screen = 0;
   // second pass of each Golub-Kahan step's classical Gram-Schmidt pair
   // (reorth = 2): 1 always (the reference's double reorthogonalisation);
-  // 2 only where the first pass cancelled more than half of ||w||^2 (the
-  // DGKS test with eta = 1/sqrt(2), decided on the device); 0 automatic:
+  // 2 only where the first pass cancelled half of ||w||^2 or more (the DGKS
+  // test with eta = 1/sqrt(2); steps are queued with one pass and a cycle is
+  // queued again from a step whose test asks for the second); 0 automatic:
   // 2 for FP32 Krylov vectors, 1 for FP64
   int second_pass = 0;
+  // eta of that test (0 < eta <= 1; 1 asks for every second pass)
+  double second_pass_eta = 0.70710678118654752;
 };
 
 struct PhaseTimes {
@@ -154,6 +157,7 @@ struct CyclicStats {
   int64_t escalated = 0;          // FP32 projector CGs solved again in FP64 (mixed)
   int64_t graph_captures = 0, graph_replays = 0;  // Golub-Kahan cycles as HIP graphs
   int64_t cgs_pairs = 0, second_passes = 0;  // Golub-Kahan CGS pairs, second passes run
+  int64_t gk_requeued = 0;  // cycles queued again from a step that needed its second pass
   PhaseTimes t;
 };
 

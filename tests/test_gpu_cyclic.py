@@ -193,10 +193,43 @@ def test_selective_second_pass(operand, precision, second_pass):
     selective = second_pass == 2 or (second_pass == 0 and precision == "mixed")
     if selective:
         assert r.second_passes < r.cgs_pairs / 4, (r.second_passes, r.cgs_pairs)
+        assert r.second_passes == r.gk_requeued  # each one forced by one requeue
     else:
-        assert r.second_passes == r.cgs_pairs
+        assert r.second_passes == r.cgs_pairs and r.gk_requeued == 0
     print(precision, second_pass, "restarts", r.restarts, "pairs", r.cgs_pairs, "second passes",
-          r.second_passes)
+          r.second_passes, "requeued", r.gk_requeued)
+
+
+@pytest.mark.parametrize("graphs", [-1, 1])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_selective_requeue_equals_both_passes(precision, graphs):
+    """The selective form queues every step with one pass and queues the
+    cycle again from the first step whose DGKS test asks for the second
+    pass, with both passes there. With eta = 1 the test asks at every step,
+    so every step is requeued once and the cycle ends with both passes at
+    every step: the solve must be bitwise the two-pass solve (also with
+    cycles replayed from captured HIP graphs, where the requeued steps run
+    as direct launches)."""
+    g, v, k = 4, 512, 4
+    A = W.regular_incidence(g, v, seed=5)
+    print(f"[case] requeue {precision} graphs={graphs}", flush=True)
+    base = dict(k=k, ncv=10, tol_rel=1e-12, tol_abs=0.0, prescreen_factor=1e-3,
+                max_restarts=400, graphs=graphs)
+    r1 = K.find_nullspace_cyclic_lanczos_ex(hip(A), K.HipVectorFactory(A.n, 42),
+                                            K.CyclicLanczosOptions(second_pass=1, **base),
+                                            precision)
+    r2 = K.find_nullspace_cyclic_lanczos_ex(
+        hip(A), K.HipVectorFactory(A.n, 42),
+        K.CyclicLanczosOptions(second_pass=2, second_pass_eta=1.0, **base), precision)
+    assert r1.right_null.shape[1] == k and r1.restarts == r2.restarts
+    assert np.array_equal(r1.right_null, r2.right_null)
+    # every step up to each cycle's stop was requeued once (steps queued
+    # after a stop are ignored and keep their single pass)
+    assert r2.cgs_pairs == r1.cgs_pairs == r1.second_passes
+    assert r2.second_passes == r2.gk_requeued > r2.cgs_pairs / 2
+    assert principal_sine(r2.right_null, W.component_indicators(g, v)) <= 1e-10
+    if graphs == 1:
+        assert r2.graph_replays > 0
 
 
 def test_small_2x3_analytic():
