@@ -219,7 +219,7 @@ class HipContext:
     def sync(self):
         _check(lib().kh_context_sync(self.h))
 
-    TRACE_CLASSES = {"vtw": 1, "vh_sub": 2, "spmv": 3, "gram": 4}
+    TRACE_CLASSES = {"vtw": 1, "vh_sub": 2, "spmv": 3, "gram": 4, "other": 5}
 
     def trace_begin(self, kernel_class):
         """Time every launch of a kernel class with HIP events (see kh_trace_begin)."""

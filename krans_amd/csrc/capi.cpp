@@ -853,7 +853,7 @@ int kh_gram_dev(kh_context *ctx, int precision, const void *Z, const void *AZ, i
 
 int kh_trace_begin(kh_context *ctx, int kernel_class) {
   return guard([&] {
-    need(ctx && kernel_class >= 0 && kernel_class <= 4, "bad argument");
+    need(ctx && kernel_class >= 0 && kernel_class <= 5, "bad argument");
     auto &c = *ctx->ctx;
     c.sync();
     c.trace_reset();
@@ -863,7 +863,7 @@ int kh_trace_begin(kh_context *ctx, int kernel_class) {
 
 int kh_trace_begin_mask(kh_context *ctx, unsigned mask) {
   return guard([&] {
-    need(ctx && (mask & ~0x1eu) == 0, "bad class mask (bits 1..4)");
+    need(ctx && (mask & ~0x3eu) == 0, "bad class mask (bits 1..5)");
     auto &c = *ctx->ctx;
     c.sync();
     c.trace_reset();

@@ -42,6 +42,8 @@ enum TraceClass : int {
   kTraceVhSub = 2,
   kTraceSpmv = 3,
   kTraceGram = 4,  // FP32-MFMA Gram tiles; the recorded amount is flops, not bytes
+  kTraceOther = 5,  // other launches with exact algorithmic bytes (FP64 V^T w,
+                    // normalised copies, window products)
 };
 struct TraceRec {
   double bytes;

@@ -2702,8 +2702,9 @@ static void launch_gemv_t(Context &ctx, const TV *V, int64_t ld, int s, const TW
       // class 1 is the FP32 reorthogonalisation product (k_vtw4_partial for
       // aligned blocks), the kernel the roofline reports
       constexpr bool kF32 = std::is_same<TV, float>::value && std::is_same<TW, float>::value;
-      TraceScope tr(ctx, kF32 ? kTraceVtW : kTraceNone,
-                    (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) + 8.0 * nb * sc,
+      TraceScope tr(ctx, kF32 ? kTraceVtW : kTraceOther,
+                    (double)len * sc * sizeof(TV) + (double)len * sizeof(TW) +
+                        8.0 * nb * (sc + sq),
                     len, sc, /*stamped=*/true);
       launch_vtw_partial(ctx, V + (int64_t)c0 * ld, ld, sc, w, len, part, nb, tr.start(),
                          tr.stop(), sq);
@@ -2804,6 +2805,7 @@ static void launch_gemv_n_sub(Context &ctx, const TV *V, int64_t ld, int s,
   // to 1024 partials (2^21 rows) on at most 2048 blocks (<= 16 MB of L2
   // reads); longer vectors take the norm and the copy as two launches
   if (nrm && dst && nb <= 1024) {
+    TraceScope tr(ctx, kTraceOther, 2.0 * len * sizeof(TW));  // w read, dst written
     hipLaunchKernelGGL(k_norm_scale_copy<TW>, dim3(std::min(grid_for(len), 2048)), dim3(kBlock),
                        0, ctx.stream, part, nb, nrm, len, (const TW *)w, dst);
   } else if (nrm) {
@@ -2857,8 +2859,9 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
                                 const double *g = nullptr) {
   if (g && nrm1)
     throw std::invalid_argument("gemv_n_sub_t: a selective pair takes no norm here");
-  // LDS-tiled form when its tile fits 64 KB and the columns fill it (below
-  // about 12 columns the two-sweep kernel is faster)
+  // LDS-tiled form when its tile fits the workgroup's LDS (160 KB on gfx950:
+  // FP64 tiles up to 128 columns) and the columns fill it (below about 12
+  // columns the two-sweep kernel is faster)
   const std::size_t lds = kLdsRows * 8 + (std::size_t)s * kLdsStride * sizeof(TV);
   const int64_t ntl = (len + kLdsRows - 1) / kLdsRows;
   constexpr bool kF32 = std::is_same<TV, float>::value;
@@ -2871,7 +2874,8 @@ static void launch_gemv_n_sub_t(Context &ctx, const TV *V, int64_t ld, int s, co
   // dword-load pipelined form needs many more registers). Both bound the
   // partials per column that k_finish sums.
   const int64_t nbl = std::min<int64_t>(ntl, vec ? 4096 : 16384);
-  if (s >= 12 && s <= 128 && lds <= 65536 && (std::size_t)nbl * (s + 1) <= ctx.part_capacity()) {
+  if (s >= 12 && s <= 128 && lds <= (std::size_t)ctx.lds_max &&
+      (std::size_t)nbl * (s + 1) <= ctx.part_capacity()) {
     double *part = ctx.partials((std::size_t)nbl * (s + 1));
     double *npart = nrm1 ? part + (std::size_t)nbl * s : nullptr;
     {
@@ -2995,6 +2999,9 @@ void gemm_nn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s,
              const double *H, int q, Prec po, void *OUT, int64_t ldo, int64_t len) {
   if (q <= 0)
     return;
+  TraceScope tr(ctx, kTraceOther,
+                (double)len * s * (pv == Prec::F64 ? 8 : 4) +
+                    (double)len * q * (po == Prec::F64 ? 8 : 4));  // V read, OUT written
   if (pv == Prec::F64 && po == Prec::F64)
     launch_gemm_nn(ctx, (const double *)V, ldv, s, H, q, (double *)OUT, ldo, len);
   else if (pv == Prec::F32 && po == Prec::F32)
@@ -3303,6 +3310,8 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
              len, H + (std::size_t)c * s);
     return;
   }
+  TraceScope tr(ctx, kTraceOther,
+                (double)len * (s * (pv == Prec::F64 ? 8 : 4) + q * (pw == Prec::F64 ? 8 : 4)));
   if (pv == Prec::F64 && pw == Prec::F64 && s <= kG2MaxQ && q <= kG2MaxQ)
     launch_gram2(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H, 1);
   else if (pv == Prec::F64 && pw == Prec::F64)

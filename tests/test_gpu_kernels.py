@@ -177,7 +177,8 @@ def test_spmm_rows_bit_exact(s, transpose):
 
 @pytest.mark.parametrize("s,n,prec", [(1, 5000, 0), (33, 100_003, 0), (97, 300_000, 1),
                                       (7, 2049, 1), (7, 4096, 1),
-                                      (20, 2**21 + 12_345, 1)])  # > 16384 tiles: pipelined
+                                      (20, 2**21 + 12_345, 1),  # > 16384 tiles: pipelined
+                                      (97, 300_000, 0), (128, 70_001, 0)])  # FP64 > 64 KB
 def test_cgs2_fused_equals_four_sweeps(ctx, s, n, prec):
     """The fused middle kernel (w -= V h1 and V^T w from one read of V) gives
     the results of the separate update and projection sweeps: the update of
