@@ -95,7 +95,9 @@ typedef struct kh_cyclic_result {
    * ran this request's Golub-Kahan cycles (0: solved on its own) */
   int64_t packed;
   /* mixed precision: projector CGs whose FP32 recurrence did not converge
-   * (stalled, cg_maxit or breakdown) and were solved again in FP64 */
+   * (stalled, cg_maxit or breakdown) and were solved again in FP64; after
+   * 4 batches in a row of which every column failed, the solve's later
+   * polishes run in FP64 directly and count here too */
   int64_t escalated;
   /* Gram-Schmidt pairs of the Golub-Kahan steps and how many of them ran
    * their second pass (all of them unless the second pass is selective);

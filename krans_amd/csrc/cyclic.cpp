@@ -83,6 +83,9 @@ constexpr int64_t kGraphAutoLen = 0;
 // column whose residual norm has not halved within this many iterations has
 // reached the FP32 rounding level and goes to FP64.
 constexpr int64_t kStall32 = 64;
+// FP32 polish batches in a row whose every column was solved again in FP64
+// after which the solve polishes in FP64 directly
+constexpr int kPolish32Misses = 4;
 
 
 }  // namespace
@@ -359,6 +362,18 @@ void CyclicDriver::polish_cg(int q, const double *rhs, double *w, Prec pc, const
     cg_solve(q, rhs, w, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
     return;
   }
+  // Where the FP32 recurrence keeps failing (every column of the last
+  // kPolish32Misses batches was solved again in FP64: residuals on the
+  // smallest singular values of an ill-conditioned A), the FP32 attempt
+  // only costs time: a failed column ends with the FP64 solve below from
+  // the same right-hand side, bitwise what this direct FP64 solve gives.
+  // The rest of the solve polishes in FP64 directly (the reference's
+  // polish), and its columns count as escalated.
+  if (polish32_off_) {
+    cg_solve(q, rhs, w, Prec::F64, diag, P_.cg_maxit, P_.cg_rel_tol, st);
+    S_.escalated += q;
+    return;
+  }
   // FP32 CG on the rounded FP64 residual; the FP64 residual of the updated
   // v decides whether the pass is kept
   float *W32 = Wm_.as<float>();
@@ -378,6 +393,9 @@ void CyclicDriver::polish_cg(int q, const double *rhs, double *w, Prec pc, const
   for (int a = 0; a < q; ++a)
     if (st[(std::size_t)a].reason >= 2)
       redo.push_back(a);
+  polish32_miss_ = (q > 0 && (int)redo.size() == q) ? polish32_miss_ + 1 : 0;
+  if (polish32_miss_ >= kPolish32Misses)
+    polish32_off_ = true;
   if (redo.empty())
     return;
   const int qr = (int)redo.size();

@@ -88,6 +88,10 @@ private:
   DBuf Wv32_, AV32_;      // FP32 screen of a window (mixed precision)
   bool screen_ = false;   // screen candidate windows in FP32 first
   int screen_miss_ = 0;   // consecutive windows the screen did not thin out
+  // mixed polish: consecutive FP32 CG batches none of whose columns
+  // converged; from kPolish32Misses on the solve polishes in FP64 directly
+  int polish32_miss_ = 0;
+  bool polish32_off_ = false;
   int wcap_ = 0;
   CgBatch cg_;
   CapturedWork gk_graph_;  // the queued Golub-Kahan cycle, keyed by the basis size
