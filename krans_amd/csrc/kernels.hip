@@ -1628,10 +1628,17 @@ __device__ inline int g2_ntiles(int mode, int s, int q) {
 // 128; the tall-skinny products of at most 64 columns use 96-row (FP32) or
 // 48-row (FP64) slabs, so every column is read 384 bytes at a time instead of
 // 128 (the short runs held those products near 1 TB/s).
-template <class T, int SLAB = kG2Slab, int MAXQ = kG2MaxQ>
+//
+// Z and AZ may be of a narrower type than the compute type T (FP32 input on
+// FP64 MFMA: every product exact, the sums in FP64). Each operand's slab is
+// staged in LDS in its own type (TLZ, TLA: FP32 operands as FP32, which
+// halves their LDS and so allows longer column runs) and converted to T as
+// the MFMA steps read it.
+template <class T, int SLAB = kG2Slab, int MAXQ = kG2MaxQ, class TZ = T, class TA = T,
+          class TLZ = T, class TLA = T>
 __global__ void __launch_bounds__(kG2Threads)
-    k_gram2_partial(int64_t len, int64_t rows_per_block, const T *__restrict__ Z,
-                    int64_t ldz, int s, const T *__restrict__ AZ, int64_t ldaz, int q,
+    k_gram2_partial(int64_t len, int64_t rows_per_block, const TZ *__restrict__ Z,
+                    int64_t ldz, int s, const TA *__restrict__ AZ, int64_t ldaz, int q,
                     int mode, T *__restrict__ part) {
   using V4 = typename Mfma16<T>::V;
   constexpr int LD = SLAB + 2;
@@ -1643,8 +1650,9 @@ __global__ void __launch_bounds__(kG2Threads)
   constexpr int PER_WAVE = (MAX_TILES + kG2Waves - 1) / kG2Waves;
   static_assert(SLAB % 4 == 0 && (SLAB * MAXQ) % kG2Threads == 0 && MAXQ % 16 == 0,
                 "slab shape");
-  __shared__ T Zs[MAXQ * LD];
-  __shared__ T As[MAXQ * LD];
+  static_assert(sizeof(TZ) <= sizeof(TLZ) && sizeof(TA) <= sizeof(TLA), "exact staging");
+  __shared__ TLZ Zs[MAXQ * LD];
+  __shared__ TLA As[MAXQ * LD];
   const int nt = (q + 15) / 16;
   const int SP = ((s + 15) / 16) * 16, QP = nt * 16;
   const int ntiles = g2_ntiles(mode, s, q);
@@ -1679,15 +1687,16 @@ __global__ void __launch_bounds__(kG2Threads)
   // staging: element e = tid + u*512 of the (SLAB rows x MAXQ cols) slab of Z
   // and of AZ: row e % SLAB, column e / SLAB
   constexpr int kLd = SLAB * MAXQ / kG2Threads;  // loads per operand and thread
-  T za[kLd], aa[kLd];
+  TLZ za[kLd];
+  TLA aa[kLd];
   auto load = [&](int64_t rb) {
 #pragma unroll
     for (int u = 0; u < kLd; ++u) {
       const int e = threadIdx.x + u * kG2Threads;
       const int r = e % SLAB, c = e / SLAB;
       const int64_t row = (rb + r) < r1 ? (rb + r) : (r1 - 1);
-      za[u] = c < s ? Z[row + (int64_t)c * ldz] : (T)0;
-      aa[u] = c < q ? AZ[row + (int64_t)c * ldaz] : (T)0;
+      za[u] = c < s ? (TLZ)Z[row + (int64_t)c * ldz] : (TLZ)0;
+      aa[u] = c < q ? (TLA)AZ[row + (int64_t)c * ldaz] : (TLA)0;
     }
   };
   if (r0 < r1)
@@ -1700,9 +1709,9 @@ __global__ void __launch_bounds__(kG2Threads)
       const int r = e % SLAB, c = e / SLAB;
       const bool inr = (rb + r) < r1;
       if (c < SP)
-        Zs[c * LD + r] = (inr && c < s) ? za[u] : (T)0;
+        Zs[c * LD + r] = (inr && c < s) ? za[u] : (TLZ)0;
       if (c < QP)
-        As[c * LD + r] = (inr && c < q) ? aa[u] : (T)0;
+        As[c * LD + r] = (inr && c < q) ? aa[u] : (TLA)0;
     }
     __syncthreads();
     if (rb + SLAB < r1)
@@ -1712,8 +1721,8 @@ __global__ void __launch_bounds__(kG2Threads)
 #pragma unroll
       for (int u = 0; u < PER_WAVE; ++u) {
         if (wid + u * kG2Waves < ntiles) {
-          const T a = Zs[aoff[u] + k];
-          const T b = bs[u] ? Zs[boff[u] + k] : As[boff[u] + k];
+          const T a = (T)Zs[aoff[u] + k];
+          const T b = bs[u] ? (T)Zs[boff[u] + k] : (T)As[boff[u] + k];
           acc[u] = Mfma16<T>::step(a, b, acc[u]);
         }
       }
@@ -1740,10 +1749,10 @@ __global__ void __launch_bounds__(kG2Threads)
 // k_gram2_partial with one tile (256 values per block), for k_tn2_finish.
 constexpr int kTnSlab = 256, kTnLd = kTnSlab + 1;
 
-template <class T>
+template <class T, class TW = T>
 __global__ void __launch_bounds__(kG2Threads)
     k_tn_narrow(int64_t len, int64_t rows_per_block, const T *__restrict__ V, int64_t ldv,
-                int s, const T *__restrict__ W, int64_t ldw, int q,
+                int s, const TW *__restrict__ W, int64_t ldw, int q,
                 double *__restrict__ part) {
   __shared__ double Vs[16 * kTnLd];
   __shared__ double Ws[16 * kTnLd];
@@ -1755,14 +1764,15 @@ __global__ void __launch_bounds__(kG2Threads)
   const int rl = threadIdx.x & (kTnSlab - 1), chi = threadIdx.x >> 8;
   // raw values stay in registers (in flight during the MFMA steps of the
   // current slab) and are converted on the LDS store
-  T va[8], wa[8];
+  T va[8];
+  TW wa[8];
   auto load = [&](int64_t rb) {
     const int64_t row = (rb + rl) < r1 ? (rb + rl) : (r1 - 1);
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int c = 2 * u + chi;
       va[u] = c < s ? V[row + (int64_t)c * ldv] : (T)0;
-      wa[u] = c < q ? W[row + (int64_t)c * ldw] : (T)0;
+      wa[u] = c < q ? W[row + (int64_t)c * ldw] : (TW)0;
     }
   };
   doublex4 acc = doublex4{0.0, 0.0, 0.0, 0.0};
@@ -3216,9 +3226,11 @@ void rows_to_cols(Context &ctx, Prec p, int64_t len, int s, const void *R, void 
 }
 
 // mode 0: SG = [Z^T Z | Z^T AZ] (s == q); mode 1: SG = Z^T AZ (s x q).
-template <class T>
-static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *AZ,
+// T is the MFMA type; Z and AZ (TZ, TA) are converted to it when staged.
+template <class T, class TZ = T, class TA = T>
+static void launch_gram2(Context &ctx, const TZ *Z, int64_t ldz, int s, const TA *AZ,
                          int64_t ldaz, int q, int64_t len, double *SG, int mode) {
+  static_assert(sizeof(TZ) <= sizeof(T) && sizeof(TA) <= sizeof(T), "exact conversion");
   if (q <= 0 || s <= 0 || len <= 0)
     return;
   if (q > kG2MaxQ || s > kG2MaxQ)
@@ -3231,22 +3243,31 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     rpb = std::max<int64_t>((rpb + kTnSlab - 1) / kTnSlab * kTnSlab, kTnSlab);
     const int64_t nblk = (len + rpb - 1) / rpb;
     double *part = ctx.partials((std::size_t)nblk * 256);
-    hipLaunchKernelGGL(k_tn_narrow<T>, dim3((unsigned)nblk), dim3(kG2Threads), 0, ctx.stream,
-                       len, rpb, Z, ldz, s, AZ, ldaz, q, part);
+    hipLaunchKernelGGL((k_tn_narrow<TZ, TA>), dim3((unsigned)nblk), dim3(kG2Threads), 0,
+                       ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, part);
     const int64_t outs = (int64_t)s * q * 64;  // one wave per output
     hipLaunchKernelGGL(k_tn2_finish<double>, dim3((unsigned)((outs + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, ctx.stream, part, (int)nblk, s, q, SG);
     KH_CHECK(hipGetLastError());
     return;
   }
-  // Tall-skinny products of at most 64 columns per operand read each column
-  // in 384-byte runs (96-row FP32 / 48-row FP64 slabs); the Gram pair and
-  // wider products keep 32-row slabs of up to 128 columns.
-  constexpr int kWideSlab = sizeof(T) == 4 ? 96 : 48;
+  // Column runs set the rate of these products (short runs at an 8 MB column
+  // stride held them near 1 TB/s). FP32 operands are staged as FP32 (also on
+  // FP64 MFMA), FP64 ones as FP64, in slabs as long as about 75 KB of LDS
+  // allows (two blocks per CU): tall-skinny products of at most 64 columns
+  // per operand take 96-row slabs unless both operands are FP64 (48 rows);
+  // the Gram pair and wider products (up to 128 columns) take 32-row slabs
+  // (64-row FP32 slabs measured slower for the Gram pair).
+  using TLZ = typename std::conditional<sizeof(TZ) == 4, float, T>::type;
+  using TLA = typename std::conditional<sizeof(TA) == 4, float, T>::type;
+  constexpr int kStage = (int)(sizeof(TLZ) + sizeof(TLA));  // bytes per row and column pair
+  constexpr int kWideSlab = kStage <= 12 ? 96 : 48;
+  constexpr int kFullSlab = kG2Slab;
   const bool wide = mode == 1 && s <= 64 && q <= 64;
-  const int slab = wide ? kWideSlab : kG2Slab;
-  // about 512 blocks, at most about 4096 rows (FP32 accumulation length)
-  // each, unless the partials would not fit the context's reduction buffer
+  const int slab = wide ? kWideSlab : kFullSlab;
+  // about 512 blocks (1024 measured no faster for the FP64 tall-skinny
+  // products), at most about 4096 rows (FP32 accumulation length) each,
+  // unless the partials would not fit the context's reduction buffer
   const int64_t maxblk = std::max<int64_t>(
       1, (int64_t)(ctx.part_capacity() * 8 / sizeof(T)) / ((int64_t)ntiles * 256));
   int64_t rpb = std::max<int64_t>((len + 511) / 512, 1);
@@ -3260,12 +3281,13 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
     TraceScope tr(ctx, (mode == 0 && sizeof(T) == 4) ? kTraceGram : kTraceNone,
                   2.0 * (double)len * s * q * (mode == 0 ? 2.0 : 1.0));
     if (wide)
-      hipLaunchKernelGGL((k_gram2_partial<T, kWideSlab, 64>), dim3((unsigned)nblk64),
-                         dim3(kG2Threads), 0, ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q,
-                         mode, part);
+      hipLaunchKernelGGL((k_gram2_partial<T, kWideSlab, 64, TZ, TA, TLZ, TLA>),
+                         dim3((unsigned)nblk64), dim3(kG2Threads), 0, ctx.stream, len, rpb, Z,
+                         ldz, s, AZ, ldaz, q, mode, part);
     else
-      hipLaunchKernelGGL(k_gram2_partial<T>, dim3((unsigned)nblk64), dim3(kG2Threads), 0,
-                         ctx.stream, len, rpb, Z, ldz, s, AZ, ldaz, q, mode, part);
+      hipLaunchKernelGGL((k_gram2_partial<T, kFullSlab, kG2MaxQ, TZ, TA, TLZ, TLA>),
+                         dim3((unsigned)nblk64), dim3(kG2Threads), 0, ctx.stream, len, rpb, Z,
+                         ldz, s, AZ, ldaz, q, mode, part);
   }
   if (mode == 0) {
     const int64_t outs = 2 * (int64_t)q * q * 64;  // one wave per output
@@ -3281,7 +3303,7 @@ static void launch_gram2(Context &ctx, const T *Z, int64_t ldz, int s, const T *
 
 void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int64_t ldaz, int q,
                 int64_t len, double *SG) {
-  launch_gram2(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
+  launch_gram2<float>(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
 }
 
 void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
@@ -3289,14 +3311,14 @@ void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const fl
   if (s <= 0 || q <= 0)
     return;
   if (q > 4 && s <= kG2MaxQ && q <= kG2MaxQ)
-    launch_gram2(ctx, V, ldv, s, W, ldw, q, len, H, 1);
+    launch_gram2<float>(ctx, V, ldv, s, W, ldw, q, len, H, 1);
   else
     gemm_tn(ctx, Prec::F32, V, ldv, s, Prec::F32, W, ldw, q, len, H);
 }
 
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
                 int q, int64_t len, double *SG) {
-  launch_gram2(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
+  launch_gram2<double>(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
 }
 
 void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
@@ -3312,9 +3334,18 @@ void gemm_tn(Context &ctx, Prec pv, const void *V, int64_t ldv, int s, Prec pw,
   }
   TraceScope tr(ctx, kTraceOther,
                 (double)len * (s * (pv == Prec::F64 ? 8 : 4) + q * (pw == Prec::F64 ? 8 : 4)));
-  if (pv == Prec::F64 && pw == Prec::F64 && s <= kG2MaxQ && q <= kG2MaxQ)
-    launch_gram2(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H, 1);
-  else if (pv == Prec::F64 && pw == Prec::F64)
+  // up to 128 columns a side on FP64 MFMA, FP32 operands converted exactly
+  // (every product exact, FP64 sums as on the VALU path below)
+  if (s <= kG2MaxQ && q <= kG2MaxQ) {
+    if (pv == Prec::F64 && pw == Prec::F64)
+      launch_gram2<double>(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H, 1);
+    else if (pv == Prec::F32 && pw == Prec::F32)
+      launch_gram2<double>(ctx, (const float *)V, ldv, s, (const float *)W, ldw, q, len, H, 1);
+    else if (pv == Prec::F64)
+      launch_gram2<double>(ctx, (const double *)V, ldv, s, (const float *)W, ldw, q, len, H, 1);
+    else
+      launch_gram2<double>(ctx, (const float *)V, ldv, s, (const double *)W, ldw, q, len, H, 1);
+  } else if (pv == Prec::F64 && pw == Prec::F64)
     launch_gemm_tn(ctx, (const double *)V, ldv, s, (const double *)W, ldw, q, len, H);
   else if (pv == Prec::F32 && pw == Prec::F32)
     launch_gemm_tn(ctx, (const float *)V, ldv, s, (const float *)W, ldw, q, len, H);

@@ -270,11 +270,15 @@ def test_cgs2_selective_pair(ctx, s, n, prec, cancel):
                                         (64, 64, 300_001, 0), (17, 33, 70_001, 0),
                                         (64, 5, 1_000_003, 0), (97, 33, 70_000, 0),
                                         (128, 128, 40_000, 0), (130, 9, 5000, 0),
-                                        (40, 24, 100_000, 1)])
+                                        (40, 24, 100_000, 1), (5, 7, 1000, 1),
+                                        (16, 16, 1_000_003, 1), (64, 64, 300_001, 1),
+                                        (97, 33, 70_000, 1), (128, 128, 40_000, 1),
+                                        (130, 9, 5000, 1)])
 def test_gemm_tn_blocks(ctx, s, q, n, prec):
     """V^T W of the block Gram-Schmidt steps: FP64 MFMA tiles for s, q <=
-    128 (FP64; s, q <= 16 through the narrow 256-row-slab kernel),
-    FP64-accumulated tiles otherwise; within 1e-13 of numpy."""
+    128 (FP32 operands converted exactly; s, q <= 16 through the narrow
+    256-row-slab kernel), FP64-accumulated VALU tiles otherwise; within
+    1e-13 of numpy in FP64 on the same (rounded) operands."""
     print(f"[case] gemm_tn s={s} q={q} n={n} prec={prec}", flush=True)
     rng = np.random.default_rng(s * 1000 + q)
     V = rng.standard_normal((n, s))
