@@ -806,8 +806,6 @@ int CyclicDriver::cycle_end(int64_t queued, int64_t nmv, int64_t nmtv) {
   S_.spmtv += nmtv;
   auto t0 = tcyc_;
   const Prec kp = kp_;
-  const std::size_t ke = esize(kp);
-  char *VBb = static_cast<char *>(vbp_);
   const int64_t ncv = ncv_;
   double *y0 = y0_.as<double>();
   // beta_j, and the DGKS tests of speculative single passes: the first step

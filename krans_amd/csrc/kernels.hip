@@ -1570,11 +1570,8 @@ __global__ void __launch_bounds__(kBlock)
 // one is consumed. Tile list: S tiles (ti <= tj) then G tiles, dealt to the
 // waves round-robin. Block partials (FP32, at most 4096 rows each) are summed
 // in FP64 in block order by k_gram2_finish.
-constexpr int kG2Threads = 512, kG2Slab = 32, kG2Ld = 34, kG2MaxQ = 128;
-constexpr int kG2MaxT = kG2MaxQ / 16;                                   // 8 column tiles
-constexpr int kG2MaxTiles = kG2MaxT * (kG2MaxT + 1) / 2 + kG2MaxT * kG2MaxT;  // 100
+constexpr int kG2Threads = 512, kG2Slab = 32, kG2MaxQ = 128;
 constexpr int kG2Waves = kG2Threads / 64;
-constexpr int kG2PerWave = (kG2MaxTiles + kG2Waves - 1) / kG2Waves;     // 13
 
 __device__ inline void g2_tile(int t, int T, int &ti, int &tj, bool &is_s) {
   const int ns = T * (T + 1) / 2;
