@@ -294,12 +294,6 @@ void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int6
 // FP64 blocks: the same pass on FP64 MFMA (v_mfma_f64_16x16x4f64).
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
                 int q, int64_t len, double *SG);
-// H (s x q, FP64 out, device) = V^T W for FP32 V, W with s, q <= 128 on the
-// matrix cores (s, q <= 16: FP64 MFMA on the exactly converted values;
-// otherwise FP32 MFMA accumulation within 32-row slabs, FP64 across slabs,
-// about FP32 accuracy), for screening; other shapes go through gemm_tn.
-void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
-                      int64_t ldw, int q, int64_t len, double *H);
 void gram_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W, int64_t ldw,
                int q, int64_t len, double *H);
 // W(:,c) -= V H(:,c) for c < q; optional norms[c] = ||W(:,c)|| afterwards.

@@ -3303,16 +3303,6 @@ void gram2_mfma(Context &ctx, const float *Z, int64_t ldz, const float *AZ, int6
   launch_gram2<float>(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
 }
 
-void gemm_tn_f32_mfma(Context &ctx, const float *V, int64_t ldv, int s, const float *W,
-                      int64_t ldw, int q, int64_t len, double *H) {
-  if (s <= 0 || q <= 0)
-    return;
-  if (q > 4 && s <= kG2MaxQ && q <= kG2MaxQ)
-    launch_gram2<float>(ctx, V, ldv, s, W, ldw, q, len, H, 1);
-  else
-    gemm_tn(ctx, Prec::F32, V, ldv, s, Prec::F32, W, ldw, q, len, H);
-}
-
 void gram2_mfma(Context &ctx, const double *Z, int64_t ldz, const double *AZ, int64_t ldaz,
                 int q, int64_t len, double *SG) {
   launch_gram2<double>(ctx, Z, ldz, q, AZ, ldaz, q, len, SG, 0);
